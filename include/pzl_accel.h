@@ -1,0 +1,126 @@
+/*
+ * pzl_accel.h -- C ABI of the MI355X jerk-search engine (libaccelsearch_lib.so).
+ *
+ * This is the thin layer between host C and the HIP kernels.  It replaces the
+ * reference's extern "C" device launchers (accel_kernels.cu:59-71:
+ * do_fft_batch, fuse_add_search_batch, sort_search_results,
+ * init_constant_device, clear_cache, gen_kernmatrix_cu) and the host helpers
+ * that fed them (accel_utils_cu.c:378-612, 857-929).  The drop-in accel.h
+ * API (accelsearch_CPU1/GPU/CPU2, subharm_fderivs_vol, ...) is declared in
+ * include/accel.h and is implemented on top of these entry points.
+ *
+ * Plain C types only: pointers, sizes, and the structs below.  Every entry
+ * point returns 0 on success, PZL_ERR_OVERFLOW (1) when the hit buffer
+ * overflows (the reference's too_large path, accelsearch_lib.c:748-782), or
+ * a negative code; pzl_last_error() describes the last failure of the
+ * calling thread.  A plan is not safe for concurrent use by two threads;
+ * different plans are independent.
+ */
+#ifndef PZL_ACCEL_H
+#define PZL_ACCEL_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PZL_ERR_OVERFLOW 1
+#define PZL_ERR_ARG (-1)
+#define PZL_ERR_HIP (-2)
+#define PZL_ERR_UNSUPPORTED (-3)
+#define PZL_ERR_NOSPECTRUM (-4)
+
+/* Search request: the fields of Cmdline (accelsearch_cmd.h:11-87) and
+ * infodata (makeinf.h:23-56) that create_accelobs (accel_utils.c:1636-2037)
+ * reads.  flo/fhi/rlo/rhi follow the *P flags exactly as the reference. */
+typedef struct PzlConfig {
+    double N;            /* samples in the time series (.inf N) */
+    double dt;           /* sample time in s (.inf dt) */
+    long long numbins;   /* complex bins in the .fft file */
+    long long lobin;     /* -lobin */
+    int zmax, wmax;      /* -zmax, -wmax as given */
+    int wmaxP;           /* -wmax given (0: acceleration-only search) */
+    int numharm;         /* 1, 2, 4, 8, 16, 32 */
+    float sigma;         /* -sigma */
+    int floP, fhiP, rloP, rhiP;
+    double flo, fhi, rlo, rhi;
+    double nph;          /* > 0: photon normalisation (-photon) */
+    int batchsize;       /* rsteps per launch group (-batchsize), >= 1 */
+    long long maxhits;   /* hit-buffer capacity, 0 = default */
+} PzlConfig;
+
+/* Search geometry (the accelobs fields, accel.h:37-82). */
+typedef struct PzlGeometry {
+    long long N, numbins, lobin, highestbin, nrsteps;
+    int maxkernlen, corr_uselen, fftlen, numharmstages, numz, numw, numharm_kernels;
+    double dt, T, rlo, rhi, zlo, zhi, wlo, whi;
+    float sigma;
+    float powcut[6];
+    long long numindep[6];
+    long long bank_bytes, plane_bytes_per_rstep, device_bytes;
+} PzlGeometry;
+
+/* One harmonic's shape (subharminfo, accel.h:111-122). */
+typedef struct PzlHarmInfo {
+    int numharm, harmnum, stage, zmax, wmax, numz, numw, fftlen, halfwidth, max_numrs;
+} PzlHarmInfo;
+
+/* Candidate (the accelcand fields the search fills, accel.h:84-97). */
+typedef struct PzlCand {
+    float power, sigma;
+    int numharm;
+    double r, z, w;
+} PzlCand;
+
+/* Timing of the last search (milliseconds; HIP events on the search stream). */
+typedef struct PzlStats {
+    double ms_wall;        /* host wall clock of pzl_search */
+    double ms_gpu;         /* first launch to last kernel end */
+    double ms_segments, ms_corr, ms_sum;  /* summed kernel durations */
+    double ms_corr_max_class;             /* the FFT length with the most time */
+    double ms_host;        /* sort, sigma, candidate insertion */
+    long long corr_launches, nhits, nrsteps;
+    double cells;          /* nrsteps * corr_uselen * numz * max(numw,1) */
+    double bytes_alg;      /* algorithmic HBM bytes of the search */
+    double bytes_alg_corr; /* ... of the correlation kernels alone */
+} PzlStats;
+
+const char *pzl_last_error(void);
+int pzl_device_count(int *count);
+
+/* Geometry and the kernel bank on 'device'; the bank stays in HBM until
+ * pzl_plan_destroy.  accel_utils.c:1636-2037 + create_subharminfos :236-282 */
+int pzl_plan_create(const PzlConfig *cfg, int device, void **plan);
+void pzl_plan_destroy(void *plan);
+int pzl_plan_geometry(void *plan, PzlGeometry *g);
+int pzl_plan_harm(void *plan, int idx, PzlHarmInfo *h);
+/* Search shape only, without touching the GPU. */
+int pzl_geometry_from_config(const PzlConfig *cfg, PzlGeometry *g);
+
+/* The spectrum: numbins complex64 values, realfft packing (read_fft.c:4-47).
+ * _device takes a caller-owned device pointer that must stay valid. */
+int pzl_set_spectrum(void *plan, const float *host_fft, long long numbins);
+int pzl_set_spectrum_device(void *plan, const void *dev_fft, long long numbins);
+
+/* Search rsteps [rstep_lo, rstep_hi) (rstep_hi < 0: to the end) and insert
+ * the hits into a candidate list with insert_new_accelcand semantics
+ * (accel_utils.c:382-480).  Writes min(ncands, maxcands) candidates. */
+int pzl_search(void *plan, long long rstep_lo, long long rstep_hi, PzlCand *cands, long long maxcands,
+               long long *ncands, PzlStats *stats);
+
+/* Hits of the last pzl_search in PRESTO C scan order: key encodes
+ * (((rstep*stages + stage)*numw + w)*numz + z)*corr_uselen + r. */
+int pzl_get_hits(void *plan, long long *keys, float *powers, long long maxhits, long long *nhits);
+
+/* Parity hooks: one FFT'd kernel (fftlen complex64) of harmonic idx, and the
+ * power plane ([numw][numz][numrs]) of (rstep, harmonic) as
+ * subharm_fderivs_vol (accel_utils.c:1094-1293) returns it. */
+int pzl_get_kernel(void *plan, int idx, int row, float *out);
+int pzl_compute_plane(void *plan, long long rstep, int idx, float *out, long long maxfloats, int *numrs,
+                      long long *rlo);
+/* Fundamental plane of 'rstep' after all harmonic stages were added. */
+int pzl_compute_sums(void *plan, long long rstep, float *out, long long maxfloats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

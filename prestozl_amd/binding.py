@@ -1,0 +1,229 @@
+"""ctypes binding of the C ABI in include/pzl_accel.h (libaccelsearch_lib.so).
+
+The library is built in-tree by prestozl_amd.build (hipcc, gfx950).  Loading
+fails loudly when it is missing: there is no CPU fallback for the search.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "lib", "libaccelsearch_lib.so")
+
+PZL_ERR_OVERFLOW = 1
+
+
+class Config(C.Structure):
+    _fields_ = [("N", C.c_double), ("dt", C.c_double), ("numbins", C.c_longlong),
+                ("lobin", C.c_longlong), ("zmax", C.c_int), ("wmax", C.c_int),
+                ("wmaxP", C.c_int), ("numharm", C.c_int), ("sigma", C.c_float),
+                ("floP", C.c_int), ("fhiP", C.c_int), ("rloP", C.c_int), ("rhiP", C.c_int),
+                ("flo", C.c_double), ("fhi", C.c_double), ("rlo", C.c_double), ("rhi", C.c_double),
+                ("nph", C.c_double), ("batchsize", C.c_int), ("maxhits", C.c_longlong)]
+
+
+class Geometry(C.Structure):
+    _fields_ = [("N", C.c_longlong), ("numbins", C.c_longlong), ("lobin", C.c_longlong),
+                ("highestbin", C.c_longlong), ("nrsteps", C.c_longlong),
+                ("maxkernlen", C.c_int), ("corr_uselen", C.c_int), ("fftlen", C.c_int),
+                ("numharmstages", C.c_int), ("numz", C.c_int), ("numw", C.c_int),
+                ("numharm_kernels", C.c_int),
+                ("dt", C.c_double), ("T", C.c_double), ("rlo", C.c_double), ("rhi", C.c_double),
+                ("zlo", C.c_double), ("zhi", C.c_double), ("wlo", C.c_double), ("whi", C.c_double),
+                ("sigma", C.c_float), ("powcut", C.c_float * 6), ("numindep", C.c_longlong * 6),
+                ("bank_bytes", C.c_longlong), ("plane_bytes_per_rstep", C.c_longlong),
+                ("device_bytes", C.c_longlong)]
+
+
+class HarmInfo(C.Structure):
+    _fields_ = [("numharm", C.c_int), ("harmnum", C.c_int), ("stage", C.c_int), ("zmax", C.c_int),
+                ("wmax", C.c_int), ("numz", C.c_int), ("numw", C.c_int), ("fftlen", C.c_int),
+                ("halfwidth", C.c_int), ("max_numrs", C.c_int)]
+
+
+class Cand(C.Structure):
+    _fields_ = [("power", C.c_float), ("sigma", C.c_float), ("numharm", C.c_int),
+                ("r", C.c_double), ("z", C.c_double), ("w", C.c_double)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("ms_wall", C.c_double), ("ms_gpu", C.c_double), ("ms_segments", C.c_double),
+                ("ms_corr", C.c_double), ("ms_sum", C.c_double), ("ms_corr_max_class", C.c_double),
+                ("ms_host", C.c_double), ("corr_launches", C.c_longlong), ("nhits", C.c_longlong),
+                ("nrsteps", C.c_longlong), ("cells", C.c_double), ("bytes_alg", C.c_double),
+                ("bytes_alg_corr", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+class PzlError(RuntimeError):
+    def __init__(self, rc, msg):
+        super().__init__("pzl error %d: %s" % (rc, msg))
+        self.rc = rc
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise RuntimeError("libaccelsearch_lib.so is not built (run __graft_entry__.build())")
+        L = C.CDLL(LIBPATH)
+        P = C.POINTER
+        L.pzl_last_error.restype = C.c_char_p
+        L.pzl_device_count.argtypes = [P(C.c_int)]
+        L.pzl_geometry_from_config.argtypes = [P(Config), P(Geometry)]
+        L.pzl_plan_create.argtypes = [P(Config), C.c_int, P(C.c_void_p)]
+        L.pzl_plan_destroy.argtypes = [C.c_void_p]
+        L.pzl_plan_destroy.restype = None
+        L.pzl_plan_geometry.argtypes = [C.c_void_p, P(Geometry)]
+        L.pzl_plan_harm.argtypes = [C.c_void_p, C.c_int, P(HarmInfo)]
+        L.pzl_set_spectrum.argtypes = [C.c_void_p, P(C.c_float), C.c_longlong]
+        L.pzl_set_spectrum_device.argtypes = [C.c_void_p, C.c_void_p, C.c_longlong]
+        L.pzl_search.argtypes = [C.c_void_p, C.c_longlong, C.c_longlong, P(Cand), C.c_longlong,
+                                 P(C.c_longlong), P(Stats)]
+        L.pzl_get_hits.argtypes = [C.c_void_p, P(C.c_longlong), P(C.c_float), C.c_longlong, P(C.c_longlong)]
+        L.pzl_get_kernel.argtypes = [C.c_void_p, C.c_int, C.c_int, P(C.c_float)]
+        L.pzl_compute_plane.argtypes = [C.c_void_p, C.c_longlong, C.c_int, P(C.c_float), C.c_longlong,
+                                        P(C.c_int), P(C.c_longlong)]
+        L.pzl_compute_sums.argtypes = [C.c_void_p, C.c_longlong, P(C.c_float), C.c_longlong]
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        raise PzlError(rc, lib().pzl_last_error().decode())
+    return rc
+
+
+def make_config(N, dt, numbins, zmax, wmax, numharm, sigma, wmaxP=True, flo=1.0, fhi=10000.0, rlo=None,
+                rhi=None, lobin=0, nph=0.0, batchsize=8, maxhits=0):
+    c = Config()
+    c.N, c.dt, c.numbins, c.lobin = float(N), float(dt), int(numbins), int(lobin)
+    c.zmax, c.wmax, c.wmaxP, c.numharm, c.sigma = int(zmax), int(wmax), int(bool(wmaxP)), int(numharm), float(sigma)
+    if rlo is not None:
+        c.floP, c.rloP, c.rlo = 0, 1, float(rlo)
+    else:
+        c.floP, c.flo = 1, float(flo)
+    if rhi is not None:
+        c.fhiP, c.rhiP, c.rhi = 0, 1, float(rhi)
+    else:
+        c.fhiP, c.fhi = 1, float(fhi)
+    c.nph, c.batchsize, c.maxhits = float(nph), int(batchsize), int(maxhits)
+    return c
+
+
+def geometry(cfg):
+    g = Geometry()
+    check(lib().pzl_geometry_from_config(C.byref(cfg), C.byref(g)))
+    return g
+
+
+def _fptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class Plan:
+    """A search plan: geometry + the kernel bank resident on one GPU."""
+
+    def __init__(self, cfg, device=0):
+        self.cfg = cfg
+        self.h = C.c_void_p()
+        check(lib().pzl_plan_create(C.byref(cfg), device, C.byref(self.h)))
+        self.geom = Geometry()
+        check(lib().pzl_plan_geometry(self.h, C.byref(self.geom)))
+        self.harms = []
+        i = 0
+        while True:
+            hi = HarmInfo()
+            if lib().pzl_plan_harm(self.h, i, C.byref(hi)) != 0:
+                break
+            self.harms.append(hi)
+            i += 1
+
+    def close(self):
+        if self.h:
+            lib().pzl_plan_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_spectrum(self, fft):
+        """fft: complex64 array (numbins) or float32 interleaved."""
+        a = np.ascontiguousarray(fft)
+        if a.dtype == np.complex64:
+            a = a.view(np.float32)
+        assert a.dtype == np.float32
+        check(lib().pzl_set_spectrum(self.h, _fptr(a), len(a) // 2))
+
+    def set_spectrum_device(self, ptr, numbins):
+        check(lib().pzl_set_spectrum_device(self.h, C.c_void_p(ptr), numbins))
+
+    def search(self, rstep_lo=0, rstep_hi=-1, maxcands=1 << 16, stats=True):
+        cands = (Cand * maxcands)()
+        n = C.c_longlong(0)
+        st = Stats() if stats else None
+        check(lib().pzl_search(self.h, rstep_lo, rstep_hi, cands, maxcands, C.byref(n),
+                               C.byref(st) if st is not None else None))
+        out = [(cands[i].power, cands[i].sigma, cands[i].numharm, cands[i].r, cands[i].z, cands[i].w)
+               for i in range(min(n.value, maxcands))]
+        return out, (st.as_dict() if st is not None else None)
+
+    def hits(self):
+        n = C.c_longlong(0)
+        check(lib().pzl_get_hits(self.h, None, None, 0, C.byref(n)))
+        keys = np.zeros(max(n.value, 1), dtype=np.int64)
+        pows = np.zeros(max(n.value, 1), dtype=np.float32)
+        check(lib().pzl_get_hits(self.h, keys.ctypes.data_as(C.POINTER(C.c_longlong)), _fptr(pows), n.value,
+                                 C.byref(n)))
+        return self.decode_keys(keys[: n.value]), pows[: n.value]
+
+    def decode_keys(self, keys):
+        g = self.geom
+        R, Z, W, S = g.corr_uselen, self.harms[0].numz, self.harms[0].numw, g.numharmstages
+        r = keys % R
+        k = keys // R
+        z = k % Z
+        k //= Z
+        w = k % W
+        k //= W
+        stage = k % S
+        rstep = k // S
+        return np.stack([rstep, stage, w, z, r], axis=1)
+
+    def kernel(self, idx, row):
+        h = self.harms[idx]
+        out = np.zeros(2 * h.fftlen, dtype=np.float32)
+        check(lib().pzl_get_kernel(self.h, idx, row, _fptr(out)))
+        return out[0::2] + 1j * out[1::2]
+
+    def plane(self, rstep, idx):
+        h = self.harms[idx]
+        cap = h.numw * h.numz * (h.max_numrs + 4)
+        out = np.zeros(cap, dtype=np.float32)
+        numrs = C.c_int(0)
+        rlo = C.c_longlong(0)
+        check(lib().pzl_compute_plane(self.h, rstep, idx, _fptr(out), cap, C.byref(numrs), C.byref(rlo)))
+        return out[: h.numw * h.numz * numrs.value].reshape(h.numw, h.numz, numrs.value), rlo.value
+
+    def sums(self, rstep):
+        h = self.harms[0]
+        n = h.numw * h.numz * self.geom.corr_uselen
+        out = np.zeros(n, dtype=np.float32)
+        check(lib().pzl_compute_sums(self.h, rstep, _fptr(out), n))
+        return out.reshape(h.numw, h.numz, self.geom.corr_uselen)

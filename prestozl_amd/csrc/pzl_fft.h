@@ -1,0 +1,169 @@
+// pzl_fft.h -- small-radix DFT butterflies and a block-cooperative LDS FFT
+// for gfx950.  Conventions: X[k] = sum_j x[j] exp(SIGN * 2 pi i j k / n),
+// unnormalised (SIGN = -1 forward, +1 inverse), as FFTW's fftwf_plan_dft_1d
+// which the reference calls (fftcalls.c:71-135, accel_utils.c:1222-1269).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+template <typename T> struct cpx_t;
+template <> struct cpx_t<float> { typedef float2 type; };
+template <> struct cpx_t<double> { typedef double2 type; };
+
+template <typename C> __device__ __forceinline__ C cadd(C a, C b) { return C{a.x + b.x, a.y + b.y}; }
+template <typename C> __device__ __forceinline__ C csub(C a, C b) { return C{a.x - b.x, a.y - b.y}; }
+template <typename C> __device__ __forceinline__ C cmul(C a, C b)
+{
+    return C{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+// multiply by SIGN * i
+template <int SIGN, typename C> __device__ __forceinline__ C cmul_si(C a)
+{
+    return SIGN > 0 ? C{-a.y, a.x} : C{a.y, -a.x};
+}
+
+// In-register DFTs of size R on v[0..R-1], outputs in natural order.
+template <int SIGN, typename C> __device__ __forceinline__ void dft2(C &a, C &b)
+{
+    C t = a;
+    a = cadd(t, b);
+    b = csub(t, b);
+}
+
+template <int SIGN, typename C> __device__ __forceinline__ void dft4(C *v)
+{
+    C s0 = cadd(v[0], v[2]), d0 = csub(v[0], v[2]);
+    C s1 = cadd(v[1], v[3]), d1 = cmul_si<SIGN>(csub(v[1], v[3]));
+    v[0] = cadd(s0, s1);
+    v[2] = csub(s0, s1);
+    v[1] = cadd(d0, d1);
+    v[3] = csub(d0, d1);
+}
+
+template <int SIGN, typename C> __device__ __forceinline__ void dft8(C *v)
+{
+    typedef decltype(v[0].x) R;
+    const R h = (R)0.70710678118654752440;
+    C e[4] = {v[0], v[2], v[4], v[6]};
+    C o[4] = {v[1], v[3], v[5], v[7]};
+    dft4<SIGN>(e);
+    dft4<SIGN>(o);
+    // o[k] *= exp(SIGN 2 pi i k / 8)
+    // k=1: (1 + SIGN i) h ; k=2: SIGN i ; k=3: (-1 + SIGN i) h
+    C o1 = o[1], o3 = o[3];
+    o[1] = C{(o1.x - (R)SIGN * o1.y) * h, ((R)SIGN * o1.x + o1.y) * h};
+    o[2] = cmul_si<SIGN>(o[2]);
+    o[3] = C{(-o3.x - (R)SIGN * o3.y) * h, ((R)SIGN * o3.x - o3.y) * h};
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        v[k] = cadd(e[k], o[k]);
+        v[k + 4] = csub(e[k], o[k]);
+    }
+}
+
+template <int SIGN, typename C> __device__ __forceinline__ void dft3(C *v)
+{
+    typedef decltype(v[0].x) R;
+    const R c1 = (R)-0.5, s1 = (R)SIGN * (R)0.86602540378443864676;
+    C t1 = cadd(v[1], v[2]), t2 = csub(v[1], v[2]);
+    C m = C{v[0].x + c1 * t1.x, v[0].y + c1 * t1.y};
+    C j = C{-s1 * t2.y, s1 * t2.x};  // i * s1 * t2
+    v[0] = cadd(v[0], t1);
+    v[1] = cadd(m, j);
+    v[2] = csub(m, j);
+}
+
+template <int SIGN, typename C> __device__ __forceinline__ void dft5(C *v)
+{
+    typedef decltype(v[0].x) R;
+    const R c1 = (R)0.30901699437494742410, c2 = (R)-0.80901699437494742410;
+    const R s1 = (R)SIGN * (R)0.95105651629515357212, s2 = (R)SIGN * (R)0.58778525229247312917;
+    C a1 = cadd(v[1], v[4]), b1 = csub(v[1], v[4]);
+    C a2 = cadd(v[2], v[3]), b2 = csub(v[2], v[3]);
+    C x0 = v[0];
+    C m1 = C{x0.x + c1 * a1.x + c2 * a2.x, x0.y + c1 * a1.y + c2 * a2.y};
+    C m2 = C{x0.x + c2 * a1.x + c1 * a2.x, x0.y + c2 * a1.y + c1 * a2.y};
+    // n1 = i (s1 b1 + s2 b2), n2 = i (s2 b1 - s1 b2)
+    C q1 = C{s1 * b1.x + s2 * b2.x, s1 * b1.y + s2 * b2.y};
+    C q2 = C{s2 * b1.x - s1 * b2.x, s2 * b1.y - s1 * b2.y};
+    C n1 = C{-q1.y, q1.x}, n2 = C{-q2.y, q2.x};
+    v[0] = C{x0.x + a1.x + a2.x, x0.y + a1.y + a2.y};
+    v[1] = cadd(m1, n1);
+    v[4] = csub(m1, n1);
+    v[2] = cadd(m2, n2);
+    v[3] = csub(m2, n2);
+}
+
+template <int SIGN, typename C> __device__ __forceinline__ void dft6(C *v)
+{
+    // 6 = 2 x 3 (Good-Thomas free: 3 then 2 with twiddles)
+    C a[3] = {v[0], v[2], v[4]}, b[3] = {v[1], v[3], v[5]};
+    dft3<SIGN>(a);
+    dft3<SIGN>(b);
+    typedef decltype(v[0].x) R;
+    const R h = (R)0.5, s = (R)SIGN * (R)0.86602540378443864676;
+    // b[k] *= exp(SIGN 2 pi i k / 6)
+    C b1 = b[1], b2 = b[2];
+    b[1] = C{b1.x * h - b1.y * s, b1.x * s + b1.y * h};
+    b[2] = C{-b2.x * h - b2.y * s, b2.x * s - b2.y * h};
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        v[k] = cadd(a[k], b[k]);
+        v[k + 3] = csub(a[k], b[k]);
+    }
+}
+
+template <int R, int SIGN, typename C> __device__ __forceinline__ void dftR(C *v)
+{
+    if constexpr (R == 2) dft2<SIGN>(v[0], v[1]);
+    else if constexpr (R == 3) dft3<SIGN>(v);
+    else if constexpr (R == 4) dft4<SIGN>(v);
+    else if constexpr (R == 5) dft5<SIGN>(v);
+    else if constexpr (R == 6) dft6<SIGN>(v);
+    else if constexpr (R == 8) dft8<SIGN>(v);
+}
+
+// Block-cooperative double-precision Stockham FFT of n points held in LDS
+// (buf, n <= 4096), any n = 2^a 3^b 5^c.  Each pass reads every butterfly's
+// inputs into registers, synchronises, then writes in place.  Twiddles come
+// from sincospi in double.  Used off the hot path (kernel bank, segments).
+template <int SIGN>
+__device__ void lds_fft_f64(double2 *buf, int n, int tid, int nt)
+{
+    int Ns = 1;
+    constexpr int MAXB = 8;  // butterflies per thread per pass
+    while (Ns < n) {
+        const int rem = n / Ns;
+        const int R = (rem % 4 == 0) ? 4 : (rem % 2 == 0) ? 2 : (rem % 3 == 0) ? 3 : 5;
+        const int nbf = n / R;
+        double2 v[MAXB][5];
+        int cnt = 0;
+        for (int j = tid; j < nbf && cnt < MAXB; j += nt, cnt++) {
+            const int jm = j % Ns;
+            for (int r = 0; r < R; r++) {
+                double2 x = buf[j + r * nbf];
+                if (Ns > 1 && r > 0) {
+                    double s, c;
+                    // exp(SIGN 2 pi i jm r / (Ns R))
+                    sincospi((double)SIGN * 2.0 * (double)(jm * r) / (double)(Ns * R), &s, &c);
+                    x = cmul(x, double2{c, s});
+                }
+                v[cnt][r] = x;
+            }
+        }
+        __syncthreads();
+        cnt = 0;
+        for (int j = tid; j < nbf && cnt < MAXB; j += nt, cnt++) {
+            double2 *w = v[cnt];
+            if (R == 4) dft4<SIGN>(w);
+            else if (R == 2) dft2<SIGN>(w[0], w[1]);
+            else if (R == 3) dft3<SIGN>(w);
+            else dft5<SIGN>(w);
+            const int jm = j % Ns;
+            const int idxD = (j / Ns) * Ns * R + jm;
+            for (int r = 0; r < R; r++) buf[idxD + r * Ns] = w[r];
+        }
+        __syncthreads();
+        Ns *= R;
+    }
+}

@@ -1,0 +1,200 @@
+"""HIP path vs the oracle (PRESTO C restated) on the same synthetic inputs.
+
+Tolerances (FP32 search, as the reference):
+  * kernel bank: |K_gpu - K_ref| <= 2e-6 * max|K_ref| per kernel row;
+  * powers and summed powers: |P_gpu - P_ref| / max(P_ref, 1) <= 1e-5, the
+    floor 1 being the normalised noise power (median / ln 2 normalisation);
+  * hits (rstep, stage, w, z, r): identical sets except cells whose power lies
+    within 1e-5 (relative) of the stage threshold; candidates (r, z, w,
+    numharm) identical except those within 1e-4 in sigma of the threshold.
+"""
+import numpy as np
+import pytest
+
+import pyoracle as O
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+TOL_POW = 1e-5
+TOL_KERN = 2e-6
+
+
+def _plan(cfg_kwargs, fft):
+    from prestozl_amd import binding as B
+    cfg = B.make_config(**cfg_kwargs)
+    plan = B.Plan(cfg)
+    plan.set_spectrum(fft)
+    return plan
+
+
+def _oracle(cfg_kwargs, mode, nthreads=16):
+    kw = dict(cfg_kwargs)
+    kw.pop("batchsize", None)
+    kw.pop("maxhits", None)
+    obs = O.create_accelobs(O.make_params(**kw))
+    return obs, O.Ctx(obs, mode=mode, nthreads=nthreads)
+
+
+def _c1(config1_data, **over):
+    fft, N, dt, opts = config1_data
+    kw = dict(N=N, dt=dt, numbins=len(fft), zmax=opts["zmax"], wmax=opts["wmax"], numharm=opts["numharm"],
+              sigma=opts["sigma"], wmaxP=True, batchsize=8)
+    kw.update(over)
+    return fft, kw
+
+
+def _c2_small():
+    """zmax 200 / wmax 300 / 16 harmonics at a reduced length (same per-rstep shapes)."""
+    from prestozl_amd import synth
+    fft, N, dt, opts, sigs = synth.config2(N=1 << 21)
+    kw = dict(N=N, dt=dt, numbins=len(fft), zmax=200, wmax=300, numharm=16, sigma=5.0, wmaxP=True, batchsize=4)
+    return fft, kw
+
+
+def test_geometry_matches_oracle(config1_data):
+    fft, kw = _c1(config1_data)
+    plan = _plan(kw, fft)
+    obs, _ = _oracle(kw, O.FP32, 1)
+    g = plan.geom
+    assert (g.fftlen, g.corr_uselen, g.numz, g.numw) == (obs.fftlen, obs.corr_uselen, obs.numz, obs.numw)
+    assert g.nrsteps == O.num_rsteps(obs)
+    assert list(g.powcut)[: g.numharmstages] == list(obs.powcut)[: obs.numharmstages]
+    plan.close()
+
+
+@pytest.mark.parametrize("which", ["c1", "c2"])
+def test_kernel_bank(config1_data, which):
+    fft, kw = _c1(config1_data) if which == "c1" else _c2_small()
+    plan = _plan(kw, fft)
+    obs = O.create_accelobs(O.make_params(**{k: v for k, v in kw.items() if k not in ("batchsize", "maxhits")}))
+    worst32 = worst64 = 0.0
+    for idx, (nh, hn) in enumerate(O.harmonics(obs)):
+        h = plan.harms[idx]
+        assert (h.numharm, h.harmnum) == (nh, hn)
+        rows = sorted({0, h.numw * h.numz - 1, (h.numw // 2) * h.numz + h.numz // 2, h.numz // 3})
+        for row in rows:
+            wi, zi = divmod(row, h.numz)
+            kg = plan.kernel(idx, row)
+            k64 = O.kernel(obs, nh, hn, zi, wi, O.FP64)
+            k32 = O.kernel(obs, nh, hn, zi, wi, O.FP32)
+            scale = np.abs(k64).max()
+            worst64 = max(worst64, np.abs(kg - k64).max() / scale)
+            worst32 = max(worst32, np.abs(kg - k32).max() / scale)
+    print("kernel bank max err vs fp64 %.3g, vs fp32 %.3g" % (worst64, worst32))
+    assert worst64 < TOL_KERN
+    assert worst32 < 2 * TOL_KERN
+    plan.close()
+
+
+@pytest.mark.parametrize("which", ["c1", "c2"])
+def test_planes(config1_data, which):
+    fft, kw = _c1(config1_data) if which == "c1" else _c2_small()
+    plan = _plan(kw, fft)
+    obs, ctx32 = _oracle(kw, O.FP32)
+    _, ctx64 = _oracle(kw, O.FP64)
+    nr = plan.geom.nrsteps
+    rstep_bins = plan.geom.corr_uselen // 2
+    hl = O.harmonics(obs)
+    worst32 = worst64 = 0.0
+    for rstep in sorted({0, 1, nr // 2, nr - 1}):
+        startr = obs.rlo + rstep * rstep_bins
+        for idx in range(len(hl)) if which == "c1" else (0, 1, 3, 8, 15):
+            nh, hn = hl[idx]
+            pg, rlo = plan.plane(rstep, idx)
+            p32, _, _, meta = ctx32.subharm(fft, nh, hn, startr, startr + rstep_bins - 0.5)
+            p64, _, _, _ = ctx64.subharm(fft, nh, hn, startr, startr + rstep_bins - 0.5)
+            assert rlo == meta[0]
+            assert pg.shape == p32.shape
+            e32, _ = rel_err(pg, p32, floor=1.0)
+            e64, _ = rel_err(pg, p64, floor=1.0)
+            worst32, worst64 = max(worst32, e32), max(worst64, e64)
+    print("plane max rel err vs fp32 %.3g vs fp64 %.3g" % (worst32, worst64))
+    assert worst32 < TOL_POW
+    assert worst64 < TOL_POW
+    plan.close()
+
+
+@pytest.mark.parametrize("which", ["c1", "c2"])
+def test_summed_powers(config1_data, which):
+    fft, kw = _c1(config1_data) if which == "c1" else _c2_small()
+    plan = _plan(kw, fft)
+    obs, ctx32 = _oracle(kw, O.FP32)
+    nr = plan.geom.nrsteps
+    for rstep in (0, nr // 3):
+        sg = plan.sums(rstep)
+        _, _, s32 = ctx32.search(fft, rstep, rstep + 1, sum_rstep=rstep)
+        e, rms = rel_err(sg, s32, floor=1.0)
+        print("rstep %d summed powers max rel err %.3g rms %.3g" % (rstep, e, rms))
+        assert e < TOL_POW
+    plan.close()
+
+
+def _compare_hits(plan, hits_gpu, pows_gpu, ohits, powcut):
+    g = {tuple(int(x) for x in h): float(p) for h, p in zip(hits_gpu, pows_gpu)}
+    o = {(h[0], h[1], h[2], h[3], h[4]): h[5] for h in ohits}
+    only_g = set(g) - set(o)
+    only_o = set(o) - set(g)
+    near = 0
+    for k in only_g:
+        assert abs(g[k] - powcut[k[1]]) <= 1e-5 * powcut[k[1]], ("extra hit", k, g[k], powcut[k[1]])
+        near += 1
+    for k in only_o:
+        assert abs(o[k] - powcut[k[1]]) <= 1e-5 * powcut[k[1]], ("missing hit", k, o[k], powcut[k[1]])
+        near += 1
+    worst = 0.0
+    for k in set(g) & set(o):
+        worst = max(worst, abs(g[k] - o[k]) / max(o[k], 1.0))
+    return near, worst, len(set(g) & set(o))
+
+
+def test_search_config1(config1_data):
+    """Full config-1 search (N=2^20, zmax 20, wmax 0, numharm 4, sigma 2)."""
+    fft, kw = _c1(config1_data)
+    plan = _plan(kw, fft)
+    obs, ctx32 = _oracle(kw, O.FP32)
+    cg, st = plan.search()
+    hg, pg = plan.hits()
+    co, ho, _ = ctx32.search(fft)
+    near, worst, common = _compare_hits(plan, hg, pg, ho, list(obs.powcut))
+    print("config1: %d gpu hits, %d oracle hits, %d near-threshold diffs, hit power err %.3g; %d/%d cands"
+          % (len(hg), len(ho), near, worst, len(cg), len(co)))
+    assert worst < TOL_POW
+    assert common > 0
+    key = lambda c: (round(c[3], 6), round(c[4], 6), round(c[5], 6), c[2])
+    if near == 0:
+        assert [key(c) for c in cg] == [key(c) for c in co]
+        for a, b in zip(cg, co):
+            assert abs(a[0] - b[0]) <= TOL_POW * max(b[0], 1.0)
+            assert abs(a[1] - b[1]) <= 1e-4 * max(b[1], 1.0)
+    plan.close()
+
+
+def test_search_config2_band():
+    """Config-2 geometry (zmax 200, wmax 300, 16 harmonics) on a band of rsteps."""
+    fft, kw = _c2_small()
+    kw["sigma"] = 3.0  # more hits in a short band
+    plan = _plan(kw, fft)
+    obs, ctx32 = _oracle(kw, O.FP32)
+    lo, hi = 60, 63  # holds the 210.7 Hz injection (r ~ 44190)
+    cg, st = plan.search(lo, hi)
+    hg, pg = plan.hits()
+    co, ho, _ = ctx32.search(fft, lo, hi)
+    near, worst, common = _compare_hits(plan, hg, pg, ho, list(obs.powcut))
+    print("config2 band: %d gpu hits, %d oracle hits, %d near, err %.3g, cands %d/%d"
+          % (len(hg), len(ho), near, worst, len(cg), len(co)))
+    assert worst < TOL_POW
+    plan.close()
+
+
+def test_batch_size_invariance(config1_data):
+    """Candidate list is independent of rsteps per launch (determinism)."""
+    fft, kw = _c1(config1_data)
+    res = []
+    for bs in (1, 3, 8):
+        kw["batchsize"] = bs
+        plan = _plan(kw, fft)
+        c, _ = plan.search(0, 60)
+        res.append(c)
+        plan.close()
+    assert res[0] == res[1] == res[2]
