@@ -1,0 +1,156 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X jerk search (one GPU).
+
+Workload (default): N = 2^24 float32 samples at dt = 1e-4 s, Gaussian noise
+plus three jerked sinusoids, real-FFT'd to 2^23 complex64 bins; search
+zmax = 200, wmax = 300, numharm = 16, sigma = 5 (11,680 rsteps).  One step is
+one complete search of that spectrum; the kernel bank is built once per plan
+and the spectrum is resident in HBM before timing starts.
+
+Prints one JSON line: cells/s (nrsteps * corr_uselen * numz * numw per
+second), the roofline of the dominant kernel (the 2048-point correlation),
+p50/p99 per-search latency, and the oracle timed on a bounded sample of the
+same workload on the host cores.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def hip_sync():
+    try:
+        ctypes.CDLL("libamdhip64.so").hipDeviceSynchronize()
+    except OSError:
+        pass
+
+
+def cpu_baseline(fft, kw, rsteps, nthreads, cells_per_rstep, start):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+    obs = O.create_accelobs(O.make_params(**kw))
+    ctx = O.Ctx(obs, O.FP32, nthreads=nthreads)
+    t0 = time.perf_counter()
+    ctx.search(fft, start, start + rsteps, maxhits=1 << 16)
+    dt = time.perf_counter() - t0
+    ctx.close()
+    return {"value": cells_per_rstep * rsteps / dt, "unit": "cells/s", "cores": nthreads, "kind": "port",
+            "sample": "oracle FP32 (PRESTO C restated, OpenMP over kernel rows) on rsteps [%d, %d) of the same "
+                      "spectrum (%.1f s), kernel-bank build excluded as on the GPU" % (start, start + rsteps, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batchsize", type=int, default=8)
+    ap.add_argument("--log2n", type=int, default=24)
+    ap.add_argument("--cpu-rsteps", type=int, default=24)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--plain", action="store_true", help="also time one-rstep-per-launch (plain GPU form)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (each rstep and each file is "
+              "independent; see DESIGN.md for how the search shards).", file=sys.stderr)
+        sys.exit(2)
+
+    from prestozl_amd import binding as B
+    from prestozl_amd import synth
+
+    fft, N, dt, opts, sigs = synth.config2(N=1 << args.log2n)
+    kw = dict(N=N, dt=dt, numbins=len(fft), zmax=opts["zmax"], wmax=opts["wmax"], numharm=opts["numharm"],
+              sigma=opts["sigma"], wmaxP=True)
+    plan = B.Plan(B.make_config(batchsize=args.batchsize, **kw))
+    plan.set_spectrum(fft)
+    g = plan.geom
+    for _ in range(args.warmup):
+        plan.search(stats=False)
+    lat, stats_all = [], []
+    hip_sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        cands, st = plan.search(stats=True)
+        lat.append((time.perf_counter() - ts) * 1e3)
+        stats_all.append(st)
+    hip_sync()
+    total = time.perf_counter() - t0
+    cells = stats_all[0]["cells"]
+    value = cells * args.steps / total
+    # dominant kernel: the correlation class with the most time
+    cls = {}
+    for st in stats_all:
+        for c in st["classes"]:
+            d = cls.setdefault(c["fftlen"], dict(ms=0.0, launches=0, bytes=0.0))
+            d["ms"] += c["ms"]
+            d["launches"] += c["launches"]
+            d["bytes"] += c["bytes"]
+    dom_len = max(cls, key=lambda k: cls[k]["ms"])
+    dom = cls[dom_len]
+    avg_ms = dom["ms"] / dom["launches"]
+    bytes_per_launch = dom["bytes"] / dom["launches"]
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    out = {
+        "metric": "f-fdot-fdotdot cells/s at zmax200/wmax300/16 harm; p50/p99 latency per search",
+        "value": value,
+        "unit": "cells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": total * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: Gaussian noise + 3 jerked sinusoids (seeded), realfft-packed complex64",
+        "config": {"workload": "plain GPU drop-in at the production shape: N=2^%d, dt=1e-4 s, zmax=200, "
+                               "wmax=300, numharm=16, sigma=5, batchsize=%d" % (args.log2n, args.batchsize),
+                   "nrsteps": g.nrsteps, "corr_uselen": g.corr_uselen, "numz": g.numz, "numw": g.numw,
+                   "cells_per_search": cells, "bank_MiB": g.bank_bytes / 2 ** 20,
+                   "planes_MiB_per_rstep": g.plane_bytes_per_rstep / 2 ** 20},
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                       "samples": len(lat)},
+        "roofline": {"bound": "hbm", "kernel": "k_corr<%d>" % dom_len, "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "avg_launch_ms": avg_ms, "alg_bytes_per_launch": bytes_per_launch},
+        "search_roofline": {"bytes_alg_per_search": stats_all[0]["bytes_alg"],
+                            "achieved_GBs": stats_all[0]["bytes_alg"] * args.steps / total / 1e9,
+                            "frac": stats_all[0]["bytes_alg"] * args.steps / total / 1e9 / HBM_PEAK_GBS},
+        "kernel_ms_per_search": {"segments": np.mean([s["ms_segments"] for s in stats_all]),
+                                 "corr": np.mean([s["ms_corr"] for s in stats_all]),
+                                 "harmsum": np.mean([s["ms_sum"] for s in stats_all]),
+                                 "host": np.mean([s["ms_host"] for s in stats_all]),
+                                 "by_fftlen": {str(k): v["ms"] / args.steps for k, v in sorted(cls.items())}},
+        "candidates": len(cands),
+        "hits": stats_all[0]["nhits"],
+    }
+    if args.plain:
+        plan1 = B.Plan(B.make_config(batchsize=1, **kw))
+        plan1.set_spectrum(fft)
+        ts = time.perf_counter()
+        c1, _ = plan1.search(stats=False)
+        out["plain_gpu"] = {"value": cells / (time.perf_counter() - ts), "unit": "cells/s", "batchsize": 1,
+                            "same_candidates": c1 == cands}
+        plan1.close()
+    plan.close()
+    if not args.no_cpu:
+        nthreads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline(fft, kw, args.cpu_rsteps, nthreads,
+                                           g.corr_uselen * g.numz * max(g.numw, 1), g.nrsteps // 2)
+        out["cpu_baseline"]["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

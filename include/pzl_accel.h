@@ -82,6 +82,11 @@ typedef struct PzlStats {
     double cells;          /* nrsteps * corr_uselen * numz * max(numw,1) */
     double bytes_alg;      /* algorithmic HBM bytes of the search */
     double bytes_alg_corr; /* ... of the correlation kernels alone */
+    int nclass;            /* correlation launches are grouped by FFT length */
+    int class_fftlen[16];
+    double class_ms[16];          /* summed duration of that class's launches */
+    long long class_launches[16];
+    double class_bytes[16];       /* algorithmic bytes of that class's launches */
 } PzlStats;
 
 const char *pzl_last_error(void);
@@ -119,6 +124,12 @@ int pzl_compute_plane(void *plan, long long rstep, int idx, float *out, long lon
                       long long *rlo);
 /* Fundamental plane of 'rstep' after all harmonic stages were added. */
 int pzl_compute_sums(void *plan, long long rstep, float *out, long long maxfloats);
+
+/* The candidate-list logic alone (host only): apply n hits in order with
+ * insert_new_accelcand semantics (accel_utils.c:382-480). */
+int pzl_insert_cands(const double *rr, const float *sigma, const float *power, const int *numharm,
+                     const double *z, const double *w, long long n, PzlCand *out, long long maxout,
+                     long long *nout);
 
 #ifdef __cplusplus
 }

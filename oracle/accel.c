@@ -497,6 +497,26 @@ static node *insert_new_accelcand(node *list, float power, float sigma, int numh
     return list;
 }
 
+/* The list logic alone: apply n hits in order (for pinning the product's
+ * host-side candidate list against the literal linked-list walk). */
+long long oracle_insert_cands(const double *rr, const float *sigma, const float *power, const int *numharm,
+                              const double *z, const double *w, long long n, oracle_cand *out, long long maxout)
+{
+    node *list = NULL, *nd;
+    long long i, cnt = 0;
+    int added;
+    for (i = 0; i < n; i++)
+        list = insert_new_accelcand(list, power[i], sigma[i], numharm[i], rr[i], z[i], w[i], &added);
+    for (nd = list; nd;) {
+        node *nx = nd->next;
+        if (cnt < maxout) out[cnt] = nd->c;
+        cnt++;
+        free(nd);
+        nd = nx;
+    }
+    return cnt;
+}
+
 typedef struct search_state {
     node *cands;
     oracle_hit *hits;

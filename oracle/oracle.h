@@ -141,6 +141,10 @@ long long oracle_search(oracle_ctx *ctx, const float *fft, long long rstep_lo, l
 
 long long oracle_num_rsteps(const oracle_obs *obs);
 
+/* insert_new_accelcand (accel_utils.c:382-480) applied to n hits in order. */
+long long oracle_insert_cands(const double *rr, const float *sigma, const float *power, const int *numharm,
+                              const double *z, const double *w, long long n, oracle_cand *out, long long maxout);
+
 #ifdef __cplusplus
 }
 #endif

@@ -100,6 +100,9 @@ def lib():
         L.oracle_search.restype = C.c_longlong
         L.oracle_num_rsteps.argtypes = [P(Obs)]
         L.oracle_num_rsteps.restype = C.c_longlong
+        L.oracle_insert_cands.argtypes = [P(C.c_double), P(C.c_float), P(C.c_float), P(C.c_int),
+                                          P(C.c_double), P(C.c_double), C.c_longlong, P(Cand), C.c_longlong]
+        L.oracle_insert_cands.restype = C.c_longlong
         _lib = L
     return _lib
 
@@ -267,3 +270,19 @@ def w_resp_halfwidth(z, w):
 
 def z_resp_halfwidth(z):
     return lib().oracle_z_resp_halfwidth(z)
+
+
+def insert_cands(rr, sigma, power, numharm, z, w):
+    """insert_new_accelcand over hits in order -> [(power, sigma, numharm, r, z, w)]."""
+    n = len(rr)
+    rr = np.ascontiguousarray(rr, dtype=np.float64)
+    sigma = np.ascontiguousarray(sigma, dtype=np.float32)
+    power = np.ascontiguousarray(power, dtype=np.float32)
+    numharm = np.ascontiguousarray(numharm, dtype=np.int32)
+    z = np.ascontiguousarray(z, dtype=np.float64)
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    cap = max(n, 1)
+    out = (Cand * cap)()
+    k = lib().oracle_insert_cands(dptr(rr), fptr(sigma), fptr(power), numharm.ctypes.data_as(C.POINTER(C.c_int)),
+                                  dptr(z), dptr(w), n, out, cap)
+    return [(out[i].power, out[i].sigma, out[i].numharm, out[i].r, out[i].z, out[i].w) for i in range(k)]

@@ -52,10 +52,15 @@ class Stats(C.Structure):
                 ("ms_corr", C.c_double), ("ms_sum", C.c_double), ("ms_corr_max_class", C.c_double),
                 ("ms_host", C.c_double), ("corr_launches", C.c_longlong), ("nhits", C.c_longlong),
                 ("nrsteps", C.c_longlong), ("cells", C.c_double), ("bytes_alg", C.c_double),
-                ("bytes_alg_corr", C.c_double)]
+                ("bytes_alg_corr", C.c_double), ("nclass", C.c_int), ("class_fftlen", C.c_int * 16),
+                ("class_ms", C.c_double * 16), ("class_launches", C.c_longlong * 16),
+                ("class_bytes", C.c_double * 16)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        d = {k: getattr(self, k) for k, _ in self._fields_ if not k.startswith("class_")}
+        d["classes"] = [dict(fftlen=self.class_fftlen[i], ms=self.class_ms[i], launches=self.class_launches[i],
+                             bytes=self.class_bytes[i]) for i in range(self.nclass)]
+        return d
 
 
 _lib = None

@@ -2,21 +2,22 @@
 // include/pzl_accel.h).
 //
 // Replaces the reference's GPU driver and host helpers:
-//   geometry and thresholds     create_accelobs, accel_utils.c:1636-2037
-//   harmonic shapes / bank      create_subharminfos :236-282, init_kernel :151-168,
+//   kernel bank                 create_subharminfos accel_utils.c:236-282 and
 //                               fkern_host_to_dev accel_utils_cu.c:378-429
-//   rstep enumeration, batches  accelsearch_GPU, accelsearch_lib.c:473-901
-//   hit drain + insertion       accelsearch_lib.c:784-866, insert_to_cands
-//                               accel_utils_cu.c:857-929, insert_new_accelcand
-//                               accel_utils.c:382-480
-// Everything per rstep runs on the GPU: segment preparation, correlation,
-// harmonic summing and compaction.  The host only orders the (rare) hits,
-// evaluates their significance in double and maintains the candidate list.
+//                               (built on the GPU here, once per plan)
+//   rstep batches               accelsearch_GPU, accelsearch_lib.c:473-901
+//   hit drain + ordering        accelsearch_lib.c:784-866, thrust sort
+//                               accel_kernels.cu:990-1006
+//   candidate insertion         insert_to_cands accel_utils_cu.c:857-929 with
+//                               insert_new_accelcand accel_utils.c:382-480
+// Per rstep everything runs on the GPU: segment preparation, correlation,
+// harmonic summing, compaction, ordering and the significance of each hit.
+// The host applies the ordered hits to the candidate list while the GPU is
+// already working on the next batch of rsteps.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <chrono>
-#include <cstdarg>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -25,321 +26,51 @@
 #include <vector>
 
 #include "pzl_accel.h"
+#include "pzl_candlist.h"
 #include "pzl_kernels.h"
 #include "pzl_math.h"
+#include "pzl_plan.h"
 #include "pzl_stats.h"
+
+using namespace pzl;
 
 namespace {
 
-thread_local std::string g_err;
-
-int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
-int fail(int code, const char *fmt, ...)
-{
-    char buf[512];
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(buf, sizeof buf, fmt, ap);
-    va_end(ap);
-    g_err = buf;
-    return code;
-}
-
-#define HIPCHK(call)                                                                                   \
-    do {                                                                                               \
-        hipError_t e_ = (call);                                                                        \
-        if (e_ != hipSuccess) return fail(PZL_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
-                                          __FILE__, __LINE__);                                         \
-    } while (0)
-
-const long long kDefaultMaxHits = 1LL << 22;
-
-struct SizeClass {
-    int fftlen;
-    std::vector<int> harms;
-    int *d_harms = nullptr;
-};
-
-struct Plan {
-    PzlConfig cfg{};
-    PzlGeometry g{};
-    std::vector<PzlHarm> harms;
-    std::vector<double> startr;
-    std::vector<SizeClass> classes;
-    std::vector<uint16_t> zinds;  // [nh][numz]
-    std::vector<int> winds;       // [nh][numw]
-    int fnumw = 1, fnumz = 1, fzlo = 0, fwlo = 0;
-    int max_fftlen = 0;
-    int B = 8;
-    int device = 0;
-    hipStream_t st = nullptr;
-    // device state
-    PzlHarm *d_harms = nullptr;
-    float2 *d_bank = nullptr;
-    double *d_startr = nullptr;
-    uint16_t *d_zinds = nullptr;
-    int *d_winds = nullptr;
-    const float2 *d_fft = nullptr;
-    float2 *d_fft_own = nullptr;
-    long long fft_bins = 0;
-    float2 *d_segs = nullptr;
-    long long seg_stride = 0;
-    uint16_t *d_rinds = nullptr;
-    PzlHrTab *d_hr = nullptr;
-    float *d_planes = nullptr;
-    long long plane_stride = 0;
-    PzlHit *d_hits = nullptr;
-    unsigned long long *d_nhits = nullptr;  // [0] hit count, [1] index-map violations
-    unsigned long long maxhits = 0;
-    float *d_sums = nullptr;
-    std::vector<PzlHit> last_hits;
-};
-
-// ---------------------------------------------------------------------------
-// Geometry (create_accelobs, accel_utils.c:1636-2037; calc_fftlen :137-149)
-// ---------------------------------------------------------------------------
-
-int twon_to_index(int n)
-{
-    int x = 0;
-    while (n > 1) {
-        n >>= 1;
-        x++;
-    }
-    return x;
-}
-
-int calc_fftlen(int numharm, int harmnum, int max_zfull, int max_wfull, int corr_uselen)
-{
-    const double hf = (double)harmnum / (double)numharm;
-    const int bins_needed = (int)ceil(corr_uselen * hf) + 2;
-    const int end_effects = 2 * PZL_NUMBETWEEN *
-                            pzl_w_resp_halfwidth(pzl_calc_required_z(hf, max_zfull), pzl_calc_required_w(hf, max_wfull));
-    return pzl_next_good_fftlen(bins_needed + end_effects);
-}
-
-int compute_geometry(const PzlConfig &c, PzlGeometry &g, std::vector<PzlHarm> *harms_out,
-                     std::vector<double> *startr_out)
-{
-    memset(&g, 0, sizeof g);
-    int zmax = c.zmax, wmax = c.wmax;
-    if (zmax < 0 || wmax < 0) return fail(PZL_ERR_ARG, "zmax and wmax must be >= 0");
-    if (zmax % PZL_DZ) zmax = (zmax / PZL_DZ + 1) * PZL_DZ;
-    const int nh_ok = c.numharm;
-    if (nh_ok != 1 && nh_ok != 2 && nh_ok != 4 && nh_ok != 8 && nh_ok != 16 && nh_ok != 32)
-        return fail(PZL_ERR_ARG, "'numharm' = %d must be a power-of-two (1..32)", c.numharm);
-    if (c.numbins < 2 || c.N <= 0 || c.dt <= 0) return fail(PZL_ERR_ARG, "bad N/dt/numbins");
-    g.N = (long long)c.N;
-    g.numbins = c.numbins;
-    g.lobin = c.lobin;
-    g.numharmstages = twon_to_index(c.numharm) + 1;
-    g.numz = (zmax / PZL_DZ) * 2 + 1;
-    if (c.wmaxP) {
-        if (wmax % PZL_DW) wmax = (wmax / PZL_DW + 1) * PZL_DW;
-        g.whi = wmax;
-        g.wlo = -wmax;
-        g.numw = (wmax / PZL_DW) * 2 + 1;
-        if (wmax == 0) g.numw = 0;
-    } else {
-        g.whi = g.wlo = 0.0;
-        g.numw = 0;
-        wmax = 0;
-    }
-    g.dt = c.dt;
-    g.T = c.dt * c.N;
-    if (c.floP) {
-        g.rlo = floor(c.flo * g.T);
-    } else {
-        g.rlo = c.rloP ? c.rlo : 1.0;
-    }
-    if (g.rlo < g.lobin) g.rlo = (double)g.lobin;
-    if (g.rlo > g.numbins - 1) return fail(PZL_ERR_ARG, "low frequency to search is above the highest bin");
-    g.highestbin = g.numbins - 1;
-    g.rhi = (double)g.highestbin;
-    if (c.fhiP || c.rhiP) {
-        g.highestbin = c.fhiP ? (long long)ceil(c.fhi * g.T) : (long long)c.rhi;
-        if (g.highestbin > g.numbins - 1) g.highestbin = g.numbins - 1;
-        g.rhi = (double)g.highestbin;
-        if (g.highestbin < g.rlo) return fail(PZL_ERR_ARG, "high frequency to search is below the low one");
-    }
-    g.zhi = zmax;
-    g.zlo = -zmax;
-    g.sigma = c.sigma;
-    for (int ii = 0; ii < g.numharmstages; ii++) {
-        const double dz = PZL_DZ, dw = PZL_DW;
-        if (g.numz == 1 && g.numw == 0)
-            g.numindep[ii] = (long long)((g.rhi - g.rlo) / (1 << ii));
-        else if (g.numz > 1 && g.numw == 0)
-            g.numindep[ii] = (long long)((g.rhi - g.rlo) * (g.numz + 1) * (dz / 6.95) / (1 << ii));
-        else
-            g.numindep[ii] =
-                (long long)((g.rhi - g.rlo) * (g.numz + 1) * (dz / 6.95) * (g.numw + 1) * (dw / 44.2) / (1 << ii));
-        g.powcut[ii] = (float)pzl_power_for_sigma(g.sigma, 1 << ii, (double)g.numindep[ii]);
-    }
-    g.maxkernlen = 2 * PZL_NUMBETWEEN * pzl_w_resp_halfwidth(g.zhi, g.whi);
-    g.fftlen = pzl_fftlen_from_kernwidth(g.maxkernlen);
-    if (g.fftlen < 2048) g.fftlen = 2048;
-    g.corr_uselen = g.fftlen - g.maxkernlen;
-    if (g.corr_uselen % PZL_RDR) g.corr_uselen = g.corr_uselen / PZL_RDR * PZL_RDR;
-    if (g.corr_uselen <= 0 || g.corr_uselen > 65535) return fail(PZL_ERR_UNSUPPORTED, "corr_uselen out of range");
-
-    // rsteps (accelsearch_lib.c:569-593 / accelsearch.c:183-188)
-    std::vector<double> startr;
-    const int rstep = (int)(g.corr_uselen * PZL_DR);
-    for (double s = g.rlo; s + rstep < g.highestbin; s += rstep) startr.push_back(s);
-    g.nrsteps = (long long)startr.size();
-
-    // harmonics: fundamental, then stage order (accelsearch.c:199-211)
-    std::vector<PzlHarm> harms;
-    long long bank_off = 0, plane_off = 0, seg_off = 0;
-    for (int stage = 0; stage < g.numharmstages; stage++) {
-        const int numharm = 1 << stage;
-        for (int harm = 1; harm < (stage ? numharm : 2); harm += 2) {
-            PzlHarm H{};
-            H.numharm = numharm;
-            H.harmnum = harm;
-            H.stage = stage;
-            H.hf = (double)harm / (double)numharm;
-            H.zmax = pzl_calc_required_z(H.hf, (int)g.zhi);
-            H.wmax = pzl_calc_required_w(H.hf, (int)g.whi);
-            H.numz = (H.zmax / PZL_DZ) * 2 + 1;
-            H.numw = (H.wmax / PZL_DW) * 2 + 1;
-            H.nrows = H.numz * H.numw;
-            H.zlo = pzl_calc_required_z(H.hf, g.zlo);
-            H.wlo = pzl_calc_required_w(H.hf, g.wlo);
-            H.fftlen = (stage == 0) ? g.fftlen : calc_fftlen(numharm, harm, (int)g.zhi, (int)g.whi, g.corr_uselen);
-            H.halfwidth = pzl_w_resp_halfwidth((double)-H.zmax, (double)-H.wmax);
-            int maxrs = 0;
-            for (double s : startr) {
-                const PzlHarmR hr = pzl_harm_r(H.hf, stage == 0, s, s + rstep - PZL_DR, g.corr_uselen);
-                maxrs = std::max(maxrs, hr.numrs);
-            }
-            if (startr.empty()) maxrs = g.corr_uselen;
-            H.max_numrs = maxrs;
-            H.rs_stride = (maxrs + 31) / 32 * 32;
-            H.bank_off = bank_off;
-            H.plane_off = plane_off;
-            H.seg_off = seg_off;
-            bank_off += (long long)H.nrows * H.fftlen;
-            plane_off += (long long)H.nrows * H.rs_stride;
-            seg_off += H.fftlen / 2;
-            harms.push_back(H);
-        }
-    }
-    g.numharm_kernels = 0;
-    for (auto &H : harms) g.numharm_kernels += H.nrows;
-    g.bank_bytes = bank_off * 8;
-    g.plane_bytes_per_rstep = plane_off * 4;
-    if (harms_out) *harms_out = harms;
-    if (startr_out) *startr_out = startr;
-    return 0;
-}
-
-// ---------------------------------------------------------------------------
-// Candidate list with insert_new_accelcand semantics (accel_utils.c:382-480)
-// ---------------------------------------------------------------------------
-
-class CandList {
-  public:
-    void insert(float power, float sigma, int numharm, double rr, double zz, double ww)
-    {
-        const PzlCand nc{power, sigma, numharm, rr, zz, ww};
-        const double CLOSE = 15.0;
-        if (v_.empty()) {
-            v_.push_back(nc);
-            return;
-        }
-        // the reference walks from the head to the first node with r >= rr
-        // (or the last node); on a non-decreasing list that is lower_bound.
-        size_t tmp;
-        if (inversions_ == 0) {
-            tmp = std::lower_bound(v_.begin(), v_.end(), rr, [](const PzlCand &a, double x) { return a.r < x; }) -
-                  v_.begin();
-        } else {
-            tmp = 0;
-            while (tmp + 1 < v_.size() && v_[tmp].r < rr) tmp++;
-        }
-        if (tmp == v_.size()) tmp = v_.size() - 1;
-        const bool has_prev = tmp > 0;
-        const size_t prev = tmp - 1;
-        const double next_diff = fabs(rr - v_[tmp].r);
-        const double prev_diff = has_prev ? fabs(rr - v_[prev].r) : CLOSE + 1.0;
-        if (prev_diff < CLOSE) {
-            bool added = false;
-            if (v_[prev].sigma < sigma) {
-                set(prev, nc);
-                added = true;
-            }
-            if (next_diff < CLOSE && v_[tmp].sigma < sigma) {
-                if (added) erase(tmp);
-                else set(tmp, nc);
-            }
-        } else if (next_diff < CLOSE) {
-            if (v_[tmp].sigma < sigma) set(tmp, nc);
-        } else {
-            if (tmp + 1 == v_.size() && v_[tmp].r < rr - CLOSE) insert_at(v_.size(), nc);
-            else insert_at(tmp, nc);  // before tmp (or at the head)
-        }
-    }
-    const std::vector<PzlCand> &items() const { return v_; }
-
-  private:
-    int inv(size_t i) const { return (i + 1 < v_.size() && v_[i].r > v_[i + 1].r) ? 1 : 0; }
-    void touch_before(size_t i)
-    {
-        if (i > 0) inversions_ -= inv(i - 1);
-        inversions_ -= inv(i);
-    }
-    void touch_after(size_t i)
-    {
-        if (i > 0) inversions_ += inv(i - 1);
-        if (i < v_.size()) inversions_ += inv(i);
-    }
-    void set(size_t i, const PzlCand &c)
-    {
-        touch_before(i);
-        v_[i] = c;
-        touch_after(i);
-    }
-    void erase(size_t i)
-    {
-        touch_before(i);
-        v_.erase(v_.begin() + i);
-        if (i > 0) inversions_ += inv(i - 1);
-    }
-    void insert_at(size_t i, const PzlCand &c)
-    {
-        if (i > 0 && i < v_.size()) inversions_ -= inv(i - 1);
-        v_.insert(v_.begin() + i, c);
-        touch_after(i);
-    }
-    std::vector<PzlCand> v_;
-    long long inversions_ = 0;
-};
-
-// ---------------------------------------------------------------------------
-// Device buffers and launches
-// ---------------------------------------------------------------------------
+const long long kDefaultHitCap = 1LL << 26;  // hits per batch slot (12 bytes each, twice)
 
 void free_plan(Plan *p)
 {
     if (!p) return;
-    hipSetDevice(p->device);
-    for (auto &c : p->classes) hipFree(c.d_harms);
-    hipFree(p->d_harms);
-    hipFree(p->d_bank);
-    hipFree(p->d_startr);
-    hipFree(p->d_zinds);
-    hipFree(p->d_winds);
-    hipFree(p->d_fft_own);
-    hipFree(p->d_segs);
-    hipFree(p->d_rinds);
-    hipFree(p->d_hr);
-    hipFree(p->d_planes);
-    hipFree(p->d_hits);
-    hipFree(p->d_nhits);
-    hipFree(p->d_sums);
-    if (p->st) hipStreamDestroy(p->st);
+    (void)hipSetDevice(p->device);
+    if (p->st) (void)hipStreamSynchronize(p->st);
+    if (p->st2) (void)hipStreamSynchronize(p->st2);
+    for (auto &c : p->classes) (void)hipFree(c.d_harms);
+    (void)hipFree(p->d_harms);
+    (void)hipFree(p->d_bank);
+    (void)hipFree(p->d_startr);
+    (void)hipFree(p->d_zinds);
+    (void)hipFree(p->d_winds);
+    (void)hipFree(p->d_fft_own);
+    (void)hipFree(p->d_segs);
+    (void)hipFree(p->d_rinds);
+    (void)hipFree(p->d_hr);
+    (void)hipFree(p->d_planes);
+    for (int s = 0; s < Plan::kSlots; s++) {
+        (void)hipFree(p->d_hkeys[s]);
+        (void)hipFree(p->d_hpows[s]);
+        if (p->ev_count[s]) (void)hipEventDestroy(p->ev_count[s]);
+        if (p->ev_free[s]) (void)hipEventDestroy(p->ev_free[s]);
+    }
+    (void)hipFree(p->d_counts);
+    if (p->h_counts) (void)hipHostFree(p->h_counts);
+    (void)hipFree(p->d_skeys);
+    (void)hipFree(p->d_spows);
+    (void)hipFree(p->d_recs);
+    (void)hipFree(p->d_sort_temp);
+    if (p->h_recs) (void)hipHostFree(p->h_recs);
+    (void)hipFree(p->d_sums);
+    if (p->st) (void)hipStreamDestroy(p->st);
+    if (p->st2) (void)hipStreamDestroy(p->st2);
     delete p;
 }
 
@@ -348,7 +79,7 @@ struct Timers {
     std::vector<hipEvent_t> pool;
     size_t used = 0;
     struct Rec {
-        int kind;  // 0 seg, 1 corr, 2 sum
+        int kind;  // 0 segments, 1 correlation, 2 harmonic sum
         int fftlen;
         hipEvent_t a, b;
     };
@@ -357,23 +88,43 @@ struct Timers {
     {
         if (used == pool.size()) {
             hipEvent_t e;
-            hipEventCreate(&e);
+            (void)hipEventCreate(&e);
             pool.push_back(e);
         }
         return pool[used++];
     }
-    void clear()
-    {
-        used = 0;
-        recs.clear();
-    }
     ~Timers()
     {
-        for (auto e : pool) hipEventDestroy(e);
+        for (auto e : pool) (void)hipEventDestroy(e);
     }
 };
 
-int launch_batch(Plan &p, long long r0, int nb, float *d_sums, Timers *tm)
+// Records HIP events around one launch when timing is on.
+struct Scope {
+    Timers *tm;
+    int kind, fftlen;
+    hipStream_t st;
+    hipEvent_t a = nullptr;
+    Scope(Timers *t, int k, int f, hipStream_t s) : tm(t), kind(k), fftlen(f), st(s)
+    {
+        if (tm && tm->on) {
+            a = tm->get();
+            (void)hipEventRecord(a, st);
+        }
+    }
+    ~Scope()
+    {
+        if (tm && tm->on) {
+            hipEvent_t b = tm->get();
+            (void)hipEventRecord(b, st);
+            tm->recs.push_back({kind, fftlen, a, b});
+        }
+    }
+};
+
+// Segments, correlations and the harmonic sum of rsteps [r0, r0 + nb) into
+// hit slot 'slot'.
+int launch_batch(Plan &p, long long r0, int nb, int slot, float *d_sums, Timers *tm)
 {
     const int nh = (int)p.harms.size();
     PzlBatchArgs ba{};
@@ -392,16 +143,9 @@ int launch_batch(Plan &p, long long r0, int nb, float *d_sums, Timers *tm)
     ba.seg_stride = p.seg_stride;
     ba.rinds = p.d_rinds;
     ba.hr_tab = p.d_hr;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (tm && tm->on) {
-        e0 = tm->get();
-        HIPCHK(hipEventRecord(e0, p.st));
-    }
-    HIPCHK(pzl_launch_segments(&ba, p.max_fftlen, p.st));
-    if (tm && tm->on) {
-        e1 = tm->get();
-        HIPCHK(hipEventRecord(e1, p.st));
-        tm->recs.push_back({0, 0, e0, e1});
+    {
+        Scope s(tm, 0, 0, p.st);
+        PZL_HIPCHK(pzl_launch_segments(&ba, p.max_fftlen, p.st));
     }
     for (auto &c : p.classes) {
         PzlCorrArgs ca{};
@@ -419,55 +163,130 @@ int launch_batch(Plan &p, long long r0, int nb, float *d_sums, Timers *tm)
         ca.planes = p.d_planes;
         ca.plane_stride = p.plane_stride;
         ca.hr_tab = p.d_hr;
-        if (tm && tm->on) {
-            e0 = tm->get();
-            HIPCHK(hipEventRecord(e0, p.st));
-        }
-        HIPCHK(pzl_launch_corr(c.fftlen, &ca, 256 * 16, p.st));
-        if (tm && tm->on) {
-            e1 = tm->get();
-            HIPCHK(hipEventRecord(e1, p.st));
-            tm->recs.push_back({1, c.fftlen, e0, e1});
-        }
+        Scope s(tm, 1, c.fftlen, p.st);
+        PZL_HIPCHK(pzl_launch_corr(c.fftlen, &ca, 256 * 16, p.st));
     }
-    if (p.g.numharmstages > 1 || true) {
-        PzlSumArgs sa{};
-        sa.harms = p.d_harms;
-        sa.nh = nh;
-        sa.nb = nb;
-        sa.nstages = p.g.numharmstages;
-        sa.rstep0 = r0;
-        sa.numz = p.fnumz;
-        sa.numw = p.fnumw;
-        sa.corr_uselen = p.g.corr_uselen;
-        sa.planes = p.d_planes;
-        sa.plane_stride = p.plane_stride;
-        sa.rinds = p.d_rinds;
-        sa.zinds = p.d_zinds;
-        sa.winds = p.d_winds;
-        for (int s = 0; s < 6; s++) sa.powcut[s] = p.g.powcut[s];
-        sa.hits = p.d_hits;
-        sa.nhits = p.d_nhits;
-        sa.maxhits = p.maxhits;
-        sa.sums_out = d_sums;
-        sa.map_errors = (unsigned int *)(p.d_nhits + 1);
-        if (tm && tm->on) {
-            e0 = tm->get();
-            HIPCHK(hipEventRecord(e0, p.st));
-        }
-        HIPCHK(pzl_launch_harmsum(&sa, p.st));
-        if (tm && tm->on) {
-            e1 = tm->get();
-            HIPCHK(hipEventRecord(e1, p.st));
-            tm->recs.push_back({2, 0, e0, e1});
-        }
+    PzlSumArgs sa{};
+    sa.harms = p.d_harms;
+    sa.nh = nh;
+    sa.nb = nb;
+    sa.nstages = p.g.numharmstages;
+    sa.rstep0 = r0;
+    sa.numz = p.fnumz;
+    sa.numw = p.fnumw;
+    sa.corr_uselen = p.g.corr_uselen;
+    sa.planes = p.d_planes;
+    sa.plane_stride = p.plane_stride;
+    sa.rinds = p.d_rinds;
+    sa.zinds = p.d_zinds;
+    sa.winds = p.d_winds;
+    for (int s = 0; s < 6; s++) sa.powcut[s] = p.g.powcut[s];
+    sa.hit_keys = p.d_hkeys[slot];
+    sa.hit_pows = p.d_hpows[slot];
+    sa.nhits = p.d_counts + 2 * slot;
+    sa.maxhits = (unsigned long long)p.hitcap;
+    sa.sums_out = d_sums;
+    sa.startr = p.d_startr;
+    sa.hr_tab = p.d_hr;
+    sa.int_rlo = p.int_rlo;
+    {
+        Scope s(tm, 2, 0, p.st);
+        PZL_HIPCHK(pzl_launch_harmsum(&sa, p.st));
     }
     return 0;
 }
 
+struct SearchState {
+    CandList cl;
+    double ms_host = 0;
+    long long nhits = 0;
+};
+
+// Order the hits of one slot on the GPU, bring them over and apply them to
+// the candidate list (the caller has waited for the slot's count).
+int apply_slot(Plan &p, int slot, long long n, SearchState &ss)
+{
+    if (n <= 0) return 0;
+    PZL_HIPCHK(pzl_sort_hits(p.d_sort_temp, &p.sort_temp_bytes, p.d_hkeys[slot], p.d_skeys, p.d_hpows[slot],
+                             p.d_spows, n, p.key_bits, p.st2));
+    PZL_HIPCHK(pzl_launch_hitprep(p.d_skeys, p.d_spows, n, p.d_startr, p.g.corr_uselen, p.fnumz, p.fnumw,
+                                  p.g.numharmstages, p.numindep_d, p.d_recs, p.st2));
+    if (n > p.h_recs_cap) {
+        if (p.h_recs) PZL_HIPCHK(hipHostFree(p.h_recs));
+        p.h_recs = nullptr;
+        const long long cap = std::min<long long>(std::max<long long>(2 * n, 1 << 20), p.hitcap);
+        PZL_HIPCHK(hipHostMalloc((void **)&p.h_recs, sizeof(PzlHostHit) * cap, hipHostMallocDefault));
+        p.h_recs_cap = cap;
+    }
+    PZL_HIPCHK(hipMemcpyAsync(p.h_recs, p.d_recs, sizeof(PzlHostHit) * n, hipMemcpyDeviceToHost, p.st2));
+    PZL_HIPCHK(hipStreamSynchronize(p.st2));
+    const auto t0 = std::chrono::steady_clock::now();
+    for (long long i = 0; i < n; i++) {
+        const PzlHostHit &h = p.h_recs[i];
+        ss.cl.insert(h.rr, h.sigma, [&]() {
+            const int numharm = 1 << h.stage;
+            PzlCand c;
+            c.power = h.power;
+            c.sigma = h.sigma;
+            c.numharm = numharm;
+            c.r = h.rr;
+            c.z = (p.fzlo + h.zi * (double)PZL_DZ) / (double)numharm;
+            c.w = (p.fwlo + h.wi * (double)PZL_DW) / (double)numharm;
+            return c;
+        });
+    }
+    if (p.last_hits_complete) {
+        if ((long long)p.last_hits.size() + n <= p.keep_hits_limit) {
+            p.last_hits.insert(p.last_hits.end(), p.h_recs, p.h_recs + n);
+        } else {
+            p.last_hits.clear();
+            p.last_hits_complete = false;
+        }
+    }
+    ss.ms_host += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    ss.nhits += n;
+    return 0;
+}
+
+int reset_slot(Plan &p, int slot, hipStream_t st)
+{
+    PZL_HIPCHK(hipMemsetAsync(p.d_counts + 2 * slot, 0, 2 * sizeof(unsigned long long), st));
+    PZL_HIPCHK(hipEventRecord(p.ev_free[slot], st));
+    return 0;
+}
+
+// Wait for the batch in 'slot', order and apply its hits.  A batch that
+// overflowed the slot is searched again one rstep at a time (one rstep never
+// exceeds a slot: hitcap >= cells per rstep x stages).
+int finalize(Plan &p, int slot, long long r0, int nb, SearchState &ss, Timers *tm)
+{
+    PZL_HIPCHK(hipEventSynchronize(p.ev_count[slot]));
+    const unsigned long long n = p.h_counts[2 * slot];
+    if ((long long)n <= p.hitcap) {
+        int rc = apply_slot(p, slot, (long long)n, ss);
+        if (rc) return rc;
+        return reset_slot(p, slot, p.st2);
+    }
+    // overflow: rerun behind whatever the search stream holds
+    for (int b = 0; b < nb; b++) {
+        int rc = reset_slot(p, slot, p.st);
+        if (rc) return rc;
+        rc = launch_batch(p, r0 + b, 1, slot, nullptr, tm);
+        if (rc) return rc;
+        PZL_HIPCHK(hipMemcpyAsync(p.h_counts + 2 * slot, p.d_counts + 2 * slot, 2 * sizeof(unsigned long long),
+                                  hipMemcpyDeviceToHost, p.st));
+        PZL_HIPCHK(hipStreamSynchronize(p.st));
+        const unsigned long long n1 = p.h_counts[2 * slot];
+        if ((long long)n1 > p.hitcap)
+            return fail(PZL_ERR_OVERFLOW, "hit buffer overflow: %llu hits in one rstep", n1);
+        rc = apply_slot(p, slot, (long long)n1, ss);
+        if (rc) return rc;
+    }
+    return reset_slot(p, slot, p.st2);
+}
+
 double plane_alg_bytes(const Plan &p)
 {
-    // written once by the correlation and read once by the harmonic sum
     double b = 0;
     for (auto &H : p.harms) b += 2.0 * H.nrows * (double)H.max_numrs * 4.0;
     return b;
@@ -485,7 +304,7 @@ const char *pzl_last_error(void) { return g_err.c_str(); }
 
 int pzl_device_count(int *count)
 {
-    HIPCHK(hipGetDeviceCount(count));
+    PZL_HIPCHK(hipGetDeviceCount(count));
     return 0;
 }
 
@@ -508,7 +327,7 @@ int pzl_plan_create(const PzlConfig *cfg, int device, void **plan_out)
         return rc;
     }
     for (auto &H : p->harms) {
-        if (!pzl_corr_supported(H.fftlen) || H.fftlen > 4096) {
+        if (!pzl_corr_supported(H.fftlen)) {
             delete p;
             return fail(PZL_ERR_UNSUPPORTED, "correlation length %d (harmonic %d/%d) is not supported", H.fftlen,
                         H.harmnum, H.numharm);
@@ -520,6 +339,7 @@ int pzl_plan_create(const PzlConfig *cfg, int device, void **plan_out)
         }
         p->max_fftlen = std::max(p->max_fftlen, H.fftlen);
     }
+    for (int s = 0; s < p->g.numharmstages; s++) p->numindep_d[s] = (double)p->g.numindep[s];
     p->B = cfg->batchsize > 0 ? cfg->batchsize : 8;
     if (p->g.nrsteps > 0 && p->B > p->g.nrsteps) p->B = (int)p->g.nrsteps;
     if (p->B < 1) p->B = 1;
@@ -543,15 +363,63 @@ int pzl_plan_create(const PzlConfig *cfg, int device, void **plan_out)
             p->winds[(size_t)h * F.numw + ii] = pzl_index_from_w(pzl_calc_required_w(H.hf, ww), H.wlo);
         }
     }
+    // The harmonic sum dereferences these maps unchecked: prove they stay in
+    // their planes for every rstep (r maps are monotone in r, so the ends
+    // bound them) and that the integer r map equals the double formula.
+    for (double s : p->startr)
+        if (s != floor(s)) p->int_rlo = 0;
+    for (int h = 1; h < nh; h++) {
+        const PzlHarm &H = p->harms[h];
+        for (int jj = 0; jj < F.numz; jj++)
+            if (p->zinds[(size_t)h * F.numz + jj] >= H.numz) {
+                delete p;
+                return fail(PZL_ERR_UNSUPPORTED, "z map leaves harmonic %d/%d", H.harmnum, H.numharm);
+            }
+        for (int ii = 0; ii < F.numw; ii++)
+            if (p->winds[(size_t)h * F.numw + ii] < 0 || p->winds[(size_t)h * F.numw + ii] >= H.numw) {
+                delete p;
+                return fail(PZL_ERR_UNSUPPORTED, "w map leaves harmonic %d/%d", H.harmnum, H.numharm);
+            }
+        const int rstep_bins = (int)(p->g.corr_uselen * PZL_DR);
+        for (double s : p->startr) {
+            const PzlHarmR hr = pzl_harm_r(H.hf, 0, s, s + rstep_bins - PZL_DR, p->g.corr_uselen);
+            const int lo = pzl_rind(H.hf, s, hr.rlo, 0), hi = pzl_rind(H.hf, s, hr.rlo, p->g.corr_uselen - 1);
+            bool ok = lo >= 0 && hi < H.rs_stride && hi < hr.numrs;
+            if (ok && p->int_rlo) {
+                // integer round-half-even of (2 s + ii) * harmnum / 2^stage
+                for (int ii : {0, 1, p->g.corr_uselen / 2, p->g.corr_uselen - 1}) {
+                    const long long prod = ((long long)(2.0 * s) + ii) * H.harmnum;
+                    long long q = prod >> H.stage;
+                    const long long rem = prod & ((1LL << H.stage) - 1), half = 1LL << (H.stage - 1);
+                    if (rem > half || (rem == half && (q & 1))) q++;
+                    ok = ok && (q - 2 * hr.rlo == pzl_rind(H.hf, s, hr.rlo, ii));
+                }
+            }
+            if (!ok) {
+                delete p;
+                return fail(PZL_ERR_UNSUPPORTED, "r map leaves harmonic %d/%d at startr %.1f", H.harmnum,
+                            H.numharm, s);
+            }
+        }
+    }
     std::map<int, std::vector<int>> cls;
     for (int h = 0; h < nh; h++) cls[p->harms[h].fftlen].push_back(h);
-    // longest first: better tail behaviour inside a batch
     for (auto it = cls.rbegin(); it != cls.rend(); ++it) {
         SizeClass c;
         c.fftlen = it->first;
         c.harms = it->second;
         p->classes.push_back(c);
     }
+    // hit slots: one rstep can never overflow a slot
+    const long long per_rstep = (long long)p->g.corr_uselen * F.numz * F.numw * p->g.numharmstages;
+    long long cap = cfg->maxhits > 0 ? cfg->maxhits : kDefaultHitCap;
+    cap = std::min<long long>(cap, per_rstep * p->B);
+    cap = std::max<long long>(cap, per_rstep);
+    p->hitcap = cap;
+    const long long maxkey = std::max<long long>(p->g.nrsteps, 1) * p->g.numharmstages * F.numw * F.numz *
+                             (long long)p->g.corr_uselen;
+    p->key_bits = 1;
+    while (p->key_bits < 63 && (1LL << p->key_bits) < maxkey) p->key_bits++;
 
     auto cleanup = [&](int code) {
         free_plan(p);
@@ -565,14 +433,22 @@ int pzl_plan_create(const PzlConfig *cfg, int device, void **plan_out)
     } while (0)
     PCHK(hipSetDevice(device));
     PCHK(hipStreamCreateWithFlags(&p->st, hipStreamNonBlocking));
+    PCHK(hipStreamCreateWithFlags(&p->st2, hipStreamNonBlocking));
+    {
+        const long long nbatches = (p->g.nrsteps + p->B - 1) / p->B;
+        p->nslots = (int)std::max<long long>(1, std::min<long long>(Plan::kSlots, nbatches));
+    }
+    for (int s = 0; s < p->nslots; s++) {
+        PCHK(hipEventCreateWithFlags(&p->ev_count[s], hipEventDisableTiming));
+        PCHK(hipEventCreateWithFlags(&p->ev_free[s], hipEventDisableTiming));
+    }
     PCHK(hipMalloc(&p->d_harms, sizeof(PzlHarm) * nh));
     PCHK(hipMemcpy(p->d_harms, p->harms.data(), sizeof(PzlHarm) * nh, hipMemcpyHostToDevice));
     for (auto &c : p->classes) {
         PCHK(hipMalloc(&c.d_harms, sizeof(int) * c.harms.size()));
         PCHK(hipMemcpy(c.d_harms, c.harms.data(), sizeof(int) * c.harms.size(), hipMemcpyHostToDevice));
     }
-    const long long bank_elems = p->g.bank_bytes / 8;
-    PCHK(hipMalloc(&p->d_bank, sizeof(float2) * bank_elems));
+    PCHK(hipMalloc(&p->d_bank, p->g.bank_bytes));
     const size_t nstart = std::max<size_t>(p->startr.size(), 1);
     PCHK(hipMalloc(&p->d_startr, sizeof(double) * nstart));
     if (!p->startr.empty())
@@ -588,14 +464,27 @@ int pzl_plan_create(const PzlConfig *cfg, int device, void **plan_out)
     PCHK(hipMalloc(&p->d_rinds, sizeof(uint16_t) * (size_t)p->B * nh * p->g.corr_uselen));
     PCHK(hipMalloc(&p->d_hr, sizeof(PzlHrTab) * (size_t)p->B * nh));
     PCHK(hipMalloc(&p->d_planes, sizeof(float) * (size_t)p->plane_stride * p->B));
-    p->maxhits = (unsigned long long)(cfg->maxhits > 0 ? cfg->maxhits : kDefaultMaxHits);
-    PCHK(hipMalloc(&p->d_hits, sizeof(PzlHit) * p->maxhits));
-    PCHK(hipMalloc(&p->d_nhits, 2 * sizeof(unsigned long long)));
+    for (int s = 0; s < p->nslots; s++) {
+        PCHK(hipMalloc(&p->d_hkeys[s], sizeof(long long) * p->hitcap));
+        PCHK(hipMalloc(&p->d_hpows[s], sizeof(float) * p->hitcap));
+    }
+    PCHK(hipMalloc(&p->d_counts, 2 * Plan::kSlots * sizeof(unsigned long long)));
+    PCHK(hipMemset(p->d_counts, 0, 2 * Plan::kSlots * sizeof(unsigned long long)));
+    PCHK(hipHostMalloc((void **)&p->h_counts, 2 * Plan::kSlots * sizeof(unsigned long long), hipHostMallocDefault));
+    PCHK(hipMalloc(&p->d_skeys, sizeof(long long) * p->hitcap));
+    PCHK(hipMalloc(&p->d_spows, sizeof(float) * p->hitcap));
+    PCHK(hipMalloc(&p->d_recs, sizeof(PzlHostHit) * p->hitcap));
+    p->sort_temp_bytes = 0;
+    PCHK(pzl_sort_hits(nullptr, &p->sort_temp_bytes, p->d_hkeys[0], p->d_skeys, p->d_hpows[0], p->d_spows,
+                       p->hitcap, p->key_bits, p->st2));
+    PCHK(hipMalloc(&p->d_sort_temp, std::max<size_t>(p->sort_temp_bytes, 16)));
     // The kernel bank, built once and kept resident.
     PCHK(pzl_launch_bank_build(p->d_harms, nh, p->g.numharm_kernels, p->max_fftlen, p->d_bank, p->st));
     PCHK(hipStreamSynchronize(p->st));
     p->g.device_bytes = p->g.bank_bytes + (long long)sizeof(float) * p->plane_stride * p->B +
-                        (long long)sizeof(float2) * p->seg_stride * p->B + (long long)sizeof(PzlHit) * p->maxhits +
+                        (long long)sizeof(float2) * p->seg_stride * p->B +
+                        (long long)(p->nslots * 12 + 12 + (long long)sizeof(PzlHostHit)) * p->hitcap +
+                        (long long)p->sort_temp_bytes +
                         (long long)sizeof(uint16_t) * p->B * nh * p->g.corr_uselen;
 #undef PCHK
     *plan_out = p;
@@ -633,13 +522,13 @@ int pzl_set_spectrum(void *plan, const float *host_fft, long long numbins)
 {
     Plan *p = (Plan *)plan;
     if (!p || !host_fft || numbins < 1) return fail(PZL_ERR_ARG, "bad spectrum");
-    HIPCHK(hipSetDevice(p->device));
+    PZL_HIPCHK(hipSetDevice(p->device));
     if (p->d_fft_own && p->fft_bins != numbins) {
-        hipFree(p->d_fft_own);
+        PZL_HIPCHK(hipFree(p->d_fft_own));
         p->d_fft_own = nullptr;
     }
-    if (!p->d_fft_own) HIPCHK(hipMalloc(&p->d_fft_own, sizeof(float2) * numbins));
-    HIPCHK(hipMemcpy(p->d_fft_own, host_fft, sizeof(float2) * numbins, hipMemcpyHostToDevice));
+    if (!p->d_fft_own) PZL_HIPCHK(hipMalloc(&p->d_fft_own, sizeof(float2) * numbins));
+    PZL_HIPCHK(hipMemcpy(p->d_fft_own, host_fft, sizeof(float2) * numbins, hipMemcpyHostToDevice));
     p->d_fft = p->d_fft_own;
     p->fft_bins = numbins;
     return 0;
@@ -661,88 +550,91 @@ int pzl_search(void *plan, long long rstep_lo, long long rstep_hi, PzlCand *cand
     if (!p) return fail(PZL_ERR_ARG, "null plan");
     if (!p->d_fft) return fail(PZL_ERR_NOSPECTRUM, "no spectrum set");
     const auto t0 = std::chrono::steady_clock::now();
-    HIPCHK(hipSetDevice(p->device));
+    PZL_HIPCHK(hipSetDevice(p->device));
     const long long nr = p->g.nrsteps;
     if (rstep_lo < 0) rstep_lo = 0;
     if (rstep_hi < 0 || rstep_hi > nr) rstep_hi = nr;
     Timers tm;
     tm.on = (stats != nullptr);
-    HIPCHK(hipMemsetAsync(p->d_nhits, 0, 2 * sizeof(unsigned long long), p->st));
+    SearchState ss;
+    p->last_hits.clear();
+    p->last_hits_complete = true;
     hipEvent_t ev_start = nullptr, ev_end = nullptr;
     if (tm.on) {
         ev_start = tm.get();
-        HIPCHK(hipEventRecord(ev_start, p->st));
+        PZL_HIPCHK(hipEventRecord(ev_start, p->st));
     }
-    for (long long r0 = rstep_lo; r0 < rstep_hi; r0 += p->B) {
+    PZL_HIPCHK(hipMemsetAsync(p->d_counts, 0, 2 * Plan::kSlots * sizeof(unsigned long long), p->st));
+    // Batches in flight, oldest first; the GPU runs up to nslots batches
+    // ahead of the host's candidate insertion.
+    struct Pending {
+        int slot, nb;
+        long long r0;
+    };
+    std::vector<Pending> pend;
+    size_t head = 0;
+    int i = 0;
+    for (long long r0 = rstep_lo; r0 < rstep_hi; r0 += p->B, i++) {
         const int nb = (int)std::min<long long>(p->B, rstep_hi - r0);
-        int rc = launch_batch(*p, r0, nb, nullptr, &tm);
+        const int slot = i % p->nslots;
+        if (pend.size() - head == (size_t)p->nslots) {
+            const Pending q = pend[head++];
+            int rc = finalize(*p, q.slot, q.r0, q.nb, ss, &tm);
+            if (rc) return rc;
+        }
+        if (i >= p->nslots) PZL_HIPCHK(hipStreamWaitEvent(p->st, p->ev_free[slot], 0));
+        int rc = launch_batch(*p, r0, nb, slot, nullptr, &tm);
         if (rc) return rc;
+        PZL_HIPCHK(hipMemcpyAsync(p->h_counts + 2 * slot, p->d_counts + 2 * slot, 2 * sizeof(unsigned long long),
+                                  hipMemcpyDeviceToHost, p->st));
+        PZL_HIPCHK(hipEventRecord(p->ev_count[slot], p->st));
+        pend.push_back({slot, nb, r0});
+        // apply whatever the GPU has already finished
+        while (head + 1 < pend.size() && hipEventQuery(p->ev_count[pend[head].slot]) == hipSuccess) {
+            const Pending q = pend[head++];
+            rc = finalize(*p, q.slot, q.r0, q.nb, ss, &tm);
+            if (rc) return rc;
+        }
     }
     if (tm.on) {
         ev_end = tm.get();
-        HIPCHK(hipEventRecord(ev_end, p->st));
+        PZL_HIPCHK(hipEventRecord(ev_end, p->st));
     }
-    unsigned long long counters[2] = {0, 0};
-    HIPCHK(hipMemcpyAsync(counters, p->d_nhits, sizeof counters, hipMemcpyDeviceToHost, p->st));
-    HIPCHK(hipStreamSynchronize(p->st));
-    const auto t1 = std::chrono::steady_clock::now();
-    const unsigned long long nh = counters[0];
-    if (counters[1]) return fail(PZL_ERR_HIP, "index maps left a subharmonic plane %llu times", counters[1]);
-    if (nh > p->maxhits) {
-        p->last_hits.clear();
-        if (ncands) *ncands = 0;
-        return fail(PZL_ERR_OVERFLOW, "hit buffer overflow: %llu hits > %llu", nh, p->maxhits);
+    while (head < pend.size()) {
+        const Pending q = pend[head++];
+        int rc = finalize(*p, q.slot, q.r0, q.nb, ss, &tm);
+        if (rc) return rc;
     }
-    p->last_hits.resize(nh);
-    if (nh) HIPCHK(hipMemcpy(p->last_hits.data(), p->d_hits, sizeof(PzlHit) * nh, hipMemcpyDeviceToHost));
-    std::sort(p->last_hits.begin(), p->last_hits.end(),
-              [](const PzlHit &a, const PzlHit &b) { return a.key < b.key; });
-    // decode in PRESTO C scan order and insert (accel_utils.c:1518-1549)
-    CandList cl;
-    const long long S = p->g.numharmstages, W = p->fnumw, Z = p->fnumz, R = p->g.corr_uselen;
-    for (const PzlHit &h : p->last_hits) {
-        long long k = h.key;
-        const int r = (int)(k % R);
-        k /= R;
-        const int z = (int)(k % Z);
-        k /= Z;
-        const int w = (int)(k % W);
-        k /= W;
-        const int stage = (int)(k % S);
-        const long long rstep = k / S;
-        const int numharm = 1 << stage;
-        const long long frlo = (long long)floor(pzl_calc_required_r(1.0, p->startr[rstep]));
-        const float pow = h.power;
-        const float sig = (float)pzl_candidate_sigma(pow, numharm, (double)p->g.numindep[stage]);
-        const double rr = (frlo + r * (double)PZL_DR) / (double)numharm;
-        const double zz = (p->fzlo + z * (double)PZL_DZ) / (double)numharm;
-        const double ww = (p->fwlo + w * (double)PZL_DW) / (double)numharm;
-        cl.insert(pow, sig, numharm, rr, zz, ww);
-    }
-    const auto &items = cl.items();
+    PZL_HIPCHK(hipStreamSynchronize(p->st));
+    PZL_HIPCHK(hipStreamSynchronize(p->st2));
+    const auto &items = ss.cl.items();
     if (ncands) *ncands = (long long)items.size();
     if (cands)
-        for (long long i = 0; i < (long long)items.size() && i < maxcands; i++) cands[i] = items[i];
+        for (long long k = 0; k < (long long)items.size() && k < maxcands; k++) cands[k] = items[k];
     const auto t2 = std::chrono::steady_clock::now();
     if (stats) {
         memset(stats, 0, sizeof *stats);
         float ms = 0;
         std::map<int, double> per_class;
+        std::map<int, long long> per_class_n;
         for (auto &r : tm.recs) {
-            hipEventElapsedTime(&ms, r.a, r.b);
-            if (r.kind == 0) stats->ms_segments += ms;
-            else if (r.kind == 1) {
+            (void)hipEventElapsedTime(&ms, r.a, r.b);
+            if (r.kind == 0) {
+                stats->ms_segments += ms;
+            } else if (r.kind == 1) {
                 stats->ms_corr += ms;
                 per_class[r.fftlen] += ms;
+                per_class_n[r.fftlen]++;
                 stats->corr_launches++;
-            } else stats->ms_sum += ms;
+            } else {
+                stats->ms_sum += ms;
+            }
         }
-        for (auto &kv : per_class) stats->ms_corr_max_class = std::max(stats->ms_corr_max_class, kv.second);
-        hipEventElapsedTime(&ms, ev_start, ev_end);
+        (void)hipEventElapsedTime(&ms, ev_start, ev_end);
         stats->ms_gpu = ms;
         stats->ms_wall = std::chrono::duration<double, std::milli>(t2 - t0).count();
-        stats->ms_host = std::chrono::duration<double, std::milli>(t2 - t1).count();
-        stats->nhits = (long long)nh;
+        stats->ms_host = ss.ms_host;
+        stats->nhits = ss.nhits;
         stats->nrsteps = rstep_hi - rstep_lo;
         stats->cells = (double)stats->nrsteps * p->g.corr_uselen * p->g.numz * std::max(p->g.numw, 1);
         double seg_bytes = 0;
@@ -751,6 +643,24 @@ int pzl_search(void *plan, long long rstep_lo, long long rstep_hi, PzlCand *cand
         stats->bytes_alg = stats->nrsteps * (planes + seg_bytes) + (double)p->g.bank_bytes;
         stats->bytes_alg_corr = stats->nrsteps * (planes / 2 + seg_bytes) +
                                 (double)p->g.bank_bytes * std::ceil((double)stats->nrsteps / p->B);
+        for (auto &kv : per_class) stats->ms_corr_max_class = std::max(stats->ms_corr_max_class, kv.second);
+        stats->nclass = 0;
+        for (auto &c : p->classes) {
+            if (stats->nclass >= 16) break;
+            const int k = stats->nclass++;
+            stats->class_fftlen[k] = c.fftlen;
+            stats->class_ms[k] = per_class.count(c.fftlen) ? per_class[c.fftlen] : 0.0;
+            const long long launches = per_class_n.count(c.fftlen) ? per_class_n[c.fftlen] : 0;
+            stats->class_launches[k] = launches;
+            // per rstep: its plane rows written + segments read; per launch: the kernels read once
+            double per_rstep = 0, per_launch = 0;
+            for (int h : c.harms) {
+                const PzlHarm &H = p->harms[h];
+                per_rstep += (double)H.nrows * H.max_numrs * 4.0 + H.fftlen / 2 * 8.0;
+                per_launch += (double)H.nrows * H.fftlen * 8.0;
+            }
+            stats->class_bytes[k] = per_rstep * (double)(rstep_hi - rstep_lo) + per_launch * (double)launches;
+        }
     }
     return 0;
 }
@@ -759,6 +669,9 @@ int pzl_get_hits(void *plan, long long *keys, float *powers, long long maxhits, 
 {
     Plan *p = (Plan *)plan;
     if (!p) return fail(PZL_ERR_ARG, "null plan");
+    if (!p->last_hits_complete)
+        return fail(PZL_ERR_UNSUPPORTED, "the last search had more than %lld hits; they were not kept",
+                    p->keep_hits_limit);
     if (nhits) *nhits = (long long)p->last_hits.size();
     for (long long i = 0; i < (long long)p->last_hits.size() && i < maxhits; i++) {
         if (keys) keys[i] = p->last_hits[i].key;
@@ -773,9 +686,9 @@ int pzl_get_kernel(void *plan, int idx, int row, float *out)
     if (!p || !out || idx < 0 || idx >= (int)p->harms.size() || row < 0 || row >= p->harms[idx].nrows)
         return fail(PZL_ERR_ARG, "bad kernel index");
     const PzlHarm &H = p->harms[idx];
-    HIPCHK(hipSetDevice(p->device));
-    HIPCHK(hipMemcpy(out, p->d_bank + H.bank_off + (long long)row * H.fftlen, sizeof(float2) * H.fftlen,
-                     hipMemcpyDeviceToHost));
+    PZL_HIPCHK(hipSetDevice(p->device));
+    PZL_HIPCHK(hipMemcpy(out, p->d_bank + H.bank_off + (long long)row * H.fftlen, sizeof(float2) * H.fftlen,
+                         hipMemcpyDeviceToHost));
     return 0;
 }
 
@@ -786,21 +699,24 @@ int pzl_compute_plane(void *plan, long long rstep, int idx, float *out, long lon
     if (!p || idx < 0 || idx >= (int)p->harms.size() || rstep < 0 || rstep >= p->g.nrsteps)
         return fail(PZL_ERR_ARG, "bad plane request");
     if (!p->d_fft) return fail(PZL_ERR_NOSPECTRUM, "no spectrum set");
-    HIPCHK(hipSetDevice(p->device));
-    int rc = launch_batch(*p, rstep, 1, nullptr, nullptr);
+    PZL_HIPCHK(hipSetDevice(p->device));
+    PZL_HIPCHK(hipMemsetAsync(p->d_counts, 0, 4 * sizeof(unsigned long long), p->st));
+    int rc = launch_batch(*p, rstep, 1, 0, nullptr, nullptr);
     if (rc) return rc;
-    HIPCHK(hipStreamSynchronize(p->st));
+    PZL_HIPCHK(hipStreamSynchronize(p->st));
     PzlHrTab hr;
-    HIPCHK(hipMemcpy(&hr, p->d_hr + idx, sizeof hr, hipMemcpyDeviceToHost));
+    PZL_HIPCHK(hipMemcpy(&hr, p->d_hr + idx, sizeof hr, hipMemcpyDeviceToHost));
     const PzlHarm &H = p->harms[idx];
     if (numrs) *numrs = hr.numrs;
     if (rlo) *rlo = hr.rlo;
     if (out) {
         const long long need = (long long)H.nrows * hr.numrs;
         if (need > maxfloats) return fail(PZL_ERR_ARG, "output too small (%lld floats needed)", need);
-        HIPCHK(hipMemcpy2D(out, sizeof(float) * hr.numrs, p->d_planes + H.plane_off, sizeof(float) * H.rs_stride,
-                           sizeof(float) * hr.numrs, H.nrows, hipMemcpyDeviceToHost));
+        PZL_HIPCHK(hipMemcpy2D(out, sizeof(float) * hr.numrs, p->d_planes + H.plane_off,
+                               sizeof(float) * H.rs_stride, sizeof(float) * hr.numrs, H.nrows,
+                               hipMemcpyDeviceToHost));
     }
+    PZL_HIPCHK(hipMemset(p->d_counts, 0, 4 * sizeof(unsigned long long)));
     return 0;
 }
 
@@ -811,14 +727,42 @@ int pzl_compute_sums(void *plan, long long rstep, float *out, long long maxfloat
     if (!p->d_fft) return fail(PZL_ERR_NOSPECTRUM, "no spectrum set");
     const long long need = (long long)p->fnumw * p->fnumz * p->g.corr_uselen;
     if (need > maxfloats) return fail(PZL_ERR_ARG, "output too small");
-    HIPCHK(hipSetDevice(p->device));
-    if (!p->d_sums) HIPCHK(hipMalloc(&p->d_sums, sizeof(float) * need));
-    HIPCHK(hipMemsetAsync(p->d_nhits, 0, 2 * sizeof(unsigned long long), p->st));
-    int rc = launch_batch(*p, rstep, 1, p->d_sums, nullptr);
+    PZL_HIPCHK(hipSetDevice(p->device));
+    if (!p->d_sums) PZL_HIPCHK(hipMalloc(&p->d_sums, sizeof(float) * need));
+    PZL_HIPCHK(hipMemsetAsync(p->d_counts, 0, 4 * sizeof(unsigned long long), p->st));
+    int rc = launch_batch(*p, rstep, 1, 0, p->d_sums, nullptr);
     if (rc) return rc;
-    HIPCHK(hipStreamSynchronize(p->st));
-    HIPCHK(hipMemcpy(out, p->d_sums, sizeof(float) * need, hipMemcpyDeviceToHost));
+    PZL_HIPCHK(hipStreamSynchronize(p->st));
+    PZL_HIPCHK(hipMemcpy(out, p->d_sums, sizeof(float) * need, hipMemcpyDeviceToHost));
+    PZL_HIPCHK(hipMemset(p->d_counts, 0, 4 * sizeof(unsigned long long)));
     return 0;
 }
 
 }  // extern "C"
+
+// Host-side candidate insertion on its own (no GPU): applies n hits in order
+// with insert_new_accelcand semantics.  Exposed for testing the list logic.
+extern "C" int pzl_insert_cands(const double *rr, const float *sigma, const float *power, const int *numharm,
+                                const double *z, const double *w, long long n, PzlCand *out, long long maxout,
+                                long long *nout)
+{
+    if (n < 0 || (n > 0 && (!rr || !sigma || !power || !numharm || !z || !w)))
+        return fail(PZL_ERR_ARG, "bad hit arrays");
+    CandList cl;
+    for (long long i = 0; i < n; i++) {
+        cl.insert(rr[i], sigma[i], [&]() {
+            PzlCand c;
+            c.power = power[i];
+            c.sigma = sigma[i];
+            c.numharm = numharm[i];
+            c.r = rr[i];
+            c.z = z[i];
+            c.w = w[i];
+            return c;
+        });
+    }
+    const auto &v = cl.items();
+    if (nout) *nout = (long long)v.size();
+    for (long long i = 0; i < (long long)v.size() && i < maxout; i++) out[i] = v[i];
+    return 0;
+}

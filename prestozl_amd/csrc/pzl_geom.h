@@ -12,6 +12,7 @@
 #include <stdint.h>
 
 #if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
 #define PZL_HD __host__ __device__ __forceinline__
 #else
 #define PZL_HD inline

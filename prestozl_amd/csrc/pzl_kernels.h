@@ -13,12 +13,17 @@ struct PzlHrTab {
     int pad;
 };
 
-// A threshold crossing: key orders hits exactly as PRESTO C inserts them
-// (rstep, stage, w, z, r) -- accelsearch.c:183-223 with accel_utils.c:1507-1555.
-struct PzlHit {
+// A threshold crossing after ordering, as the host consumes it: key orders
+// hits exactly as PRESTO C inserts them (rstep, stage, w, z, r) --
+// accelsearch.c:183-223 with accel_utils.c:1507-1555; rr is the candidate's
+// fundamental r (accel_utils.c:1531) and sigma its float significance.
+struct PzlHostHit {
+    double rr;
     long long key;
+    float sigma;
     float power;
-    int pad;
+    short zi, wi;  // fundamental z and w indices
+    int stage;
 };
 
 struct PzlBatchArgs {
@@ -63,11 +68,14 @@ struct PzlSumArgs {
     const uint16_t *zinds;   // [nh][numz]
     const int *winds;        // [nh][numw]
     float powcut[6];
-    PzlHit *hits;
+    long long *hit_keys;     // unordered hits of this batch: keys ...
+    float *hit_pows;         // ... and summed powers
     unsigned long long *nhits;
     unsigned long long maxhits;
     float *sums_out;         // optional [nb][numw][numz][corr_uselen]: final summed powers
-    unsigned int *map_errors;  // count of index-map entries outside a plane (must stay 0)
+    const double *startr;    // fundamental startr of every rstep
+    const PzlHrTab *hr_tab;  // [nb][nh] subharmonic rlo of the batch
+    int int_rlo;             // startr are integers: r maps computed in integer arithmetic
 };
 
 #ifdef __cplusplus
@@ -78,6 +86,14 @@ hipError_t pzl_launch_bank_build(const PzlHarm *d_harms, int nh, int total_rows,
 hipError_t pzl_launch_segments(const PzlBatchArgs *a, int max_fftlen, hipStream_t st);
 hipError_t pzl_launch_corr(int N, const PzlCorrArgs *a, int nblocks, hipStream_t st);
 hipError_t pzl_launch_harmsum(const PzlSumArgs *a, hipStream_t st);
+// Radix sort of n (key, power) pairs on 'end_bit' key bits; temp == nullptr
+// queries the scratch size into *temp_bytes.
+hipError_t pzl_sort_hits(void *temp, size_t *temp_bytes, const long long *keys_in, long long *keys_out,
+                         const float *pows_in, float *pows_out, long long n, int end_bit, hipStream_t st);
+// Sorted (key, power) -> host records with rr and sigma (double precision).
+hipError_t pzl_launch_hitprep(const long long *keys, const float *pows, long long n, const double *startr,
+                              int corr_uselen, int numz, int numw, int nstages, const double *numindep,
+                              PzlHostHit *out, hipStream_t st);
 int pzl_corr_supported(int n);
 #ifdef __cplusplus
 }

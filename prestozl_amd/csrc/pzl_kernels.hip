@@ -15,9 +15,12 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <hipcub/hipcub.hpp>
+
 #include "pzl_fft.h"
 #include "pzl_kernels.h"
 #include "pzl_math.h"
+#include "pzl_stats.h"
 
 // ---------------------------------------------------------------------------
 // Kernel bank
@@ -407,8 +410,27 @@ __global__ __launch_bounds__(NT) void k_corr(PzlCorrArgs a)
 
 #define SUM_NT 256
 
+// rint((m * j) / 2^s) with round-half-even, in integers: the subharmonic r
+// index of fundamental point m = 2*fullrlo + ii (accel_utils.c:1115-1122 with
+// integer fullrlo; rint rounds half to even).
+__device__ __forceinline__ long long rint_dyadic(long long m, int j, int s)
+{
+    const long long prod = m * j;
+    long long q = prod >> s;
+    if (s > 0) {
+        const long long rem = prod & ((1LL << s) - 1), half = 1LL << (s - 1);
+        if (rem > half || (rem == half && (q & 1))) q++;
+    }
+    return q;
+}
+
+// One workgroup: SUM_NT consecutive r of one (rstep, w, z) row of the
+// fundamental.  All subharmonic powers of a cell are loaded first and then
+// added in the reference's order (stage 1..S, odd harmonics ascending).
+template <int NSTAGES>
 __global__ __launch_bounds__(SUM_NT) void k_harmsum(PzlSumArgs a)
 {
+    constexpr int NSUB = (1 << (NSTAGES - 1)) - 1;
     const int tid = threadIdx.x;
     const int chunks = (a.corr_uselen + SUM_NT - 1) / SUM_NT;
     const long long blk = blockIdx.x;
@@ -418,51 +440,52 @@ __global__ __launch_bounds__(SUM_NT) void k_harmsum(PzlSumArgs a)
     const int w = (int)((rowid / a.numz) % a.numw);
     const int b = (int)(rowid / ((long long)a.numz * a.numw));
     const int r = chunk * SUM_NT + tid;
-    const bool active = (r < a.corr_uselen) && (b < a.nb);
-    const PzlHarm F = a.harms[0];
+    const bool active = r < a.corr_uselen;
+    const int rr = active ? r : a.corr_uselen - 1;
     const long long rstep = a.rstep0 + b;
-    float acc = 0.f;
-    if (active) acc = a.planes[(long long)b * a.plane_stride + F.plane_off + ((long long)w * a.numz + z) * F.rs_stride + r];
+    const float *base = a.planes + (long long)b * a.plane_stride;
+    const PzlHarm F = a.harms[0];
+    float acc = base[F.plane_off + ((long long)w * a.numz + z) * F.rs_stride + rr];
 
+    float v[NSUB > 0 ? NSUB : 1];
+    if constexpr (NSUB > 0) {
+        const long long m = a.int_rlo ? (long long)(2.0 * a.startr[rstep]) + rr : 0;
+#pragma unroll
+        for (int q = 0; q < NSUB; q++) {
+            const int h = q + 1;
+            const PzlHarm H = a.harms[h];
+            const float *row = base + H.plane_off +
+                               ((long long)a.winds[h * a.numw + w] * H.numz + a.zinds[h * a.numz + z]) * H.rs_stride;
+            int rind;
+            if (a.int_rlo)
+                rind = (int)(rint_dyadic(m, H.harmnum, H.stage) - 2 * a.hr_tab[b * a.nh + h].rlo);
+            else
+                rind = a.rinds[((long long)b * a.nh + h) * a.corr_uselen + rr];
+            v[q] = row[rind];
+        }
+    }
     const unsigned long long lanemask_lt = (1ull << (__lane_id())) - 1ull;
-    int h = 1;
-    for (int s = 0; s < a.nstages; s++) {
+    int q = 0;
+#pragma unroll
+    for (int s = 0; s < NSTAGES; s++) {
         if (s > 0) {
-            const int nsub = 1 << (s - 1);
-            for (int q = 0; q < nsub; q++, h++) {
-                if (active) {
-                    const PzlHarm H = a.harms[h];
-                    const int wind = a.winds[h * a.numw + w];
-                    const int zind = a.zinds[h * a.numz + z];
-                    const int rind = a.rinds[((long long)b * a.nh + h) * a.corr_uselen + r];
-                    // the maps stay inside the subharmonic plane by construction;
-                    // a violation is counted, never dereferenced
-                    if ((unsigned)wind < (unsigned)H.numw && (unsigned)zind < (unsigned)H.numz &&
-                        rind < H.rs_stride) {
-                        acc += a.planes[(long long)b * a.plane_stride + H.plane_off +
-                                        ((long long)wind * H.numz + zind) * H.rs_stride + rind];
-                    } else {
-                        atomicAdd(a.map_errors, 1u);
-                    }
-                }
-            }
+#pragma unroll
+            for (int k = 0; k < (1 << (s - 1)); k++, q++) acc += v[q];
         }
         const bool hit = active && (acc > a.powcut[s]);
         const unsigned long long mask = __ballot(hit);
         if (mask) {
             const int cnt = __popcll(mask);
             const int leader = __ffsll((long long)mask) - 1;
-            unsigned long long base = 0;
-            if ((int)__lane_id() == leader) base = atomicAdd(a.nhits, (unsigned long long)cnt);
-            base = __shfl(base, leader);
+            unsigned long long slot0 = 0;
+            if ((int)__lane_id() == leader) slot0 = atomicAdd(a.nhits, (unsigned long long)cnt);
+            slot0 = __shfl(slot0, leader);
             if (hit) {
-                const unsigned long long slot = base + __popcll(mask & lanemask_lt);
+                const unsigned long long slot = slot0 + __popcll(mask & lanemask_lt);
                 if (slot < a.maxhits) {
-                    PzlHit hv;
-                    hv.key = ((((rstep * a.nstages + s) * a.numw + w) * (long long)a.numz + z) * a.corr_uselen + r);
-                    hv.power = acc;
-                    hv.pad = 0;
-                    a.hits[slot] = hv;
+                    a.hit_keys[slot] =
+                        ((((rstep * NSTAGES + s) * a.numw + w) * (long long)a.numz + z) * a.corr_uselen + r);
+                    a.hit_pows[slot] = acc;
                 }
             }
         }
@@ -472,8 +495,80 @@ __global__ __launch_bounds__(SUM_NT) void k_harmsum(PzlSumArgs a)
 }
 
 // ---------------------------------------------------------------------------
+// Hit records: decode, fundamental r and significance of every sorted hit
+// (accel_utils.c:1518-1533 with candidate_sigma, characteristics.c:647-668)
+// ---------------------------------------------------------------------------
+
+struct HitPrepArgs {
+    const long long *keys;
+    const float *pows;
+    long long n;
+    const double *startr;
+    int corr_uselen, numz, numw, nstages;
+    double numindep[6];
+    PzlHostHit *out;
+};
+
+__global__ __launch_bounds__(256) void k_hitprep(HitPrepArgs a)
+{
+    for (long long i = blockIdx.x * 256ll + threadIdx.x; i < a.n; i += (long long)gridDim.x * 256) {
+        const long long key = a.keys[i];
+        long long k = key;
+        const int r = (int)(k % a.corr_uselen);
+        k /= a.corr_uselen;
+        const int zi = (int)(k % a.numz);
+        k /= a.numz;
+        const int wi = (int)(k % a.numw);
+        k /= a.numw;
+        const int stage = (int)(k % a.nstages);
+        const long long rstep = k / a.nstages;
+        const int numharm = 1 << stage;
+        const double frlo = floor(pzl_calc_required_r(1.0, a.startr[rstep]));
+        PzlHostHit h;
+        h.key = key;
+        h.power = a.pows[i];
+        h.rr = (frlo + r * (double)PZL_DR) / (double)numharm;
+        h.sigma = (float)pzl_candidate_sigma((double)h.power, numharm, a.numindep[stage]);
+        h.zi = (short)zi;
+        h.wi = (short)wi;
+        h.stage = stage;
+        a.out[i] = h;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Launchers (C linkage for the host engine)
 // ---------------------------------------------------------------------------
+
+extern "C" hipError_t pzl_sort_hits(void *temp, size_t *temp_bytes, const long long *keys_in, long long *keys_out,
+                                    const float *pows_in, float *pows_out, long long n, int end_bit,
+                                    hipStream_t st)
+{
+    return hipcub::DeviceRadixSort::SortPairs(temp, *temp_bytes, keys_in, keys_out, pows_in, pows_out, (int)n, 0,
+                                              end_bit, st);
+}
+
+extern "C" hipError_t pzl_launch_hitprep(const long long *keys, const float *pows, long long n,
+                                         const double *startr, int corr_uselen, int numz, int numw, int nstages,
+                                         const double *numindep, PzlHostHit *out, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    HitPrepArgs a;
+    a.keys = keys;
+    a.pows = pows;
+    a.n = n;
+    a.startr = startr;
+    a.corr_uselen = corr_uselen;
+    a.numz = numz;
+    a.numw = numw;
+    a.nstages = nstages;
+    for (int s = 0; s < 6; s++) a.numindep[s] = numindep[s];
+    a.out = out;
+    long long blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(k_hitprep, dim3((unsigned)blocks), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
 
 extern "C" hipError_t pzl_launch_bank_build(const PzlHarm *d_harms, int nh, int total_rows, int max_fftlen,
                                             float2 *d_bank, hipStream_t st)
@@ -535,6 +630,15 @@ extern "C" hipError_t pzl_launch_harmsum(const PzlSumArgs *a, hipStream_t st)
 {
     const int chunks = (a->corr_uselen + SUM_NT - 1) / SUM_NT;
     const long long nblk = (long long)a->nb * a->numw * a->numz * chunks;
-    hipLaunchKernelGGL(k_harmsum, dim3((unsigned)nblk), dim3(SUM_NT), 0, st, *a);
+    const dim3 g((unsigned)nblk), t(SUM_NT);
+    switch (a->nstages) {
+    case 1: hipLaunchKernelGGL(k_harmsum<1>, g, t, 0, st, *a); break;
+    case 2: hipLaunchKernelGGL(k_harmsum<2>, g, t, 0, st, *a); break;
+    case 3: hipLaunchKernelGGL(k_harmsum<3>, g, t, 0, st, *a); break;
+    case 4: hipLaunchKernelGGL(k_harmsum<4>, g, t, 0, st, *a); break;
+    case 5: hipLaunchKernelGGL(k_harmsum<5>, g, t, 0, st, *a); break;
+    case 6: hipLaunchKernelGGL(k_harmsum<6>, g, t, 0, st, *a); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }

@@ -1,0 +1,93 @@
+// pzl_plan.h -- host-side state of one search plan (internal).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "pzl_accel.h"
+#include "pzl_kernels.h"
+
+namespace pzl {
+
+int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+extern thread_local std::string g_err;
+
+#define PZL_HIPCHK(call)                                                                                \
+    do {                                                                                                \
+        hipError_t e_ = (call);                                                                         \
+        if (e_ != hipSuccess)                                                                           \
+            return ::pzl::fail(PZL_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, \
+                               __LINE__);                                                               \
+    } while (0)
+
+struct SizeClass {
+    int fftlen;
+    std::vector<int> harms;
+    int *d_harms = nullptr;
+};
+
+// create_accelobs (accel_utils.c:1636-2037) + the harmonic shapes of
+// create_subharminfos (:236-282) + the rstep list of accelsearch.c:183-188.
+int compute_geometry(const PzlConfig &c, PzlGeometry &g, std::vector<PzlHarm> *harms_out,
+                     std::vector<double> *startr_out);
+
+struct Plan {
+    PzlConfig cfg{};
+    PzlGeometry g{};
+    std::vector<PzlHarm> harms;
+    std::vector<double> startr;
+    std::vector<SizeClass> classes;
+    std::vector<uint16_t> zinds;  // [nh][numz]
+    std::vector<int> winds;       // [nh][numw]
+    int fnumw = 1, fnumz = 1, fzlo = 0, fwlo = 0;
+    int max_fftlen = 0;
+    int int_rlo = 1;  // every startr is an integer (integer r maps on the GPU)
+    int B = 8;
+    int device = 0;
+    hipStream_t st = nullptr;   // search stream
+    hipStream_t st2 = nullptr;  // hit ordering stream
+    // resident
+    PzlHarm *d_harms = nullptr;
+    float2 *d_bank = nullptr;
+    double *d_startr = nullptr;
+    double numindep_d[6] = {0, 0, 0, 0, 0, 0};
+    uint16_t *d_zinds = nullptr;
+    int *d_winds = nullptr;
+    const float2 *d_fft = nullptr;
+    float2 *d_fft_own = nullptr;
+    long long fft_bins = 0;
+    // per batch
+    float2 *d_segs = nullptr;
+    long long seg_stride = 0;
+    uint16_t *d_rinds = nullptr;
+    PzlHrTab *d_hr = nullptr;
+    float *d_planes = nullptr;
+    long long plane_stride = 0;
+    // hits: a ring of unordered slots (one per batch in flight), one ordered copy
+    static constexpr int kSlots = 4;
+    int nslots = 2;
+    long long hitcap = 0;
+    int key_bits = 64;
+    long long *d_hkeys[kSlots] = {};
+    float *d_hpows[kSlots] = {};
+    unsigned long long *d_counts = nullptr;  // [slot][count, spare]
+    unsigned long long *h_counts = nullptr;  // pinned mirror
+    long long *d_skeys = nullptr;
+    float *d_spows = nullptr;
+    PzlHostHit *d_recs = nullptr;
+    void *d_sort_temp = nullptr;
+    size_t sort_temp_bytes = 0;
+    PzlHostHit *h_recs = nullptr;  // pinned
+    long long h_recs_cap = 0;
+    hipEvent_t ev_count[kSlots] = {};
+    hipEvent_t ev_free[kSlots] = {};
+    float *d_sums = nullptr;
+    // results of the last search
+    std::vector<PzlHostHit> last_hits;
+    bool last_hits_complete = true;
+    long long keep_hits_limit = 1LL << 22;
+};
+
+}  // namespace pzl
