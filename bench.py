@@ -48,6 +48,30 @@ def cpu_baseline(fft, kw, rsteps, nthreads, cells_per_rstep, start):
                       "spectrum (%.1f s), kernel-bank build excluded as on the GPU" % (start, start + rsteps, dt)}
 
 
+def copy_calibrate(mib):
+    """A plain copy kernel of known size: under rocprofv3 --pmc its FETCH_SIZE
+    and WRITE_SIZE per dispatch give the counters' scale (bytes moved / KiB read)."""
+    import torch
+    n = mib * 2 ** 20 // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "dispatches": 5}))
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of 'kernel' at this workload from the committed
+    rocprofv3 --pmc measurement (FETCH_SIZE + WRITE_SIZE, copy-calibrated)."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            t = json.load(f)
+        return t.get(kernel, {}).get("bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -59,7 +83,14 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain", action="store_true", help="also time one-rstep-per-launch (plain GPU form)")
+    ap.add_argument("--rsteps", type=int, default=0,
+                    help="profiling runs only: search this many rsteps from the middle of the band")
+    ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
+                    help="run a plain device copy of MIB MiB five times and exit (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibrate:
+        copy_calibrate(args.copy_calibrate)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each rstep and each file is "
               "independent; see DESIGN.md for how the search shards).", file=sys.stderr)
@@ -74,14 +105,18 @@ def main():
     plan = B.Plan(B.make_config(batchsize=args.batchsize, **kw))
     plan.set_spectrum(fft)
     g = plan.geom
-    for _ in range(args.warmup):
-        plan.search(stats=False)
     lat, stats_all = [], []
+    lo, hi = 0, -1
+    if args.rsteps:
+        lo = g.nrsteps // 2
+        hi = lo + args.rsteps
+    for _ in range(args.warmup):
+        plan.search(lo, hi, stats=False)
     hip_sync()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         ts = time.perf_counter()
-        cands, st = plan.search(stats=True)
+        cands, st = plan.search(lo, hi, stats=True)
         lat.append((time.perf_counter() - ts) * 1e3)
         stats_all.append(st)
     hip_sync()
@@ -116,13 +151,14 @@ def main():
         "data": "synthetic: Gaussian noise + 3 jerked sinusoids (seeded), realfft-packed complex64",
         "config": {"workload": "plain GPU drop-in at the production shape: N=2^%d, dt=1e-4 s, zmax=200, "
                                "wmax=300, numharm=16, sigma=5, batchsize=%d" % (args.log2n, args.batchsize),
-                   "nrsteps": g.nrsteps, "corr_uselen": g.corr_uselen, "numz": g.numz, "numw": g.numw,
+                   "nrsteps": g.nrsteps if not args.rsteps else args.rsteps, "corr_uselen": g.corr_uselen, "numz": g.numz, "numw": g.numw,
                    "cells_per_search": cells, "bank_MiB": g.bank_bytes / 2 ** 20,
                    "planes_MiB_per_rstep": g.plane_bytes_per_rstep / 2 ** 20},
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                        "samples": len(lat)},
         "roofline": {"bound": "hbm", "kernel": "k_corr<%d>" % dom_len, "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic("k_corr<%d>" % dom_len),
                      "avg_launch_ms": avg_ms, "alg_bytes_per_launch": bytes_per_launch},
         "search_roofline": {"bytes_alg_per_search": stats_all[0]["bytes_alg"],
                             "achieved_GBs": stats_all[0]["bytes_alg"] * args.steps / total / 1e9,

@@ -92,6 +92,14 @@ typedef struct PzlStats {
 const char *pzl_last_error(void);
 int pzl_device_count(int *count);
 
+/* Kernel banks are cached per (device, harmonic shapes) and shared by plans;
+ * unreferenced banks are evicted oldest-first above the byte budget
+ * (default 64 GiB).  _clear drops every unreferenced bank. */
+int pzl_bank_cache_set_budget(long long bytes);
+int pzl_bank_cache_clear(void);
+/* 1 if creating this plan built its bank, 0 if it came from the cache. */
+int pzl_plan_bank_built(void *plan);
+
 /* Geometry and the kernel bank on 'device'; the bank stays in HBM until
  * pzl_plan_destroy.  accel_utils.c:1636-2037 + create_subharminfos :236-282 */
 int pzl_plan_create(const PzlConfig *cfg, int device, void **plan);
@@ -112,6 +120,9 @@ int pzl_set_spectrum_device(void *plan, const void *dev_fft, long long numbins);
 int pzl_search(void *plan, long long rstep_lo, long long rstep_hi, PzlCand *cands, long long maxcands,
                long long *ncands, PzlStats *stats);
 
+/* The candidate list of the last pzl_search (the same as it returned). */
+int pzl_get_cands(void *plan, PzlCand *cands, long long maxcands, long long *ncands);
+
 /* Hits of the last pzl_search in PRESTO C scan order: key encodes
  * (((rstep*stages + stage)*numw + w)*numz + z)*corr_uselen + r. */
 int pzl_get_hits(void *plan, long long *keys, float *powers, long long maxhits, long long *nhits);
@@ -122,6 +133,10 @@ int pzl_get_hits(void *plan, long long *keys, float *powers, long long maxhits, 
 int pzl_get_kernel(void *plan, int idx, int row, float *out);
 int pzl_compute_plane(void *plan, long long rstep, int idx, float *out, long long maxfloats, int *numrs,
                       long long *rlo);
+/* The same at an arbitrary fundamental startr (subharm_fderivs_vol's
+ * fullrlo; fullrhi = fullrlo + corr_uselen/2 - 0.5). */
+int pzl_compute_plane_at(void *plan, double fullrlo, int idx, float *out, long long maxfloats, int *numrs,
+                         long long *rlo);
 /* Fundamental plane of 'rstep' after all harmonic stages were added. */
 int pzl_compute_sums(void *plan, long long rstep, float *out, long long maxfloats);
 

@@ -286,3 +286,39 @@ def insert_cands(rr, sigma, power, numharm, z, w):
     k = lib().oracle_insert_cands(dptr(rr), fptr(sigma), fptr(power), numharm.ctypes.data_as(C.POINTER(C.c_int)),
                                   dptr(z), dptr(w), n, out, cap)
     return [(out[i].power, out[i].sigma, out[i].numharm, out[i].r, out[i].z, out[i].w) for i in range(k)]
+
+
+def sort_cands(cands):
+    """sort_accelcands (accel_utils.c:361-380): decreasing sigma, ties by
+    decreasing power; g_slist_sort is a stable merge sort."""
+    return sorted(cands, key=lambda c: (-c[1], -c[0]))
+
+
+# eliminate_harmonics (accel_utils.c:482-571): the ratios p/q, besides the
+# integer multiples and fractions 1..16, for which a later candidate counts as
+# harmonically related to an earlier one (|r_earlier * p / q - r_later| < 1.5).
+HARMONIC_RATIOS = ((3, 2), (5, 2), (2, 3), (4, 3), (5, 3), (3, 4), (5, 4), (2, 5), (3, 5), (4, 5), (5, 6),
+                   (2, 7), (3, 7), (4, 7), (3, 8), (5, 8), (2, 9), (3, 10), (2, 11), (3, 11), (2, 13),
+                   (3, 13), (2, 15))
+
+
+def harmonically_related(r_cur, r_other, tooclose=1.5, maxharm=16):
+    for ii in range(1, maxharm + 1):
+        if abs(r_cur / ii - r_other) < tooclose:
+            return True
+    for ii in range(1, maxharm + 1):
+        if abs(r_cur * ii - r_other) < tooclose:
+            return True
+    return any(abs(r_cur * p / q - r_other) < tooclose for p, q in HARMONIC_RATIOS)
+
+
+def eliminate_harmonics(cands):
+    """eliminate_harmonics (accel_utils.c:482-571) over a sorted list: for each
+    surviving candidate in order, drop every later one related to it."""
+    out = list(cands)
+    i = 0
+    while i < len(out) - 1:
+        cur = out[i][3]
+        out = out[: i + 1] + [c for c in out[i + 1:] if not harmonically_related(cur, c[3])]
+        i += 1
+    return out

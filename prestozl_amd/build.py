@@ -1,6 +1,6 @@
 """Build step for libaccelsearch_lib.so (gfx950) and the test oracle.
 
-    python -m prestozl_amd.build            # library + oracle (+ oracle/_ref)
+    python -m prestozl_amd.build            # library + driver + oracle (+ oracle/_ref)
 
 hipcc compiles the HIP kernels and the host C++ into one shared library in
 prestozl_amd/lib/.  Packed FP32 instructions are disabled for the device code
@@ -19,7 +19,10 @@ ARCH = os.environ.get("PZL_ARCH", "gfx950")
 
 SOURCES = ["pzl_kernels.hip", "pzl_engine.cpp", "pzl_geometry.cpp", "pzl_stats.cpp", "accel_api.cpp"]
 GLIB_INC = ["/opt/conda/include/glib-2.0", "/opt/conda/lib/glib-2.0/include"]
-GLIB_LIB = "/opt/conda/lib"
+# Headers from the conda tree; the library is the system's libglib-2.0.so.0
+# (found by the loader without an rpath, so no other library of the conda
+# tree, such as its older libstdc++, shadows the system's).
+GLIB_LINK = ["-l:libglib-2.0.so.0"]
 
 
 def _run(cmd, cwd=None):
@@ -44,7 +47,25 @@ def build_library(out=None):
         cmd += ["-DPZL_HAVE_GLIB=1"] + ["-I" + d for d in GLIB_INC]
     cmd += srcs + ["-o", out + ".tmp"]
     if have_glib:
-        cmd += ["-L" + GLIB_LIB, "-Wl,-rpath," + GLIB_LIB, "-lglib-2.0"]
+        cmd += GLIB_LINK
+    _run(cmd)
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_driver(lib=None):
+    """The accelsearch command-line driver (accel.h API), linked to the library."""
+    lib = lib or os.path.join(LIBDIR, "libaccelsearch_lib.so")
+    out = os.path.join(LIBDIR, "accelsearch")
+    if not all(os.path.isdir(d) for d in GLIB_INC):
+        print("glib-2.0 headers not found: accelsearch driver not built", flush=True)
+        return None
+    cmd = ["gcc", "-O2", "-I" + os.path.join(ROOT, "include")] + ["-I" + d for d in GLIB_INC]
+    cmd += [os.path.join(CSRC, "accelsearch_main.c"), "-o", out + ".tmp",
+            "-L" + LIBDIR, "-l:" + os.path.basename(lib), "-Wl,-rpath,$ORIGIN",
+            # the HIP runtime needs a newer libstdc++ than gcc 11 links against;
+            # the system's, found at load time, has it
+            "-Wl,--allow-shlib-undefined"] + GLIB_LINK
     _run(cmd)
     os.replace(out + ".tmp", out)
     return out
@@ -58,6 +79,7 @@ def build_oracle():
 
 def build_all():
     build_library()
+    build_driver()
     build_oracle()
 
 

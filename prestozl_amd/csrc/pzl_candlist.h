@@ -8,11 +8,11 @@
 // overwrote the previous node, removes the next one when that is also close and
 // weaker); otherwise it is inserted as a new node.
 //
-// The list is held in a vector.  While it is non-decreasing in r (every
-// operation preserves that except one rare overwrite), the head walk is a
-// lower_bound, and consecutive hits of one plane row move the search position
-// by at most a node, so a cursor makes the typical hit O(1).  When an
-// overwrite leaves the vector out of order the walk is done literally.
+// The list is held in a vector.  The head walk is a binary search over the
+// running maximum of r (equal to r itself while the list is sorted, which
+// every operation preserves except one rare overwrite at the tail), and
+// consecutive hits of one plane row move the search position by at most a
+// node, so a cursor makes the typical hit O(1).
 #pragma once
 
 #include <algorithm>
@@ -35,7 +35,7 @@ class CandList {
         const size_t n = v_.size();
         if (n == 0) {
             v_.push_back(decode());
-            pos_valid_ = false;
+            touch(0);
             return;
         }
         size_t tmp = find(rr);
@@ -66,58 +66,49 @@ class CandList {
     const std::vector<PzlCand> &items() const { return v_; }
 
   private:
-    // first index with r >= rr (walking as the reference does), n if none
+    // First index with r >= rr (the reference's walk from the head), n if
+    // none.  The walk stops at the first node whose r reaches rr, which is the
+    // first index where the running maximum of r reaches rr; that running
+    // maximum is non-decreasing even when an overwrite left the list out of
+    // order, so a binary search (or the cursor) finds it.
     size_t find(double rr)
     {
         const size_t n = v_.size();
-        if (inversions_ == 0) {
-            if (pos_valid_ && pos_ <= n && (pos_ == 0 || v_[pos_ - 1].r < rr) && (pos_ == n || v_[pos_].r >= rr))
-                return pos_;
-            if (pos_valid_ && pos_ < n && v_[pos_].r < rr && (pos_ + 1 == n || v_[pos_ + 1].r >= rr)) {
-                pos_ = pos_ + 1;
-                return pos_;
-            }
-            pos_ = std::lower_bound(v_.begin(), v_.end(), rr, [](const PzlCand &a, double x) { return a.r < x; }) -
-                   v_.begin();
-            pos_valid_ = true;
-            return pos_;
-        }
-        size_t t = 0;
-        while (t + 1 < n && v_[t].r < rr) t++;
-        if (t + 1 == n && v_[t].r < rr) return n;
-        return t;
+        refresh();
+        const double *m = pm_.data();
+        if (pos_ <= n && (pos_ == 0 || m[pos_ - 1] < rr) && (pos_ == n || m[pos_] >= rr)) return pos_;
+        if (pos_ < n && m[pos_] < rr && (pos_ + 1 == n || m[pos_ + 1] >= rr)) return ++pos_;
+        pos_ = std::lower_bound(m, m + n, rr) - m;
+        return pos_;
     }
-    int inv(size_t i) const { return (i + 1 < v_.size() && v_[i].r > v_[i + 1].r) ? 1 : 0; }
+    void refresh()
+    {
+        const size_t n = v_.size();
+        if (pm_.size() != n) pm_.resize(n);
+        for (size_t t = dirty_; t < n; t++) pm_[t] = t ? std::max(pm_[t - 1], v_[t].r) : v_[t].r;
+        dirty_ = n;
+    }
+    void touch(size_t i) { dirty_ = std::min(dirty_, i); }
     void set(size_t i, const PzlCand &c)
     {
-        if (i > 0) inversions_ -= inv(i - 1);
-        inversions_ -= inv(i);
         v_[i] = c;
-        if (i > 0) inversions_ += inv(i - 1);
-        inversions_ += inv(i);
-        pos_valid_ = false;
+        touch(i);
     }
     void erase(size_t i)
     {
-        if (i > 0) inversions_ -= inv(i - 1);
-        inversions_ -= inv(i);
         v_.erase(v_.begin() + i);
-        if (i > 0) inversions_ += inv(i - 1);
-        pos_valid_ = false;
+        touch(i);
     }
     void insert_at(size_t i, const PzlCand &c)
     {
-        if (i > 0) inversions_ -= inv(i - 1);
         v_.insert(v_.begin() + i, c);
-        if (i > 0) inversions_ += inv(i - 1);
-        inversions_ += inv(i);
-        pos_valid_ = false;
+        touch(i);
     }
 
     std::vector<PzlCand> v_;
-    long long inversions_ = 0;
-    size_t pos_ = 0;
-    bool pos_valid_ = false;
+    std::vector<double> pm_;  // running maximum of v_[..].r
+    size_t dirty_ = 0;        // pm_ is valid below this index
+    size_t pos_ = 0;          // last find() result (a hint)
 };
 
 }  // namespace pzl

@@ -18,10 +18,14 @@
 
 #include <algorithm>
 #include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <thread>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -36,7 +40,93 @@ using namespace pzl;
 
 namespace {
 
-const long long kDefaultHitCap = 1LL << 26;  // hits per batch slot (12 bytes each, twice)
+const long long kDefaultHitCap = 1LL << 26;  // hits per batch slot (12 bytes each)
+
+// Kernel banks are shared by every plan with the same harmonic shapes on a
+// device (files of one (zmax, wmax, numharm) class, DM trials, served
+// requests) and stay resident until evicted: unreferenced banks are dropped
+// oldest-first once the cache exceeds its byte budget.
+struct BankEntry {
+    float2 *ptr = nullptr;
+    long long bytes = 0;
+    int refs = 0;
+    unsigned long long last_use = 0;
+};
+std::mutex g_bank_mu;
+std::map<std::vector<long long>, BankEntry> g_banks;
+unsigned long long g_bank_clock = 0;
+long long g_bank_budget = 64LL << 30;
+
+std::vector<long long> bank_key(const Plan &p)
+{
+    std::vector<long long> k{p.device};
+    for (auto &H : p.harms) {
+        k.push_back(H.numharm);
+        k.push_back(H.harmnum);
+        k.push_back(H.zmax);
+        k.push_back(H.wmax);
+        k.push_back(H.fftlen);
+    }
+    return k;
+}
+
+void release_bank(Plan *p)
+{
+    if (!p->d_bank) return;
+    std::lock_guard<std::mutex> lk(g_bank_mu);
+    auto it = g_banks.find(bank_key(*p));
+    if (it != g_banks.end() && it->second.ptr == p->d_bank) it->second.refs--;
+    p->d_bank = nullptr;
+}
+
+void evict_banks_locked(long long incoming)
+{
+    long long total = incoming;
+    for (auto &kv : g_banks) total += kv.second.bytes;
+    while (total > g_bank_budget) {
+        auto victim = g_banks.end();
+        for (auto it = g_banks.begin(); it != g_banks.end(); ++it)
+            if (it->second.refs == 0 && (victim == g_banks.end() || it->second.last_use < victim->second.last_use))
+                victim = it;
+        if (victim == g_banks.end()) break;
+        (void)hipSetDevice((int)victim->first[0]);
+        (void)hipFree(victim->second.ptr);
+        total -= victim->second.bytes;
+        g_banks.erase(victim);
+    }
+}
+
+// The bank of this plan's shape: from the cache, or built on the GPU now.
+int acquire_bank(Plan *p, bool *built)
+{
+    std::lock_guard<std::mutex> lk(g_bank_mu);
+    const auto key = bank_key(*p);
+    auto it = g_banks.find(key);
+    *built = false;
+    if (it != g_banks.end()) {
+        it->second.refs++;
+        it->second.last_use = ++g_bank_clock;
+        p->d_bank = it->second.ptr;
+        return 0;
+    }
+    evict_banks_locked(p->g.bank_bytes);
+    BankEntry e;
+    e.bytes = p->g.bank_bytes;
+    PZL_HIPCHK(hipMalloc(&e.ptr, e.bytes));
+    hipError_t err = pzl_launch_bank_build(p->d_harms, (int)p->harms.size(), p->g.numharm_kernels, p->max_fftlen,
+                                           e.ptr, p->st);
+    if (err == hipSuccess) err = hipStreamSynchronize(p->st);
+    if (err != hipSuccess) {
+        (void)hipFree(e.ptr);
+        return fail(PZL_ERR_HIP, "kernel bank build failed: %s", hipGetErrorString(err));
+    }
+    e.refs = 1;
+    e.last_use = ++g_bank_clock;
+    g_banks[key] = e;
+    p->d_bank = e.ptr;
+    *built = true;
+    return 0;
+}
 
 void free_plan(Plan *p)
 {
@@ -46,7 +136,7 @@ void free_plan(Plan *p)
     if (p->st2) (void)hipStreamSynchronize(p->st2);
     for (auto &c : p->classes) (void)hipFree(c.d_harms);
     (void)hipFree(p->d_harms);
-    (void)hipFree(p->d_bank);
+    release_bank(p);
     (void)hipFree(p->d_startr);
     (void)hipFree(p->d_zinds);
     (void)hipFree(p->d_winds);
@@ -67,7 +157,8 @@ void free_plan(Plan *p)
     (void)hipFree(p->d_spows);
     (void)hipFree(p->d_recs);
     (void)hipFree(p->d_sort_temp);
-    if (p->h_recs) (void)hipHostFree(p->h_recs);
+    for (int s = 0; s < Plan::kSlots; s++)
+        if (p->h_recs[s]) (void)hipHostFree(p->h_recs[s]);
     (void)hipFree(p->d_sums);
     if (p->st) (void)hipStreamDestroy(p->st);
     if (p->st2) (void)hipStreamDestroy(p->st2);
@@ -124,7 +215,7 @@ struct Scope {
 
 // Segments, correlations and the harmonic sum of rsteps [r0, r0 + nb) into
 // hit slot 'slot'.
-int launch_batch(Plan &p, long long r0, int nb, int slot, float *d_sums, Timers *tm)
+int launch_batch(Plan &p, long long r0, int nb, int slot, float *d_sums, Timers *tm, bool with_sum = true)
 {
     const int nh = (int)p.harms.size();
     PzlBatchArgs ba{};
@@ -166,6 +257,7 @@ int launch_batch(Plan &p, long long r0, int nb, int slot, float *d_sums, Timers 
         Scope s(tm, 1, c.fftlen, p.st);
         PZL_HIPCHK(pzl_launch_corr(c.fftlen, &ca, 256 * 16, p.st));
     }
+    if (!with_sum) return 0;
     PzlSumArgs sa{};
     sa.harms = p.d_harms;
     sa.nh = nh;
@@ -196,14 +288,98 @@ int launch_batch(Plan &p, long long r0, int nb, int slot, float *d_sums, Timers 
     return 0;
 }
 
+// Candidate insertion runs on a thread of its own so that the launching
+// thread keeps the GPU queue full.  Jobs are whole slots, applied in batch
+// order (the scan order of the reference's insertion).
 struct SearchState {
+    Plan *p = nullptr;
     CandList cl;
-    double ms_host = 0;
+    double ms_host = 0;  // insertion thread busy time
     long long nhits = 0;
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<std::pair<int, long long>> jobs;
+    bool busy[Plan::kSlots] = {};
+    bool stop = false;
+    std::thread th;
+
+    explicit SearchState(Plan *plan) : p(plan) { th = std::thread([this] { run(); }); }
+    ~SearchState() { finish(); }
+    void finish()
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            stop = true;
+        }
+        cv.notify_all();
+        if (th.joinable()) th.join();
+    }
+    void submit(int slot, long long n)
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            busy[slot] = true;
+            jobs.emplace_back(slot, n);
+        }
+        cv.notify_all();
+    }
+    // until the insertion thread no longer reads this slot's host records
+    void wait_slot(int slot)
+    {
+        std::unique_lock<std::mutex> lk(mu);
+        cv.wait(lk, [&] { return !busy[slot]; });
+    }
+    void run()
+    {
+        for (;;) {
+            std::pair<int, long long> j;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv.wait(lk, [&] { return stop || !jobs.empty(); });
+                if (jobs.empty()) return;
+                j = jobs.front();
+                jobs.pop_front();
+            }
+            insert_hits(p->h_recs[j.first], j.second);
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                busy[j.first] = false;
+            }
+            cv.notify_all();
+        }
+    }
+    void insert_hits(const PzlHostHit *recs, long long n)
+    {
+        const auto t0 = std::chrono::steady_clock::now();
+        const double fzlo = p->fzlo, fwlo = p->fwlo;
+        for (long long i = 0; i < n; i++) {
+            const PzlHostHit &h = recs[i];
+            cl.insert(h.rr, h.sigma, [&]() {
+                const int numharm = 1 << h.stage;
+                PzlCand c;
+                c.power = h.power;
+                c.sigma = h.sigma;
+                c.numharm = numharm;
+                c.r = h.rr;
+                c.z = (fzlo + h.zi * (double)PZL_DZ) / (double)numharm;
+                c.w = (fwlo + h.wi * (double)PZL_DW) / (double)numharm;
+                return c;
+            });
+        }
+        if (p->last_hits_complete) {
+            if ((long long)p->last_hits.size() + n <= p->keep_hits_limit) {
+                p->last_hits.insert(p->last_hits.end(), recs, recs + n);
+            } else {
+                p->last_hits.clear();
+                p->last_hits_complete = false;
+            }
+        }
+        ms_host += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
 };
 
-// Order the hits of one slot on the GPU, bring them over and apply them to
-// the candidate list (the caller has waited for the slot's count).
+// Order the hits of one slot on the GPU, bring them over and queue them for
+// the insertion thread (the caller has waited for the slot's count).
 int apply_slot(Plan &p, int slot, long long n, SearchState &ss)
 {
     if (n <= 0) return 0;
@@ -211,40 +387,19 @@ int apply_slot(Plan &p, int slot, long long n, SearchState &ss)
                              p.d_spows, n, p.key_bits, p.st2));
     PZL_HIPCHK(pzl_launch_hitprep(p.d_skeys, p.d_spows, n, p.d_startr, p.g.corr_uselen, p.fnumz, p.fnumw,
                                   p.g.numharmstages, p.numindep_d, p.d_recs, p.st2));
-    if (n > p.h_recs_cap) {
-        if (p.h_recs) PZL_HIPCHK(hipHostFree(p.h_recs));
-        p.h_recs = nullptr;
+    ss.wait_slot(slot);
+    if (n > p.h_recs_cap[slot]) {
+        if (p.h_recs[slot]) PZL_HIPCHK(hipHostFree(p.h_recs[slot]));
+        p.h_recs[slot] = nullptr;
+        p.h_recs_cap[slot] = 0;
         const long long cap = std::min<long long>(std::max<long long>(2 * n, 1 << 20), p.hitcap);
-        PZL_HIPCHK(hipHostMalloc((void **)&p.h_recs, sizeof(PzlHostHit) * cap, hipHostMallocDefault));
-        p.h_recs_cap = cap;
+        PZL_HIPCHK(hipHostMalloc((void **)&p.h_recs[slot], sizeof(PzlHostHit) * cap, hipHostMallocDefault));
+        p.h_recs_cap[slot] = cap;
     }
-    PZL_HIPCHK(hipMemcpyAsync(p.h_recs, p.d_recs, sizeof(PzlHostHit) * n, hipMemcpyDeviceToHost, p.st2));
+    PZL_HIPCHK(hipMemcpyAsync(p.h_recs[slot], p.d_recs, sizeof(PzlHostHit) * n, hipMemcpyDeviceToHost, p.st2));
     PZL_HIPCHK(hipStreamSynchronize(p.st2));
-    const auto t0 = std::chrono::steady_clock::now();
-    for (long long i = 0; i < n; i++) {
-        const PzlHostHit &h = p.h_recs[i];
-        ss.cl.insert(h.rr, h.sigma, [&]() {
-            const int numharm = 1 << h.stage;
-            PzlCand c;
-            c.power = h.power;
-            c.sigma = h.sigma;
-            c.numharm = numharm;
-            c.r = h.rr;
-            c.z = (p.fzlo + h.zi * (double)PZL_DZ) / (double)numharm;
-            c.w = (p.fwlo + h.wi * (double)PZL_DW) / (double)numharm;
-            return c;
-        });
-    }
-    if (p.last_hits_complete) {
-        if ((long long)p.last_hits.size() + n <= p.keep_hits_limit) {
-            p.last_hits.insert(p.last_hits.end(), p.h_recs, p.h_recs + n);
-        } else {
-            p.last_hits.clear();
-            p.last_hits_complete = false;
-        }
-    }
-    ss.ms_host += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     ss.nhits += n;
+    ss.submit(slot, n);
     return 0;
 }
 
@@ -301,6 +456,27 @@ double plane_alg_bytes(const Plan &p)
 extern "C" {
 
 const char *pzl_last_error(void) { return g_err.c_str(); }
+
+int pzl_bank_cache_set_budget(long long bytes)
+{
+    if (bytes < 0) return fail(PZL_ERR_ARG, "negative budget");
+    std::lock_guard<std::mutex> lk(g_bank_mu);
+    g_bank_budget = bytes;
+    evict_banks_locked(0);
+    return 0;
+}
+
+int pzl_bank_cache_clear(void)
+{
+    std::lock_guard<std::mutex> lk(g_bank_mu);
+    const long long keep = g_bank_budget;
+    g_bank_budget = 0;
+    evict_banks_locked(0);
+    g_bank_budget = keep;
+    return 0;
+}
+
+int pzl_plan_bank_built(void *plan) { return plan ? (int)((Plan *)plan)->bank_built : 0; }
 
 int pzl_device_count(int *count)
 {
@@ -448,8 +624,8 @@ int pzl_plan_create(const PzlConfig *cfg, int device, void **plan_out)
         PCHK(hipMalloc(&c.d_harms, sizeof(int) * c.harms.size()));
         PCHK(hipMemcpy(c.d_harms, c.harms.data(), sizeof(int) * c.harms.size(), hipMemcpyHostToDevice));
     }
-    PCHK(hipMalloc(&p->d_bank, p->g.bank_bytes));
-    const size_t nstart = std::max<size_t>(p->startr.size(), 1);
+    // one spare entry after the rsteps: planes at an arbitrary startr
+    const size_t nstart = p->startr.size() + 1;
     PCHK(hipMalloc(&p->d_startr, sizeof(double) * nstart));
     if (!p->startr.empty())
         PCHK(hipMemcpy(p->d_startr, p->startr.data(), sizeof(double) * p->startr.size(), hipMemcpyHostToDevice));
@@ -478,9 +654,13 @@ int pzl_plan_create(const PzlConfig *cfg, int device, void **plan_out)
     PCHK(pzl_sort_hits(nullptr, &p->sort_temp_bytes, p->d_hkeys[0], p->d_skeys, p->d_hpows[0], p->d_spows,
                        p->hitcap, p->key_bits, p->st2));
     PCHK(hipMalloc(&p->d_sort_temp, std::max<size_t>(p->sort_temp_bytes, 16)));
-    // The kernel bank, built once and kept resident.
-    PCHK(pzl_launch_bank_build(p->d_harms, nh, p->g.numharm_kernels, p->max_fftlen, p->d_bank, p->st));
-    PCHK(hipStreamSynchronize(p->st));
+    // The kernel bank: built once per shape and kept resident.
+    {
+        bool built = false;
+        int brc = acquire_bank(p, &built);
+        if (brc) return cleanup(brc);
+        p->bank_built = built;
+    }
     p->g.device_bytes = p->g.bank_bytes + (long long)sizeof(float) * p->plane_stride * p->B +
                         (long long)sizeof(float2) * p->seg_stride * p->B +
                         (long long)(p->nslots * 12 + 12 + (long long)sizeof(PzlHostHit)) * p->hitcap +
@@ -556,9 +736,9 @@ int pzl_search(void *plan, long long rstep_lo, long long rstep_hi, PzlCand *cand
     if (rstep_hi < 0 || rstep_hi > nr) rstep_hi = nr;
     Timers tm;
     tm.on = (stats != nullptr);
-    SearchState ss;
     p->last_hits.clear();
     p->last_hits_complete = true;
+    SearchState ss(p);
     hipEvent_t ev_start = nullptr, ev_end = nullptr;
     if (tm.on) {
         ev_start = tm.get();
@@ -607,7 +787,9 @@ int pzl_search(void *plan, long long rstep_lo, long long rstep_hi, PzlCand *cand
     }
     PZL_HIPCHK(hipStreamSynchronize(p->st));
     PZL_HIPCHK(hipStreamSynchronize(p->st2));
-    const auto &items = ss.cl.items();
+    ss.finish();
+    p->last_cands = ss.cl.items();
+    const auto &items = p->last_cands;
     if (ncands) *ncands = (long long)items.size();
     if (cands)
         for (long long k = 0; k < (long long)items.size() && k < maxcands; k++) cands[k] = items[k];
@@ -665,6 +847,15 @@ int pzl_search(void *plan, long long rstep_lo, long long rstep_hi, PzlCand *cand
     return 0;
 }
 
+int pzl_get_cands(void *plan, PzlCand *cands, long long maxcands, long long *ncands)
+{
+    Plan *p = (Plan *)plan;
+    if (!p) return fail(PZL_ERR_ARG, "null plan");
+    if (ncands) *ncands = (long long)p->last_cands.size();
+    for (long long k = 0; k < (long long)p->last_cands.size() && k < maxcands; k++) cands[k] = p->last_cands[k];
+    return 0;
+}
+
 int pzl_get_hits(void *plan, long long *keys, float *powers, long long maxhits, long long *nhits)
 {
     Plan *p = (Plan *)plan;
@@ -717,6 +908,33 @@ int pzl_compute_plane(void *plan, long long rstep, int idx, float *out, long lon
                                hipMemcpyDeviceToHost));
     }
     PZL_HIPCHK(hipMemset(p->d_counts, 0, 4 * sizeof(unsigned long long)));
+    return 0;
+}
+
+int pzl_compute_plane_at(void *plan, double fullrlo, int idx, float *out, long long maxfloats, int *numrs,
+                         long long *rlo)
+{
+    Plan *p = (Plan *)plan;
+    if (!p || idx < 0 || idx >= (int)p->harms.size()) return fail(PZL_ERR_ARG, "bad plane request");
+    if (!p->d_fft) return fail(PZL_ERR_NOSPECTRUM, "no spectrum set");
+    PZL_HIPCHK(hipSetDevice(p->device));
+    const long long spare = p->g.nrsteps;
+    PZL_HIPCHK(hipMemcpyAsync(p->d_startr + spare, &fullrlo, sizeof(double), hipMemcpyHostToDevice, p->st));
+    int rc = launch_batch(*p, spare, 1, 0, nullptr, nullptr, false);
+    if (rc) return rc;
+    PZL_HIPCHK(hipStreamSynchronize(p->st));
+    PzlHrTab hr;
+    PZL_HIPCHK(hipMemcpy(&hr, p->d_hr + idx, sizeof hr, hipMemcpyDeviceToHost));
+    const PzlHarm &H = p->harms[idx];
+    if (numrs) *numrs = hr.numrs;
+    if (rlo) *rlo = hr.rlo;
+    if (out) {
+        const long long need = (long long)H.nrows * hr.numrs;
+        if (need > maxfloats) return fail(PZL_ERR_ARG, "output too small (%lld floats needed)", need);
+        PZL_HIPCHK(hipMemcpy2D(out, sizeof(float) * hr.numrs, p->d_planes + H.plane_off,
+                               sizeof(float) * H.rs_stride, sizeof(float) * hr.numrs, H.nrows,
+                               hipMemcpyDeviceToHost));
+    }
     return 0;
 }
 

@@ -95,6 +95,11 @@ hipError_t pzl_launch_hitprep(const long long *keys, const float *pows, long lon
                               int corr_uselen, int numz, int numw, int nstages, const double *numindep,
                               PzlHostHit *out, hipStream_t st);
 int pzl_corr_supported(int n);
+hipError_t pzl_launch_add_plane(float *fund, int numws, int numzs, int numrs, const float *sub, int sub_numzs,
+                                int sub_numrs, const int *winds, const uint16_t *zinds, const uint16_t *rinds,
+                                hipStream_t st);
+hipError_t pzl_launch_threshold(const float *plane, long long n, float powcut, long long *idx,
+                                unsigned long long *count, long long cap, hipStream_t st);
 #ifdef __cplusplus
 }
 #endif

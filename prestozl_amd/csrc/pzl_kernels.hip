@@ -537,6 +537,65 @@ __global__ __launch_bounds__(256) void k_hitprep(HitPrepArgs a)
 }
 
 // ---------------------------------------------------------------------------
+// Plane utilities behind the per-plane accel.h calls (add_ffdotpows,
+// search_ffdotpows): fund[w][z][r] += sub[wind[w]][zind[z]][rind[r]]
+// (accel_utils.c:1356-1377) and threshold + ordered compaction
+// (accel_utils.c:1507-1555 scan order w, z, r).
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_add_plane(float *fund, int numws, int numzs, int numrs, const float *sub,
+                                                   int sub_numzs, int sub_numrs, const int *winds,
+                                                   const uint16_t *zinds, const uint16_t *rinds)
+{
+    const long long n = (long long)numws * numzs * numrs;
+    for (long long i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+        const int r = (int)(i % numrs);
+        const int z = (int)((i / numrs) % numzs);
+        const int w = (int)(i / ((long long)numrs * numzs));
+        fund[i] += sub[((long long)winds[w] * sub_numzs + zinds[z]) * sub_numrs + rinds[r]];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_threshold(const float *plane, long long n, float powcut, long long *idx,
+                                                   unsigned long long *count, long long cap)
+{
+    const unsigned long long lanemask_lt = (1ull << (__lane_id())) - 1ull;
+    for (long long base = blockIdx.x * 256ll; base < n; base += (long long)gridDim.x * 256) {
+        const long long i = base + threadIdx.x;
+        const bool hit = i < n && plane[i] > powcut;
+        const unsigned long long mask = __ballot(hit);
+        if (mask) {
+            const int leader = __ffsll((long long)mask) - 1;
+            unsigned long long slot0 = 0;
+            if ((int)__lane_id() == leader) slot0 = atomicAdd(count, (unsigned long long)__popcll(mask));
+            slot0 = __shfl(slot0, leader);
+            const unsigned long long slot = slot0 + __popcll(mask & lanemask_lt);
+            if (hit && (long long)slot < cap) idx[slot] = i;
+        }
+    }
+}
+
+extern "C" hipError_t pzl_launch_add_plane(float *fund, int numws, int numzs, int numrs, const float *sub,
+                                           int sub_numzs, int sub_numrs, const int *winds, const uint16_t *zinds,
+                                           const uint16_t *rinds, hipStream_t st)
+{
+    const long long n = (long long)numws * numzs * numrs;
+    const long long blocks = std::min<long long>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_add_plane, dim3((unsigned)std::max<long long>(blocks, 1)), dim3(256), 0, st, fund, numws,
+                       numzs, numrs, sub, sub_numzs, sub_numrs, winds, zinds, rinds);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t pzl_launch_threshold(const float *plane, long long n, float powcut, long long *idx,
+                                           unsigned long long *count, long long cap, hipStream_t st)
+{
+    const long long blocks = std::min<long long>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_threshold, dim3((unsigned)std::max<long long>(blocks, 1)), dim3(256), 0, st, plane, n,
+                       powcut, idx, count, cap);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // Launchers (C linkage for the host engine)
 // ---------------------------------------------------------------------------
 

@@ -50,7 +50,8 @@ struct Plan {
     hipStream_t st2 = nullptr;  // hit ordering stream
     // resident
     PzlHarm *d_harms = nullptr;
-    float2 *d_bank = nullptr;
+    float2 *d_bank = nullptr;  // shared, owned by the bank cache
+    bool bank_built = false;   // this plan's creation built the bank
     double *d_startr = nullptr;
     double numindep_d[6] = {0, 0, 0, 0, 0, 0};
     uint16_t *d_zinds = nullptr;
@@ -79,13 +80,14 @@ struct Plan {
     PzlHostHit *d_recs = nullptr;
     void *d_sort_temp = nullptr;
     size_t sort_temp_bytes = 0;
-    PzlHostHit *h_recs = nullptr;  // pinned
-    long long h_recs_cap = 0;
+    PzlHostHit *h_recs[kSlots] = {};  // pinned, one per slot (read by the insertion thread)
+    long long h_recs_cap[kSlots] = {};
     hipEvent_t ev_count[kSlots] = {};
     hipEvent_t ev_free[kSlots] = {};
     float *d_sums = nullptr;
     // results of the last search
     std::vector<PzlHostHit> last_hits;
+    std::vector<PzlCand> last_cands;
     bool last_hits_complete = true;
     long long keep_hits_limit = 1LL << 22;
 };

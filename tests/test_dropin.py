@@ -1,0 +1,296 @@
+"""The accel.h drop-in API (libaccelsearch_lib.so) through ctypes, as the
+reference's callers bind it (accelsearch_cu.c:22-34,
+bin/accelsearch_pipeline_cu.py:51-233).
+
+CPU tests: command-line parsing, create_accelobs / create_subharminfos
+against the oracle's geometry, the candidate-list operations against the
+oracle's restatement.  GPU tests: the PRESTO C per-rstep loop
+(accelsearch.c:183-223) over subharm_fderivs_vol / add_ffdotpows /
+search_ffdotpows, and the accelsearch command-line driver end to end.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import dropin as D
+import pyoracle as O
+from prestozl_amd import binding, synth
+
+DRIVER = os.path.join(os.path.dirname(binding.__file__), "lib", "accelsearch")
+
+
+def parse(args):
+    n, av = D.argv(["accelsearch"] + args)
+    return D.api().parseCmdline(n, av).contents, av
+
+
+def test_parse_defaults():
+    cmd, _av = parse(["x.fft"])
+    # defaults of accelsearch_cmd.c:20-96
+    assert (cmd.batchsizeP, cmd.batchsize) == (b"\x01", 8)
+    assert (cmd.numharmP, cmd.numharm) == (b"\x01", 8)
+    assert (cmd.zmaxP, cmd.zmax) == (b"\x01", 200)
+    assert (cmd.wmaxP, cmd.wmax) == (b"\x00", 0)
+    assert cmd.sigma == pytest.approx(2.0)
+    assert (cmd.floP, cmd.flo, cmd.fhiP, cmd.fhi) == (b"\x01", 1.0, b"\x01", 10000.0)
+    assert (cmd.rloP, cmd.rhiP) == (b"\x00", b"\x00")
+    assert cmd.argc == 1 and cmd.argv[0] == b"x.fft"
+    assert cmd.full_cmd_line == b"accelsearch x.fft"
+
+
+def test_parse_values():
+    cmd, _av = parse(["-zmax", "51", "-wmax", "150", "-numharm", "16", "-sigma", "5.0", "-batchsize", "12",
+                      "-rlo", "100", "-noharmremove", "-ncpus", "4", "f.fft"])
+    assert cmd.zmax == 51 and cmd.wmaxP == b"\x01" and cmd.wmax == 150
+    assert cmd.numharm == 16 and cmd.sigma == pytest.approx(5.0) and cmd.batchsize == 12
+    assert cmd.rloP == b"\x01" and cmd.rlo == 100.0 and cmd.floP == b"\x01"  # -flo stays on (default)
+    assert cmd.noharmremoveP == b"\x01" and cmd.ncpus == 4
+    assert cmd.argc == 1 and cmd.argv[0] == b"f.fft"
+
+
+@pytest.mark.parametrize("args", [["-numharm", "64"], ["-zmax", "1201"], ["-sigma", "0.5"], ["-wmax", "-20"],
+                                  ["-bogus"]])
+def test_parse_rejects(args):
+    r = subprocess.run([DRIVER] + args + ["x.fft"], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0
+    assert "option" in r.stdout
+
+
+def open_obs(root, args, usemmap=1):
+    cmd, av = parse(args + [root + ".fft"])
+    obs, idata = D.AccelObs(), D.InfoData()
+    D.api().create_accelobs(C.byref(obs), C.byref(idata), C.byref(cmd), usemmap)
+    return obs, idata, cmd, av
+
+
+GEOM_CASES = [
+    # (log2 N, dt, zmax, wmax, numharm, sigma)
+    (20, 64e-6, 20, 0, 4, 2.0),
+    (21, 1e-4, 200, 300, 16, 5.0),
+    (19, 6.4e-4, 200, 300, 8, 3.0),
+    (20, 1e-4, 50, None, 8, 2.0),
+    (21, 1e-4, 51, 150, 2, 6.0),
+]
+
+
+@pytest.mark.parametrize("case", GEOM_CASES)
+def test_create_accelobs_matches_oracle(tmp_path, case):
+    lg, dt, zmax, wmax, numharm, sigma = case
+    N = 1 << lg
+    fft = np.zeros(N // 2, dtype=np.complex64)
+    root = D.setup_files(tmp_path, fft, N, dt)
+    args = ["-zmax", str(zmax), "-numharm", str(numharm), "-sigma", str(sigma)]
+    if wmax is not None:
+        args += ["-wmax", str(wmax)]
+    obs, idata, cmd, _av = open_obs(root, args)
+    ref = O.create_accelobs(O.make_params(N, dt, N // 2, zmax, wmax or 0, numharm, sigma,
+                                          wmaxP=wmax is not None))
+    assert idata.N == N and idata.dt == pytest.approx(dt, rel=1e-15)
+    for f in ("N", "numbins", "highestbin", "maxkernlen", "corr_uselen", "fftlen", "numharmstages", "numz",
+              "numw"):
+        assert getattr(obs, f) == getattr(ref, f), f
+    for f in ("T", "rlo", "rhi", "zlo", "zhi", "wlo", "whi"):
+        assert getattr(obs, f) == getattr(ref, f), f
+    for s in range(obs.numharmstages):
+        assert obs.powcut[s] == ref.powcut[s]
+        assert obs.numindep[s] == ref.numindep[s]
+    assert obs.inmem == 0 and obs.dr == 0.5 and obs.numbetween == 2
+    assert obs.candnm.decode().startswith(root + "_ACCEL_")
+    # the .fft is mapped, not copied
+    assert obs.mmap_file > 0 and obs.fft[5].r == 0.0
+
+    shis = D.api().create_subharminfos(C.byref(obs))
+    for stage in range(obs.numharmstages):
+        harms = [1] if stage == 0 else list(range(1, 1 << stage, 2))
+        for j in harms:
+            sh = shis[stage][0 if stage == 0 else j - 1]
+            h = O.harm_info(ref, 1 << stage, j)
+            assert (sh.numharm, sh.harmnum, sh.zmax, sh.wmax) == (1 << stage, j, h.zmax, h.wmax)
+            assert sh.numkern_zdim == h.numz and sh.numkern_wdim == h.numw
+            assert sh.numkern == h.numz * h.numw
+            k0 = sh.kern[0][0]
+            assert (k0.z, k0.w) == (-h.zmax, -h.wmax)
+            assert k0.fftlen == h.fftlen and k0.kern_half_width == h.halfwidth
+            assert k0.numgoodbins == h.fftlen - 2 * 2 * h.halfwidth
+            kl = sh.kern[h.numw - 1][h.numz - 1]
+            assert (kl.z, kl.w) == (h.zmax, h.wmax)
+            assert not kl.data  # the FFT'd kernels live in GPU memory
+    D.api().free_subharminfos(C.byref(obs), shis)
+    D.api().free_accelobs(C.byref(obs))
+
+
+def test_get_fourier_amplitudes(tmp_path):
+    N = 1 << 14
+    fft = (np.arange(N // 2) + 1j * np.arange(N // 2) * 0.5).astype(np.complex64)
+    root = D.setup_files(tmp_path, fft, N, 1e-3)
+    obs, _i, _c, _av = open_obs(root, ["-zmax", "10", "-numharm", "2"])
+    for lobin, n in [(-40, 100), (0, 64), (1000, 512), (N // 2 - 30, 100), (N // 2 + 5, 10)]:
+        p = D.api().get_fourier_amplitudes(lobin, n, C.byref(obs))
+        got = np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_float)), shape=(2 * n,))
+        got = got[0::2] + 1j * got[1::2]
+        exp = np.zeros(n, dtype=np.complex64)
+        for i in range(n):
+            b = lobin + i
+            if 0 <= b < N // 2:
+                exp[i] = fft[b]
+        np.testing.assert_array_equal(got, exp)
+    D.api().free_accelobs(C.byref(obs))
+
+
+def random_cands(rng, n):
+    """Candidates with harmonic families, near-misses and sigma ties."""
+    base = rng.uniform(50, 5000, size=n // 3)
+    rs = list(base)
+    for r in base[: n // 3]:
+        kind = rng.integers(0, 4)
+        if kind == 0:
+            rs.append(r * rng.integers(2, 17) + rng.uniform(-1.4, 1.4))
+        elif kind == 1:
+            p, q = O.HARMONIC_RATIOS[rng.integers(len(O.HARMONIC_RATIOS))]
+            rs.append(r * p / q + rng.uniform(-1.6, 1.6))
+        elif kind == 2:
+            rs.append(r / rng.integers(2, 17) + rng.uniform(-1.4, 1.4))
+        else:
+            rs.append(rng.uniform(50, 5000))
+    rs = rs[:n]
+    sig = rng.choice([3.0, 4.5, 6.25, 9.0], size=len(rs)) + np.round(rng.uniform(0, 2, size=len(rs)), 1)
+    pw = rng.choice([10.0, 20.0, 30.0], size=len(rs))
+    return [(float(pw[i]), float(sig[i]), int(rng.choice([1, 2, 4, 8, 16])), float(rs[i]),
+             float(rng.integers(-50, 50) * 2), float(rng.integers(-5, 5) * 20)) for i in range(len(rs))]
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_sort_and_eliminate_harmonics(seed):
+    rng = np.random.default_rng(seed)
+    cands = random_cands(rng, 300)
+    lst = D.make_list(cands)
+    lst = D.api().sort_accelcands(lst)
+    got_sorted = D.read_list(lst)
+    f32 = lambda c: (float(np.float32(c[0])), float(np.float32(c[1])), c[2], c[3], c[4], c[5])
+    exp_sorted = O.sort_cands([f32(c) for c in cands])
+    assert got_sorted == exp_sorted
+    n = C.c_int(len(got_sorted))
+    lst = D.api().eliminate_harmonics(lst, C.byref(n))
+    got = D.read_list(lst)
+    exp = O.eliminate_harmonics(exp_sorted)
+    assert got == exp and n.value == len(exp)
+    assert len(exp) < len(exp_sorted)
+    D.free_list(lst)
+
+
+def test_eliminate_oracle_pinned():
+    """Known answer worked by hand, plus the defining invariant checked by
+    brute force: no survivor is related to an earlier survivor, and every
+    removed candidate is related to an earlier survivor."""
+    c = lambda s, r: (10.0, s, 1, r, 0.0, 0.0)
+    cands = [c(9, 100.0), c(8, 200.0), c(7, 150.0), c(6, 137.0), c(5, 33.6), c(4, 41.0), c(3, 1600.9),
+             c(2, 274.0)]
+    # 200 = 2x100, 150 = 3/2x100, 33.6 ~ 100/3 (|33.33-33.6| < 1.5), 1600.9 ~ 16x100;
+    # 41.0 = 2/5 x 100 + 1.0; 274 = 2 x 137 (removed by the survivor 137)
+    assert [x[3] for x in O.eliminate_harmonics(cands)] == [100.0, 137.0]
+    rng = np.random.default_rng(7)
+    cands = O.sort_cands(random_cands(rng, 200))
+    out = O.eliminate_harmonics(cands)
+    kept = set(id(x) for x in out)
+    for i, a in enumerate(out):
+        for b in out[i + 1:]:
+            assert not O.harmonically_related(a[3], b[3])
+    for x in cands:
+        if id(x) in kept:
+            continue
+        pos = cands.index(x)
+        assert any(O.harmonically_related(s[3], x[3]) for s in out if cands.index(s) < pos)
+
+
+def test_sigma_wrappers():
+    L = D.api()
+    for sigma, numsum, ntr in [(5.0, 16, 1e9), (2.0, 1, 1e5), (8.0, 4, 3e7)]:
+        p = L.power_for_sigma(sigma, numsum, ntr)
+        assert p == pytest.approx(O.power_for_sigma(sigma, numsum, ntr), rel=1e-12)
+        # candidate_sigma switches to an asymptotic series above chi2/dof = 15, so
+        # the round trip is only close; the oracle gives the same value
+        assert L.candidate_sigma(p, numsum, ntr) == pytest.approx(O.candidate_sigma(p, numsum, ntr), abs=1e-9)
+        assert L.candidate_sigma(p, numsum, ntr) == pytest.approx(sigma, abs=1e-3)
+
+
+# --------------------------------------------------------------------- GPU
+
+
+@pytest.mark.gpu
+def test_presto_loop_through_plane_api(tmp_path, config1_data):
+    """accelsearch.c:183-223 driven through subharm_fderivs_vol, add_ffdotpows
+    and search_ffdotpows: the candidate list equals the oracle's search of the
+    same rsteps."""
+    fft, N, dt, opts = config1_data
+    root = D.setup_files(tmp_path, fft, N, dt)
+    obs, _i, cmd, _av = open_obs(root, ["-zmax", "20", "-wmax", "0", "-numharm", "4", "-sigma", "2.0"])
+    L = D.api()
+    shis = L.create_subharminfos(C.byref(obs))
+    rstep = obs.corr_uselen * 0.5
+    lo, hi = 5, 12
+    cands = C.POINTER(D.GSList)()
+    for k in range(lo, hi):
+        startr = obs.rlo + k * rstep
+        lastr = startr + rstep - 0.5
+        fund = L.subharm_fderivs_vol(1, 1, startr, lastr, C.byref(shis[0][0]), C.byref(obs))
+        cands = L.search_ffdotpows(fund, 1, C.byref(obs), cands)
+        for stage in range(1, obs.numharmstages):
+            h2s = 1 << stage
+            for harm in range(1, h2s, 2):
+                sub = L.subharm_fderivs_vol(h2s, harm, startr, lastr, C.byref(shis[stage][harm - 1]),
+                                            C.byref(obs))
+                L.add_ffdotpows(fund, sub, h2s, harm)
+                L.free_ffdotpows(sub)
+            cands = L.search_ffdotpows(fund, h2s, C.byref(obs), cands)
+        L.free_ffdotpows(fund)
+    got = D.read_list(cands)
+    ref_obs = O.create_accelobs(O.make_params(N, dt, len(fft), 20, 0, 4, 2.0, wmaxP=True))
+    ctx = O.Ctx(ref_obs, O.FP32, nthreads=8)
+    exp, _hits = ctx.search(fft, lo, hi)
+    ctx.close()
+    print("plane API: %d candidates, oracle %d" % (len(got), len(exp)))
+    assert [(c[2], c[3], c[4], c[5]) for c in got] == [(c[2], c[3], c[4], c[5]) for c in exp]
+    for g, e in zip(got, exp):
+        assert g[0] == pytest.approx(e[0], rel=1e-5)
+        assert g[1] == pytest.approx(e[1], abs=1e-4)
+    D.free_list(cands)
+    L.free_subharminfos(C.byref(obs), shis)
+    L.free_accelobs(C.byref(obs))
+
+
+def read_pzlcands(path):
+    out = []
+    with open(path) as f:
+        for line in f:
+            if line.startswith("#"):
+                continue
+            s, p, nh, r, z, w = line.split()
+            out.append((float(p), float(s), int(nh), float(r), float(z), float(w)))
+    return out
+
+
+@pytest.mark.gpu
+def test_driver_end_to_end(tmp_path, config1_data):
+    """The accelsearch driver (CPU1 -> GPU -> CPU2) on a config-1 file: its
+    sorted, harmonic-filtered list equals the oracle's."""
+    fft, N, dt, opts = config1_data
+    root = D.setup_files(tmp_path, fft, N, dt)
+    r = subprocess.run([DRIVER, "-zmax", "20", "-wmax", "0", "-numharm", "4", "-sigma", "2.0", root + ".fft"],
+                       capture_output=True, text=True, timeout=300, cwd=str(tmp_path))
+    print(r.stdout[-2000:])
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = read_pzlcands(root + "_ACCEL_20_JERK_0.pzlcands")
+    ref_obs = O.create_accelobs(O.make_params(N, dt, len(fft), 20, 0, 4, 2.0, wmaxP=True))
+    ctx = O.Ctx(ref_obs, O.FP32, nthreads=8)
+    exp, _hits = ctx.search(fft)
+    ctx.close()
+    exp = O.eliminate_harmonics(O.sort_cands(exp))
+    print("driver: %d candidates, oracle %d" % (len(got), len(exp)))
+    assert len(got) == len(exp)
+    assert [(c[2], round(c[3], 4), round(c[4], 4), round(c[5], 4)) for c in got] == \
+           [(c[2], round(c[3], 4), round(c[4], 4), round(c[5], 4)) for c in exp]
+    for g, e in zip(got, exp):
+        assert g[1] == pytest.approx(e[1], abs=1e-4)
