@@ -83,6 +83,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain", action="store_true", help="also time one-rstep-per-launch (plain GPU form)")
+    ap.add_argument("--no-tune", action="store_true", help="use the default correlation variants")
     ap.add_argument("--rsteps", type=int, default=0,
                     help="profiling runs only: search this many rsteps from the middle of the band")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
@@ -110,6 +111,7 @@ def main():
     if args.rsteps:
         lo = g.nrsteps // 2
         hi = lo + args.rsteps
+    tune = [] if args.no_tune else plan.autotune(reps=3)
     for _ in range(args.warmup):
         plan.search(lo, hi, stats=False)
     hip_sync()
@@ -168,6 +170,9 @@ def main():
                                  "harmsum": np.mean([s["ms_sum"] for s in stats_all]),
                                  "host": np.mean([s["ms_host"] for s in stats_all]),
                                  "by_fftlen": {str(k): v["ms"] / args.steps for k, v in sorted(cls.items())}},
+        "autotune": tune,
+        "variants": {str(c["fftlen"]): B.lib().pzl_corr_variant_name(c["fftlen"], plan.variant(c["fftlen"])).decode()
+                     for c in stats_all[0]["classes"]},
         "candidates": len(cands),
         "hits": stats_all[0]["nhits"],
     }

@@ -120,6 +120,16 @@ int pzl_set_spectrum_device(void *plan, const void *dev_fft, long long numbins);
 int pzl_search(void *plan, long long rstep_lo, long long rstep_hi, PzlCand *cands, long long maxcands,
                long long *ncands, PzlStats *stats);
 
+/* Autotuner: times every correlation-kernel variant (FFT radices, threads and
+ * kernel rows per workgroup) of each FFT length of the plan on one batch of
+ * its own data and keeps the fastest, for this plan and for later plans of the
+ * same shape on the device.  report (optional) receives one line per variant.
+ * Replaces the reference's per-shape cuFFT plan choice (accel_kernels.cu:96-158). */
+int pzl_autotune(void *plan, int reps, char *report, int report_len);
+/* The variant in use for an FFT length of the plan; set one explicitly. */
+int pzl_plan_variant(void *plan, int fftlen, int *variant);
+int pzl_plan_set_variant(void *plan, int fftlen, int variant);
+
 /* The candidate list of the last pzl_search (the same as it returned). */
 int pzl_get_cands(void *plan, PzlCand *cands, long long maxcands, long long *ncands);
 

@@ -96,6 +96,12 @@ def lib():
         L.pzl_compute_plane.argtypes = [C.c_void_p, C.c_longlong, C.c_int, P(C.c_float), C.c_longlong,
                                         P(C.c_int), P(C.c_longlong)]
         L.pzl_compute_sums.argtypes = [C.c_void_p, C.c_longlong, P(C.c_float), C.c_longlong]
+        L.pzl_autotune.argtypes = [C.c_void_p, C.c_int, C.c_char_p, C.c_int]
+        L.pzl_plan_variant.argtypes = [C.c_void_p, C.c_int, P(C.c_int)]
+        L.pzl_plan_set_variant.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.pzl_corr_num_variants.argtypes = [C.c_int]
+        L.pzl_corr_variant_name.argtypes = [C.c_int, C.c_int]
+        L.pzl_corr_variant_name.restype = C.c_char_p
         _lib = L
     return _lib
 
@@ -175,6 +181,21 @@ class Plan:
             a = a.view(np.float32)
         assert a.dtype == np.float32
         check(lib().pzl_set_spectrum(self.h, _fptr(a), len(a) // 2))
+
+    def autotune(self, reps=3):
+        """Time every correlation variant per FFT length on this plan's data and
+        keep the fastest; returns the report lines."""
+        buf = C.create_string_buffer(1 << 16)
+        check(lib().pzl_autotune(self.h, int(reps), buf, len(buf)))
+        return buf.value.decode().splitlines()
+
+    def variant(self, fftlen):
+        v = C.c_int(0)
+        check(lib().pzl_plan_variant(self.h, int(fftlen), C.byref(v)))
+        return v.value
+
+    def set_variant(self, fftlen, v):
+        check(lib().pzl_plan_set_variant(self.h, int(fftlen), int(v)))
 
     def set_spectrum_device(self, ptr, numbins):
         check(lib().pzl_set_spectrum_device(self.h, C.c_void_p(ptr), numbins))

@@ -128,6 +128,25 @@ int acquire_bank(Plan *p, bool *built)
     return 0;
 }
 
+// Correlation variants chosen by pzl_autotune, per (device, FFT length,
+// batch size, row counts of the class): later plans of the same shape use them.
+std::mutex g_tune_mu;
+std::map<std::vector<long long>, int> g_tuned;
+
+std::vector<long long> tune_key(const Plan &p, const SizeClass &c)
+{
+    std::vector<long long> k{p.device, c.fftlen, p.B};
+    for (int n : c.nrows) k.push_back(n);
+    return k;
+}
+
+int tuned_variant(const Plan &p, const SizeClass &c)
+{
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tuned.find(tune_key(p, c));
+    return it == g_tuned.end() ? 0 : it->second;
+}
+
 void free_plan(Plan *p)
 {
     if (!p) return;
@@ -213,6 +232,23 @@ struct Scope {
     }
 };
 
+PzlCorrArgs corr_args(const Plan &p, const SizeClass &c, int nb)
+{
+    PzlCorrArgs ca{};
+    ca.harms = p.d_harms;
+    ca.nh = (int)p.harms.size();
+    ca.nb = nb;
+    ca.class_harms = c.d_harms;
+    ca.nclass = (int)c.harms.size();
+    ca.segs = p.d_segs;
+    ca.seg_stride = p.seg_stride;
+    ca.bank = p.d_bank;
+    ca.planes = p.d_planes;
+    ca.plane_stride = p.plane_stride;
+    ca.hr_tab = p.d_hr;
+    return ca;
+}
+
 // Segments, correlations and the harmonic sum of rsteps [r0, r0 + nb) into
 // hit slot 'slot'.
 int launch_batch(Plan &p, long long r0, int nb, int slot, float *d_sums, Timers *tm, bool with_sum = true)
@@ -239,23 +275,9 @@ int launch_batch(Plan &p, long long r0, int nb, int slot, float *d_sums, Timers 
         PZL_HIPCHK(pzl_launch_segments(&ba, p.max_fftlen, p.st));
     }
     for (auto &c : p.classes) {
-        PzlCorrArgs ca{};
-        ca.harms = p.d_harms;
-        ca.nh = nh;
-        ca.nb = nb;
-        ca.class_harms = c.d_harms;
-        ca.nclass = (int)c.harms.size();
-        long long items = 0;
-        for (int h : c.harms) items += (long long)p.harms[h].nrows * nb;
-        ca.total_items = items;
-        ca.segs = p.d_segs;
-        ca.seg_stride = p.seg_stride;
-        ca.bank = p.d_bank;
-        ca.planes = p.d_planes;
-        ca.plane_stride = p.plane_stride;
-        ca.hr_tab = p.d_hr;
+        const PzlCorrArgs ca = corr_args(p, c, nb);
         Scope s(tm, 1, c.fftlen, p.st);
-        PZL_HIPCHK(pzl_launch_corr(c.fftlen, &ca, 256 * 16, p.st));
+        PZL_HIPCHK(pzl_launch_corr(c.fftlen, c.variant, &ca, c.nrows.data(), p.st));
     }
     if (!with_sum) return 0;
     PzlSumArgs sa{};
@@ -584,6 +606,8 @@ int pzl_plan_create(const PzlConfig *cfg, int device, void **plan_out)
         SizeClass c;
         c.fftlen = it->first;
         c.harms = it->second;
+        for (int h : c.harms) c.nrows.push_back(p->harms[h].nrows);
+        c.variant = tuned_variant(*p, c);
         p->classes.push_back(c);
     }
     // hit slots: one rstep can never overflow a slot
@@ -845,6 +869,94 @@ int pzl_search(void *plan, long long rstep_lo, long long rstep_hi, PzlCand *cand
         }
     }
     return 0;
+}
+
+int pzl_autotune(void *plan, int reps, char *report, int report_len)
+{
+    Plan *p = (Plan *)plan;
+    if (!p) return fail(PZL_ERR_ARG, "null plan");
+    if (!p->d_fft) return fail(PZL_ERR_NOSPECTRUM, "no spectrum set");
+    PZL_HIPCHK(hipSetDevice(p->device));
+    if (reps < 1) reps = 1;
+    const long long nr = p->g.nrsteps;
+    const long long r0 = std::max<long long>(0, std::min<long long>(nr - p->B, nr / 2));
+    const int nb = (int)std::min<long long>(p->B, nr - r0);
+    if (nb < 1) return 0;
+    // the segments of one batch from the middle of the band
+    int rc = launch_batch(*p, r0, nb, 0, nullptr, nullptr, false);
+    if (rc) return rc;
+    PZL_HIPCHK(hipStreamSynchronize(p->st));
+    hipEvent_t e0, e1;
+    PZL_HIPCHK(hipEventCreate(&e0));
+    PZL_HIPCHK(hipEventCreate(&e1));
+    std::string rep;
+    char line[256];
+    for (auto &c : p->classes) {
+        const PzlCorrArgs ca = corr_args(*p, c, nb);
+        const int nv = pzl_corr_num_variants(c.fftlen);
+        int best = c.variant;
+        float best_ms = 1e30f;
+        for (int v = 0; v < nv; v++) {
+            hipError_t e = pzl_launch_corr(c.fftlen, v, &ca, c.nrows.data(), p->st);
+            if (e == hipSuccess) e = hipEventRecord(e0, p->st);
+            for (int k = 0; k < reps && e == hipSuccess; k++)
+                e = pzl_launch_corr(c.fftlen, v, &ca, c.nrows.data(), p->st);
+            if (e == hipSuccess) e = hipEventRecord(e1, p->st);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            float ms = 0;
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+            if (e != hipSuccess) {
+                (void)hipGetLastError();
+                snprintf(line, sizeof line, "fftlen %d variant %d (%s): failed: %s\n", c.fftlen, v,
+                         pzl_corr_variant_name(c.fftlen, v), hipGetErrorString(e));
+                rep += line;
+                continue;
+            }
+            ms /= reps;
+            snprintf(line, sizeof line, "fftlen %d variant %d (%s): %.4f ms\n", c.fftlen, v,
+                     pzl_corr_variant_name(c.fftlen, v), ms);
+            rep += line;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = v;
+            }
+        }
+        c.variant = best;
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tuned[tune_key(*p, c)] = best;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (report && report_len > 0) {
+        snprintf(report, (size_t)report_len, "%s", rep.c_str());
+    }
+    return 0;
+}
+
+int pzl_plan_variant(void *plan, int fftlen, int *variant)
+{
+    Plan *p = (Plan *)plan;
+    if (!p) return fail(PZL_ERR_ARG, "null plan");
+    for (auto &c : p->classes)
+        if (c.fftlen == fftlen) {
+            if (variant) *variant = c.variant;
+            return 0;
+        }
+    return fail(PZL_ERR_ARG, "no FFT length %d in this plan", fftlen);
+}
+
+int pzl_plan_set_variant(void *plan, int fftlen, int variant)
+{
+    Plan *p = (Plan *)plan;
+    if (!p) return fail(PZL_ERR_ARG, "null plan");
+    if (variant < 0 || variant >= pzl_corr_num_variants(fftlen))
+        return fail(PZL_ERR_ARG, "FFT length %d has no variant %d", fftlen, variant);
+    for (auto &c : p->classes)
+        if (c.fftlen == fftlen) {
+            c.variant = variant;
+            return 0;
+        }
+    return fail(PZL_ERR_ARG, "no FFT length %d in this plan", fftlen);
 }
 
 int pzl_get_cands(void *plan, PzlCand *cands, long long maxcands, long long *ncands)

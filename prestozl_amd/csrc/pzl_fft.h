@@ -6,31 +6,33 @@
 
 #include <hip/hip_runtime.h>
 
+#define PZL_FFT_HD __host__ __device__ __forceinline__
+
 template <typename T> struct cpx_t;
 template <> struct cpx_t<float> { typedef float2 type; };
 template <> struct cpx_t<double> { typedef double2 type; };
 
-template <typename C> __device__ __forceinline__ C cadd(C a, C b) { return C{a.x + b.x, a.y + b.y}; }
-template <typename C> __device__ __forceinline__ C csub(C a, C b) { return C{a.x - b.x, a.y - b.y}; }
-template <typename C> __device__ __forceinline__ C cmul(C a, C b)
+template <typename C> PZL_FFT_HD C cadd(C a, C b) { return C{a.x + b.x, a.y + b.y}; }
+template <typename C> PZL_FFT_HD C csub(C a, C b) { return C{a.x - b.x, a.y - b.y}; }
+template <typename C> PZL_FFT_HD C cmul(C a, C b)
 {
     return C{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
 }
 // multiply by SIGN * i
-template <int SIGN, typename C> __device__ __forceinline__ C cmul_si(C a)
+template <int SIGN, typename C> PZL_FFT_HD C cmul_si(C a)
 {
     return SIGN > 0 ? C{-a.y, a.x} : C{a.y, -a.x};
 }
 
 // In-register DFTs of size R on v[0..R-1], outputs in natural order.
-template <int SIGN, typename C> __device__ __forceinline__ void dft2(C &a, C &b)
+template <int SIGN, typename C> PZL_FFT_HD void dft2(C &a, C &b)
 {
     C t = a;
     a = cadd(t, b);
     b = csub(t, b);
 }
 
-template <int SIGN, typename C> __device__ __forceinline__ void dft4(C *v)
+template <int SIGN, typename C> PZL_FFT_HD void dft4(C *v)
 {
     C s0 = cadd(v[0], v[2]), d0 = csub(v[0], v[2]);
     C s1 = cadd(v[1], v[3]), d1 = cmul_si<SIGN>(csub(v[1], v[3]));
@@ -40,7 +42,7 @@ template <int SIGN, typename C> __device__ __forceinline__ void dft4(C *v)
     v[3] = csub(d0, d1);
 }
 
-template <int SIGN, typename C> __device__ __forceinline__ void dft8(C *v)
+template <int SIGN, typename C> PZL_FFT_HD void dft8(C *v)
 {
     typedef decltype(v[0].x) R;
     const R h = (R)0.70710678118654752440;
@@ -61,7 +63,7 @@ template <int SIGN, typename C> __device__ __forceinline__ void dft8(C *v)
     }
 }
 
-template <int SIGN, typename C> __device__ __forceinline__ void dft3(C *v)
+template <int SIGN, typename C> PZL_FFT_HD void dft3(C *v)
 {
     typedef decltype(v[0].x) R;
     const R c1 = (R)-0.5, s1 = (R)SIGN * (R)0.86602540378443864676;
@@ -73,7 +75,7 @@ template <int SIGN, typename C> __device__ __forceinline__ void dft3(C *v)
     v[2] = csub(m, j);
 }
 
-template <int SIGN, typename C> __device__ __forceinline__ void dft5(C *v)
+template <int SIGN, typename C> PZL_FFT_HD void dft5(C *v)
 {
     typedef decltype(v[0].x) R;
     const R c1 = (R)0.30901699437494742410, c2 = (R)-0.80901699437494742410;
@@ -94,7 +96,7 @@ template <int SIGN, typename C> __device__ __forceinline__ void dft5(C *v)
     v[3] = csub(m2, n2);
 }
 
-template <int SIGN, typename C> __device__ __forceinline__ void dft6(C *v)
+template <int SIGN, typename C> PZL_FFT_HD void dft6(C *v)
 {
     // 6 = 2 x 3 (Good-Thomas free: 3 then 2 with twiddles)
     C a[3] = {v[0], v[2], v[4]}, b[3] = {v[1], v[3], v[5]};
@@ -113,14 +115,122 @@ template <int SIGN, typename C> __device__ __forceinline__ void dft6(C *v)
     }
 }
 
-template <int R, int SIGN, typename C> __device__ __forceinline__ void dftR(C *v)
+// exp(SIGN 2 pi i m / R) at compile time: cos and sin by Taylor series on
+// [0, pi/4] after octant reduction, in double, rounded once to the working type.
+constexpr double ct_pi = 3.14159265358979323846264338327950288;
+constexpr double ct_taylor_sin(double x)
 {
-    if constexpr (R == 2) dft2<SIGN>(v[0], v[1]);
+    double term = x, sum = x;
+    for (int k = 1; k < 30; k++) {
+        term *= -x * x / ((2.0 * k) * (2.0 * k + 1.0));
+        sum += term;
+    }
+    return sum;
+}
+constexpr double ct_taylor_cos(double x)
+{
+    double term = 1.0, sum = 1.0;
+    for (int k = 1; k < 30; k++) {
+        term *= -x * x / ((2.0 * k - 1.0) * (2.0 * k));
+        sum += term;
+    }
+    return sum;
+}
+// cos / sin of 2 pi m / R for integer m, R > 0 (exact octant symmetries)
+constexpr double ct_cos2pi(long m, long R)
+{
+    m = ((m % R) + R) % R;
+    // angle = 2 pi m / R; reduce with the 8 octants in integer arithmetic
+    const long o = (8 * m) / R;            // octant 0..7
+    const double f = (double)(8 * m - o * R) / (double)R;  // fraction within the octant
+    const double a = f * ct_pi / 4.0;
+    switch (o) {
+    case 0: return ct_taylor_cos(a);
+    case 1: return ct_taylor_sin(ct_pi / 4.0 - a);
+    case 2: return -ct_taylor_sin(a);
+    case 3: return -ct_taylor_cos(ct_pi / 4.0 - a);
+    case 4: return -ct_taylor_cos(a);
+    case 5: return -ct_taylor_sin(ct_pi / 4.0 - a);
+    case 6: return ct_taylor_sin(a);
+    default: return ct_taylor_cos(ct_pi / 4.0 - a);
+    }
+}
+constexpr double ct_sin2pi(long m, long R) { return ct_cos2pi(4 * m - R, 4 * R); }
+
+// x * exp(SIGN 2 pi i M / R) with the trivial factors done without multiplies
+template <int M, int R, int SIGN, typename C> PZL_FFT_HD C twc(C x)
+{
+    typedef decltype(x.x) T;
+    constexpr int m = ((M % R) + R) % R;
+    if constexpr (m == 0) {
+        return x;
+    } else if constexpr (4 * m == R) {
+        return cmul_si<SIGN>(x);
+    } else if constexpr (2 * m == R) {
+        return C{-x.x, -x.y};
+    } else if constexpr (4 * m == 3 * R) {
+        return cmul_si<-SIGN>(x);
+    } else {
+        constexpr T c = (T)ct_cos2pi(m, R);
+        constexpr T s = (T)(SIGN * ct_sin2pi(m, R));
+        return C{x.x * c - x.y * s, x.x * s + x.y * c};
+    }
+}
+
+template <int R, int SIGN, typename C> PZL_FFT_HD void dftR(C *v);
+
+// DFT of size R1*R2 as R2 DFTs of size R1, twiddles, R1 DFTs of size R2:
+// X[k1 + R1 k2] = sum_n2 W_R2^(n2 k2) W_R^(n2 k1) sum_n1 x[R2 n1 + n2] W_R1^(n1 k1)
+template <int R1, int R2, int SIGN, int N2 = 0, typename C> PZL_FFT_HD void comp_stage1(C (*t)[R1], C *v)
+{
+    if constexpr (N2 < R2) {
+#pragma unroll
+        for (int n1 = 0; n1 < R1; n1++) t[N2][n1] = v[R2 * n1 + N2];
+        dftR<R1, SIGN>(t[N2]);
+        comp_stage1<R1, R2, SIGN, N2 + 1>(t, v);
+    }
+}
+template <int R1, int R2, int SIGN, int N2, int K1, typename C> PZL_FFT_HD void comp_tw(C (*t)[R1])
+{
+    if constexpr (N2 < R2) {
+        if constexpr (K1 < R1) {
+            t[N2][K1] = twc<N2 * K1, R1 * R2, SIGN>(t[N2][K1]);
+            comp_tw<R1, R2, SIGN, N2, K1 + 1>(t);
+        } else {
+            comp_tw<R1, R2, SIGN, N2 + 1, 0>(t);
+        }
+    }
+}
+template <int R1, int R2, int SIGN, typename C> PZL_FFT_HD void dft_comp(C *v)
+{
+    C t[R2][R1];
+    comp_stage1<R1, R2, SIGN>(t, v);
+    comp_tw<R1, R2, SIGN, 1, 1>(t);
+#pragma unroll
+    for (int k1 = 0; k1 < R1; k1++) {
+        C u[R2];
+#pragma unroll
+        for (int n2 = 0; n2 < R2; n2++) u[n2] = t[n2][k1];
+        dftR<R2, SIGN>(u);
+#pragma unroll
+        for (int k2 = 0; k2 < R2; k2++) v[k1 + R1 * k2] = u[k2];
+    }
+}
+
+template <int R, int SIGN, typename C> PZL_FFT_HD void dftR(C *v)
+{
+    if constexpr (R == 1) return;
+    else if constexpr (R == 2) dft2<SIGN>(v[0], v[1]);
     else if constexpr (R == 3) dft3<SIGN>(v);
     else if constexpr (R == 4) dft4<SIGN>(v);
     else if constexpr (R == 5) dft5<SIGN>(v);
     else if constexpr (R == 6) dft6<SIGN>(v);
     else if constexpr (R == 8) dft8<SIGN>(v);
+    else if constexpr (R == 10) dft_comp<2, 5, SIGN>(v);
+    else if constexpr (R == 12) dft_comp<4, 3, SIGN>(v);
+    else if constexpr (R == 16) dft_comp<4, 4, SIGN>(v);
+    else if constexpr (R == 20) dft_comp<4, 5, SIGN>(v);
+    else static_assert(R == 2, "radix not provided");
 }
 
 // Block-cooperative double-precision Stockham FFT of n points held in LDS

@@ -48,7 +48,7 @@ struct PzlCorrArgs {
     int nh, nb;
     const int *class_harms;  // harmonics of one FFT length
     int nclass;
-    long long total_items;   // sum over the class of nrows * nb
+    long long total_items;   // work units: per harmonic of the class, ceil(nrows / rows per workgroup)
     const float2 *segs;
     long long seg_stride;
     const float2 *bank;
@@ -84,7 +84,11 @@ extern "C" {
 hipError_t pzl_launch_bank_build(const PzlHarm *d_harms, int nh, int total_rows, int max_fftlen, float2 *d_bank,
                                  hipStream_t st);
 hipError_t pzl_launch_segments(const PzlBatchArgs *a, int max_fftlen, hipStream_t st);
-hipError_t pzl_launch_corr(int N, const PzlCorrArgs *a, int nblocks, hipStream_t st);
+// Correlation of one FFT-length class with variant 'variant' (0 = default);
+// class_nrows: host copy of the kernel-row counts of the class's harmonics.
+hipError_t pzl_launch_corr(int N, int variant, const PzlCorrArgs *a, const int *class_nrows, hipStream_t st);
+int pzl_corr_num_variants(int n);
+const char *pzl_corr_variant_name(int n, int v);
 hipError_t pzl_launch_harmsum(const PzlSumArgs *a, hipStream_t st);
 // Radix sort of n (key, power) pairs on 'end_bit' key bits; temp == nullptr
 // queries the scratch size into *temp_bytes.

@@ -253,154 +253,221 @@ __global__ __launch_bounds__(SEG_NT) void k_segments(PzlBatchArgs a)
 // ---------------------------------------------------------------------------
 // Correlation: product, inverse FFT, power, plane store
 // ---------------------------------------------------------------------------
+//
+// One workgroup takes G kernel rows of one harmonic and runs them against the
+// segments of every rstep of the batch: the rows are read from HBM once per
+// batch and stay in registers.  Per (row, rstep) the inverse FFT is a Stockham
+// sequence of in-register radix-R DFTs with LDS exchanges between passes
+// (ping-pong buffers: one barrier per exchange).  Pass 0 forms the product
+// with conj(kernel) straight from the segment; the last pass turns the
+// natural-order outputs into powers and stores the wanted r window.
+// Twiddles depend only on the thread and the pass, so each workgroup computes
+// them once (double-precision sincospi) and keeps them in registers for all of
+// its (row, rstep) pairs (the grid is persistent).
 
-__host__ __device__ __forceinline__ int lds_pad(int p) { return p + (p >> 4); }
+__host__ __device__ constexpr int lds_pad(int p) { return p + (p >> 4); }
 
-struct SlotDesc {
-    const float2 *seg;
-    const float2 *kern;
-    float *plane;
-    int offset, numrs, valid;
+template <int N, int NT, int G, bool PING, int... Rs> struct CorrPlan {
+    static constexpr int NP = sizeof...(Rs);
+    static constexpr int RAD[NP] = {Rs...};
+    static constexpr int ns(int p)
+    {
+        int s = 1;
+        for (int i = 0; i < p; i++) s *= RAD[i];
+        return s;
+    }
+    static constexpr int nbf(int p) { return G * N / RAD[p]; }     // butterflies of pass p
+    static constexpr int per(int p) { return (nbf(p) + NT - 1) / NT; }  // ... per thread
+    static constexpr bool full(int p) { return nbf(p) % NT == 0; }
+    static constexpr int twoff(int p)
+    {
+        int o = 0;
+        for (int i = 1; i < p; i++) o += per(i) * (RAD[i] - 1);
+        return o;
+    }
+    static constexpr int TWN = twoff(NP);
+    static constexpr int PPT = per(0) * RAD[0];  // kernel values held per thread
+    static constexpr int BUF = lds_pad(G * N) + 1;
+    static constexpr size_t smem() { return (size_t)(PING ? 2 : 1) * BUF * sizeof(float2); }
+    static_assert(ns(NP) == N, "radices do not multiply to N");
 };
 
-// Stockham pass with radix R on G FFTs of length N held in LDS; Ns is the
-// product of the radices already applied.  The last pass turns the
-// natural-order outputs into powers and stores the wanted r range.
-template <int N, int G, int NT, int Ns, int R, int... Rest>
-__device__ __forceinline__ void corr_pass(float2 *data, const float2 *tw, const SlotDesc *sd, int tid, float norm)
-{
-    constexpr int NB = N / R;
-    constexpr int PER = G * NB / NT;
-    static_assert(PER * NT == G * NB, "pass does not tile the workgroup");
-    constexpr bool last = (sizeof...(Rest) == 0);
-    constexpr int TWS = N / (Ns * R);
-    float2 v[PER][R];
+template <int N, int NT, int G, bool PING, int... Rs> struct CorrFFT {
+    using CP = CorrPlan<N, NT, G, PING, Rs...>;
+    static constexpr int NP = CP::NP;
+
+    // butterfly bf of pass P: FFT g, index j within that FFT's N / R butterflies
+    template <int P> static __device__ __forceinline__ void where(int bf, int &g, int &j)
+    {
+        constexpr int NBP = N / CP::RAD[P];
+        g = bf / NBP;
+        j = bf - g * NBP;
+    }
+
+    template <int P> static __device__ __forceinline__ void init_tw(float2 *tw, int tid)
+    {
+        if constexpr (P < NP) {
+            constexpr int R = CP::RAD[P], Ns = CP::ns(P);
 #pragma unroll
-    for (int i = 0; i < PER; i++) {
-        const int bf = tid + i * NT;
-        const int g = bf / NB, j = bf % NB;
-        const int step = (j % Ns) * TWS;
+            for (int i = 0; i < CP::per(P); i++) {
+                int g, j;
+                where<P>(tid + i * NT, g, j);
+                const int m = j % Ns;
 #pragma unroll
-        for (int r = 0; r < R; r++) {
-            float2 x = data[lds_pad(g * N + j + r * NB)];
-            if (r > 0) x = cmul(x, tw[step * r]);
-            v[i][r] = x;
+                for (int r = 1; r < R; r++) {
+                    double s, c;
+                    sincospi(2.0 * (double)(m * r) / (double)(Ns * R), &s, &c);
+                    tw[CP::twoff(P) + i * (R - 1) + (r - 1)] = float2{(float)c, (float)s};
+                }
+            }
+            init_tw<P + 1>(tw, tid);
         }
     }
-    __syncthreads();
+
+    // passes 1 .. NP-1 of one FFT set; 'src' holds the output of pass P-1
+    template <int P>
+    static __device__ __forceinline__ void pass(float2 *b0, float2 *b1, const float2 *tw, int tid, float *pbase,
+                                                long long rs_stride, int gvalid, int numrs, int offset, float norm)
+    {
+        constexpr int R = CP::RAD[P], Ns = CP::ns(P), NBP = N / R;
+        constexpr bool last = (P == NP - 1);
+        const float2 *src = ((P - 1) & 1) && PING ? b1 : b0;
+        float2 *dst = (P & 1) && PING ? b1 : b0;
+        float2 v[CP::per(P)][R];
 #pragma unroll
-    for (int i = 0; i < PER; i++) {
-        const int bf = tid + i * NT;
-        const int g = bf / NB, j = bf % NB;
-        dftR<R, 1>(v[i]);
-        if constexpr (!last) {
-            const int idxD = (j / Ns) * Ns * R + (j % Ns);
-#pragma unroll
-            for (int r = 0; r < R; r++) data[lds_pad(g * N + idxD + r * Ns)] = v[i][r];
-        } else {
-            const SlotDesc d = sd[g];
-            if (d.valid) {
+        for (int i = 0; i < CP::per(P); i++) {
+            const int bf = tid + i * NT;
+            if (CP::full(P) || bf < CP::nbf(P)) {
+                int g, j;
+                where<P>(bf, g, j);
 #pragma unroll
                 for (int r = 0; r < R; r++) {
-                    const int ro = j + r * NB - d.offset;  // natural-order index minus offset
-                    if (ro >= 0 && ro < d.numrs) {
-                        const float2 y = v[i][r];
-                        d.plane[ro] = (y.x * y.x + y.y * y.y) * norm;
+                    float2 x = src[lds_pad(g * N + j + r * NBP)];
+                    if (r > 0) x = cmul(x, tw[CP::twoff(P) + i * (R - 1) + (r - 1)]);
+                    v[i][r] = x;
+                }
+            }
+        }
+        if constexpr (!PING && !last) __syncthreads();
+#pragma unroll
+        for (int i = 0; i < CP::per(P); i++) {
+            const int bf = tid + i * NT;
+            if (CP::full(P) || bf < CP::nbf(P)) {
+                int g, j;
+                where<P>(bf, g, j);
+                dftR<R, 1>(v[i]);
+                if constexpr (!last) {
+                    const int idxD = (j / Ns) * Ns * R + (j % Ns);
+#pragma unroll
+                    for (int r = 0; r < R; r++) dst[lds_pad(g * N + idxD + r * Ns)] = v[i][r];
+                } else {
+                    if (g < gvalid) {
+                        float *pl = pbase + g * rs_stride;
+#pragma unroll
+                        for (int r = 0; r < R; r++) {
+                            const int ro = j + r * NBP - offset;  // natural-order index minus offset
+                            if (ro >= 0 && ro < numrs) {
+                                const float2 y = v[i][r];
+                                pl[ro] = (y.x * y.x + y.y * y.y) * norm;
+                            }
+                        }
                     }
                 }
             }
         }
+        if constexpr (!last) {
+            __syncthreads();
+            pass<P + 1>(b0, b1, tw, tid, pbase, rs_stride, gvalid, numrs, offset, norm);
+        }
     }
-    if constexpr (!last) {
-        __syncthreads();
-        corr_pass<N, G, NT, Ns * R, Rest...>(data, tw, sd, tid, norm);
-    }
-}
+};
 
-template <int N, int G, int NT, int R0, int... Rest>
+template <int N, int NT, int G, bool PING, int... Rs>
 __global__ __launch_bounds__(NT) void k_corr(PzlCorrArgs a)
 {
+    using F = CorrFFT<N, NT, G, PING, Rs...>;
+    using CP = typename F::CP;
+    constexpr int R0 = CP::RAD[0], NB0 = N / R0;
     extern __shared__ float2 smem[];
-    float2 *data = smem;                     // G*N (+pad)
-    float2 *tw = smem + lds_pad(G * N) + 1;  // N twiddles exp(+2 pi i m / N)
-    __shared__ SlotDesc sd[G];
+    float2 *b0 = smem, *b1 = PING ? smem + CP::BUF : smem;
     const int tid = threadIdx.x;
     const float norm = (float)(1.0 / ((double)N * (double)N));
 
-    for (int m = tid; m < N; m += NT) {
-        double s, c;
-        sincospi(2.0 * (double)m / (double)N, &s, &c);
-        tw[m] = float2{(float)c, (float)s};
-    }
+    float2 tw[CP::TWN > 0 ? CP::TWN : 1];
+    F::template init_tw<1>(tw, tid);
 
-    for (long long grp = blockIdx.x; grp * G < a.total_items; grp += gridDim.x) {
-        __syncthreads();
-        if (tid < G) {
-            const long long item = grp * G + tid;
-            SlotDesc d;
-            d.valid = 0;
-            d.seg = nullptr;
-            d.kern = nullptr;
-            d.plane = nullptr;
-            d.offset = 0;
-            d.numrs = 0;
-            if (item < a.total_items) {
-                // items: for each harmonic of this size class, rows x batch
-                long long acc = 0;
-                for (int c = 0; c < a.nclass; c++) {
-                    const int h = a.class_harms[c];
-                    const PzlHarm H = a.harms[h];
-                    const long long cnt = (long long)H.nrows * a.nb;
-                    if (item < acc + cnt) {
-                        const long long loc = item - acc;
-                        const int row = (int)(loc / a.nb), b = (int)(loc % a.nb);
-                        d.valid = 1;
-                        d.seg = a.segs + (long long)b * a.seg_stride + H.seg_off;
-                        d.kern = a.bank + H.bank_off + (long long)row * N;
-                        d.plane = a.planes + (long long)b * a.plane_stride + H.plane_off +
-                                  (long long)row * H.rs_stride;
-                        d.offset = 2 * H.halfwidth;
-                        d.numrs = a.hr_tab[b * a.nh + h].numrs;
-                        break;
-                    }
-                    acc += cnt;
-                }
-            }
-            sd[tid] = d;
-        }
-        __syncthreads();
-
-        // pass 0 (Ns = 1): P[k] = seg[k mod N/2] * conj(kern[k]) straight from HBM/L2
+    for (long long unit = blockIdx.x; unit < a.total_items; unit += gridDim.x) {
+        // the unit's harmonic and first row (units: per harmonic of the class, ceil(nrows / G))
+        int h = -1, row0 = 0;
         {
-            constexpr int R = R0;
-            constexpr int NB = N / R;
-            constexpr int PER = G * NB / NT;
-            static_assert(PER * NT == G * NB, "pass 0 does not tile the workgroup");
-#pragma unroll
-            for (int i = 0; i < PER; i++) {
-                const int bf = tid + i * NT;
-                const int g = bf / NB, j = bf % NB;
-                const SlotDesc d = sd[g];
-                float2 v[R];
-#pragma unroll
-                for (int r = 0; r < R; r++) {
-                    const int k = j + r * NB;
-                    float2 x = float2{0.f, 0.f};
-                    if (d.valid) {
-                        const float2 pd = d.seg[k < N / 2 ? k : k - N / 2];
-                        const float2 kk = d.kern[k];
-                        // re = dr*kr + di*ki, im = di*kr - dr*ki (accel_utils.c:1262-1265)
-                        x = float2{pd.x * kk.x + pd.y * kk.y, pd.y * kk.x - pd.x * kk.y};
-                    }
-                    v[r] = x;
+            long long acc = 0;
+            for (int c = 0; c < a.nclass; c++) {
+                const int hh = a.class_harms[c];
+                const long long cnt = (a.harms[hh].nrows + G - 1) / G;
+                if (unit < acc + cnt) {
+                    h = hh;
+                    row0 = (int)(unit - acc) * G;
+                    break;
                 }
-                dftR<R, 1>(v);
-#pragma unroll
-                for (int r = 0; r < R; r++) data[lds_pad(g * N + j * R + r)] = v[r];
+                acc += cnt;
             }
         }
-        __syncthreads();
-        corr_pass<N, G, NT, R0, Rest...>(data, tw, sd, tid, norm);
+        const PzlHarm &H = a.harms[h];
+        const int nrows = H.nrows;
+        // the kernel rows, at the positions pass 0 reads them
+        float2 kv[CP::per(0)][R0];
+#pragma unroll
+        for (int i = 0; i < CP::per(0); i++) {
+            const int bf = tid + i * NT;
+            int g, j;
+            F::template where<0>(bf, g, j);
+            const bool ok = (CP::full(0) || bf < CP::nbf(0)) && row0 + g < nrows;
+            const float2 *kr = a.bank + H.bank_off + (long long)(row0 + g) * N;
+#pragma unroll
+            for (int r = 0; r < R0; r++) kv[i][r] = ok ? kr[j + r * NB0] : float2{0.f, 0.f};
+        }
+        const int offset = 2 * H.halfwidth;
+        for (int b = 0; b < a.nb; b++) {
+            const float2 *seg = a.segs + (long long)b * a.seg_stride + H.seg_off;
+            const int numrs = a.hr_tab[b * a.nh + h].numrs;
+            float *pbase = a.planes + (long long)b * a.plane_stride + H.plane_off + (long long)row0 * H.rs_stride;
+            // pass 0: P[k] = seg[k mod N/2] * conj(kern[k]); the spread segment repeats
+            // with period N/2, so an even radix reads each segment value once
+#pragma unroll
+            for (int i = 0; i < CP::per(0); i++) {
+                const int bf = tid + i * NT;
+                if (CP::full(0) || bf < CP::nbf(0)) {
+                    int g, j;
+                    F::template where<0>(bf, g, j);
+                    float2 v[R0];
+                    if constexpr (R0 % 2 == 0) {
+#pragma unroll
+                        for (int r = 0; r < R0 / 2; r++) {
+                            const float2 pd = seg[j + r * NB0];
+                            const float2 k1 = kv[i][r], k2 = kv[i][r + R0 / 2];
+                            // re = dr*kr + di*ki, im = di*kr - dr*ki (accel_utils.c:1262-1265)
+                            v[r] = float2{pd.x * k1.x + pd.y * k1.y, pd.y * k1.x - pd.x * k1.y};
+                            v[r + R0 / 2] = float2{pd.x * k2.x + pd.y * k2.y, pd.y * k2.x - pd.x * k2.y};
+                        }
+                    } else {
+#pragma unroll
+                        for (int r = 0; r < R0; r++) {
+                            const int k = j + r * NB0;
+                            const float2 pd = seg[k < N / 2 ? k : k - N / 2];
+                            const float2 kk = kv[i][r];
+                            v[r] = float2{pd.x * kk.x + pd.y * kk.y, pd.y * kk.x - pd.x * kk.y};
+                        }
+                    }
+                    dftR<R0, 1>(v);
+#pragma unroll
+                    for (int r = 0; r < R0; r++) b0[lds_pad(g * N + j * R0 + r)] = v[r];
+                }
+            }
+            __syncthreads();
+            F::template pass<1>(b0, b1, tw, tid, pbase, H.rs_stride, nrows - row0, numrs, offset, norm);
+            // the next pass 0 overwrites b0: wait for this FFT's readers of b0
+            if constexpr (!PING || (CP::NP % 2 == 0)) __syncthreads();
+        }
     }
 }
 
@@ -410,88 +477,125 @@ __global__ __launch_bounds__(NT) void k_corr(PzlCorrArgs a)
 
 #define SUM_NT 256
 
-// rint((m * j) / 2^s) with round-half-even, in integers: the subharmonic r
-// index of fundamental point m = 2*fullrlo + ii (accel_utils.c:1115-1122 with
-// integer fullrlo; rint rounds half to even).
-__device__ __forceinline__ long long rint_dyadic(long long m, int j, int s)
-{
-    const long long prod = m * j;
-    long long q = prod >> s;
-    if (s > 0) {
-        const long long rem = prod & ((1LL << s) - 1), half = 1LL << (s - 1);
-        if (rem > half || (rem == half && (q & 1))) q++;
-    }
-    return q;
-}
+#define SUM_ZB 8  // fundamental z rows per workgroup
 
-// One workgroup: SUM_NT consecutive r of one (rstep, w, z) row of the
-// fundamental.  All subharmonic powers of a cell are loaded first and then
-// added in the reference's order (stage 1..S, odd harmonics ascending).
+// One workgroup: SUM_NT consecutive r of SUM_ZB consecutive z rows of one
+// (rstep, w) of the fundamental.  The subharmonic r indices depend only on
+// (rstep, harmonic, r), so each thread computes them once (integer
+// round-half-even of m j / 2^s split into a uniform 64-bit part and a small
+// per-thread 32-bit part) and keeps them in registers; the row offsets of the
+// subharmonic z/w maps for the workgroup's rows are staged in LDS.  For every
+// cell all subharmonic powers are loaded first and then added in the
+// reference's order (stage 1..S, odd harmonics ascending).
+//
+// Workgroup ids are dealt round-robin over the 8 XCDs; each XCD is given a
+// contiguous range of the (rstep, w, z block, r chunk) space so that the
+// subharmonic rows that neighbouring z/w rows share are fetched into one L2.
 template <int NSTAGES>
 __global__ __launch_bounds__(SUM_NT) void k_harmsum(PzlSumArgs a)
 {
     constexpr int NSUB = (1 << (NSTAGES - 1)) - 1;
+    constexpr int NQ = NSUB > 0 ? NSUB : 1;
+    __shared__ long long zoff[NQ][SUM_ZB];
     const int tid = threadIdx.x;
     const int chunks = (a.corr_uselen + SUM_NT - 1) / SUM_NT;
-    const long long blk = blockIdx.x;
-    const int chunk = (int)(blk % chunks);
-    const long long rowid = blk / chunks;  // (b, w, z)
-    const int z = (int)(rowid % a.numz);
-    const int w = (int)((rowid / a.numz) % a.numw);
-    const int b = (int)(rowid / ((long long)a.numz * a.numw));
+    const int zblocks = (a.numz + SUM_ZB - 1) / SUM_ZB;
+    const long long nblk = (long long)a.nb * a.numw * zblocks * chunks;
+    const long long per_xcd = (nblk + 7) / 8;
+    const long long L = (long long)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+    if (L >= nblk) return;
+    const int chunk = (int)(L % chunks);
+    long long t = L / chunks;
+    const int zb = (int)(t % zblocks);
+    t /= zblocks;
+    const int w = (int)(t % a.numw);
+    const int b = (int)(t / a.numw);
+    const int z0 = zb * SUM_ZB;
+    const int nz = min(SUM_ZB, a.numz - z0);
     const int r = chunk * SUM_NT + tid;
     const bool active = r < a.corr_uselen;
     const int rr = active ? r : a.corr_uselen - 1;
     const long long rstep = a.rstep0 + b;
     const float *base = a.planes + (long long)b * a.plane_stride;
     const PzlHarm F = a.harms[0];
-    float acc = base[F.plane_off + ((long long)w * a.numz + z) * F.rs_stride + rr];
 
-    float v[NSUB > 0 ? NSUB : 1];
     if constexpr (NSUB > 0) {
-        const long long m = a.int_rlo ? (long long)(2.0 * a.startr[rstep]) + rr : 0;
+        for (int k = tid; k < NSUB * SUM_ZB; k += SUM_NT) {
+            const int q = k / SUM_ZB, zi = k % SUM_ZB;
+            const int h = q + 1;
+            const PzlHarm &H = a.harms[h];
+            const int z = min(z0 + zi, a.numz - 1);
+            zoff[q][zi] = H.plane_off +
+                          ((long long)a.winds[h * a.numw + w] * H.numz + a.zinds[h * a.numz + z]) * H.rs_stride;
+        }
+    }
+    int rind[NQ];
+    if constexpr (NSUB > 0) {
+        const long long m0 = a.int_rlo ? (long long)(2.0 * a.startr[rstep]) : 0;
 #pragma unroll
         for (int q = 0; q < NSUB; q++) {
             const int h = q + 1;
-            const PzlHarm H = a.harms[h];
-            const float *row = base + H.plane_off +
-                               ((long long)a.winds[h * a.numw + w] * H.numz + a.zinds[h * a.numz + z]) * H.rs_stride;
-            int rind;
-            if (a.int_rlo)
-                rind = (int)(rint_dyadic(m, H.harmnum, H.stage) - 2 * a.hr_tab[b * a.nh + h].rlo);
-            else
-                rind = a.rinds[((long long)b * a.nh + h) * a.corr_uselen + rr];
-            v[q] = row[rind];
-        }
-    }
-    const unsigned long long lanemask_lt = (1ull << (__lane_id())) - 1ull;
-    int q = 0;
-#pragma unroll
-    for (int s = 0; s < NSTAGES; s++) {
-        if (s > 0) {
-#pragma unroll
-            for (int k = 0; k < (1 << (s - 1)); k++, q++) acc += v[q];
-        }
-        const bool hit = active && (acc > a.powcut[s]);
-        const unsigned long long mask = __ballot(hit);
-        if (mask) {
-            const int cnt = __popcll(mask);
-            const int leader = __ffsll((long long)mask) - 1;
-            unsigned long long slot0 = 0;
-            if ((int)__lane_id() == leader) slot0 = atomicAdd(a.nhits, (unsigned long long)cnt);
-            slot0 = __shfl(slot0, leader);
-            if (hit) {
-                const unsigned long long slot = slot0 + __popcll(mask & lanemask_lt);
-                if (slot < a.maxhits) {
-                    a.hit_keys[slot] =
-                        ((((rstep * NSTAGES + s) * a.numw + w) * (long long)a.numz + z) * a.corr_uselen + r);
-                    a.hit_pows[slot] = acc;
+            if (a.int_rlo) {
+                // rint((m0 + rr) j / 2^s) - 2 rlo_h with m0 j split as q0 2^s + rem0
+                const int j = a.harms[h].harmnum, sh = a.harms[h].stage;
+                const long long p0 = m0 * j;
+                const long long q0 = p0 >> sh;
+                const int rem0 = (int)(p0 - (q0 << sh));
+                const int base_h = (int)(q0 - 2 * a.hr_tab[b * a.nh + h].rlo);
+                const int tt = rem0 + rr * j;
+                int qq = tt >> sh;
+                const int rem = tt - (qq << sh);
+                if (sh > 0) {
+                    const int half = 1 << (sh - 1);
+                    // round half to even on the full quotient q0 + qq
+                    if (rem > half || (rem == half && ((q0 + qq) & 1))) qq++;
                 }
+                rind[q] = base_h + qq;
+            } else {
+                rind[q] = a.rinds[((long long)b * a.nh + h) * a.corr_uselen + rr];
             }
         }
     }
-    if (a.sums_out && active)
-        a.sums_out[(((long long)b * a.numw + w) * a.numz + z) * a.corr_uselen + r] = acc;
+    __syncthreads();
+
+    const unsigned long long lanemask_lt = (1ull << (__lane_id())) - 1ull;
+    const float *frow = base + F.plane_off + ((long long)w * a.numz + z0) * F.rs_stride;
+    for (int zi = 0; zi < nz; zi++) {
+        const int z = z0 + zi;
+        float acc = frow[(long long)zi * F.rs_stride + rr];
+        float v[NQ];
+        if constexpr (NSUB > 0) {
+#pragma unroll
+            for (int q = 0; q < NSUB; q++) v[q] = base[zoff[q][zi] + rind[q]];
+        }
+        int q = 0;
+#pragma unroll
+        for (int s = 0; s < NSTAGES; s++) {
+            if (s > 0) {
+#pragma unroll
+                for (int k = 0; k < (1 << (s - 1)); k++, q++) acc += v[q];
+            }
+            const bool hit = active && (acc > a.powcut[s]);
+            const unsigned long long mask = __ballot(hit);
+            if (mask) {
+                const int cnt = __popcll(mask);
+                const int leader = __ffsll((long long)mask) - 1;
+                unsigned long long slot0 = 0;
+                if ((int)__lane_id() == leader) slot0 = atomicAdd(a.nhits, (unsigned long long)cnt);
+                slot0 = __shfl(slot0, leader);
+                if (hit) {
+                    const unsigned long long slot = slot0 + __popcll(mask & lanemask_lt);
+                    if (slot < a.maxhits) {
+                        a.hit_keys[slot] =
+                            ((((rstep * NSTAGES + s) * a.numw + w) * (long long)a.numz + z) * a.corr_uselen + r);
+                        a.hit_pows[slot] = acc;
+                    }
+                }
+            }
+        }
+        if (a.sums_out && active)
+            a.sums_out[(((long long)b * a.numw + w) * a.numz + z) * a.corr_uselen + r] = acc;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -647,49 +751,111 @@ extern "C" hipError_t pzl_launch_segments(const PzlBatchArgs *a, int max_fftlen,
     return hipGetLastError();
 }
 
-template <int N, int G, int NT, int... Rs>
-static hipError_t launch_corr_t(const PzlCorrArgs *a, int nblocks, hipStream_t st)
+// One correlation variant: FFT length N, NT threads, G rows per workgroup,
+// ping-pong LDS or not, radices.  The grid is persistent: as many workgroups
+// as fit on the device at once (or fewer when there are fewer units).
+template <int N, int NT, int G, bool PING, int... Rs>
+static hipError_t launch_corr_t(const PzlCorrArgs *a, const int *class_nrows, hipStream_t st)
 {
-    const size_t smem = (size_t)(lds_pad(G * N) + 1 + N) * sizeof(float2);
-    long long groups = (a->total_items + G - 1) / G;
-    int grid = (int)(groups < nblocks ? groups : nblocks);
-    if (grid < 1) return hipSuccess;
-    hipLaunchKernelGGL((k_corr<N, G, NT, Rs...>), dim3(grid), dim3(NT), smem, st, *a);
+    using CP = CorrPlan<N, NT, G, PING, Rs...>;
+    static int max_grid = 0;
+    if (max_grid == 0) {
+        int dev = 0, cus = 0, per_cu = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e == hipSuccess)
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_corr<N, NT, G, PING, Rs...>, NT, CP::smem());
+        if (e != hipSuccess) return e;
+        max_grid = std::max(1, cus * std::max(1, per_cu));
+    }
+    long long units = 0;
+    for (int c = 0; c < a->nclass; c++) units += (class_nrows[c] + G - 1) / G;
+    if (units < 1) return hipSuccess;
+    PzlCorrArgs a2 = *a;
+    a2.total_items = units;
+    const int grid = (int)std::min<long long>(units, max_grid);
+    hipLaunchKernelGGL((k_corr<N, NT, G, PING, Rs...>), dim3(grid), dim3(NT), CP::smem(), st, a2);
     return hipGetLastError();
 }
 
-extern "C" int pzl_corr_supported(int n)
+typedef hipError_t (*CorrLaunch)(const PzlCorrArgs *, const int *, hipStream_t);
+struct CorrVariant {
+    int n;
+    CorrLaunch fn;
+    const char *name;
+};
+
+#define PZL_CV(N, NT, G, PING, ...) \
+    {N, launch_corr_t<N, NT, G, PING, __VA_ARGS__>, #N ":" #NT "t:" #G "r:" #PING ":" #__VA_ARGS__}
+
+// Candidates per FFT length; the first of each length is the default, the
+// autotuner may pick another on the device.
+static const CorrVariant g_corr_variants[] = {
+    PZL_CV(128, 128, 16, true, 16, 8),
+    PZL_CV(128, 256, 16, true, 8, 4, 4),
+    PZL_CV(192, 256, 16, true, 12, 4, 4),
+    PZL_CV(192, 64, 4, true, 12, 4, 4),
+    PZL_CV(256, 128, 8, true, 16, 16),
+    PZL_CV(256, 256, 8, true, 8, 8, 4),
+    PZL_CV(384, 256, 8, true, 6, 4, 4, 4),
+    PZL_CV(384, 64, 2, true, 12, 4, 4, 2),
+    PZL_CV(512, 128, 4, true, 8, 8, 8),
+    PZL_CV(512, 256, 4, true, 8, 8, 8),
+    PZL_CV(512, 128, 4, true, 16, 16, 2),
+    PZL_CV(768, 256, 4, true, 12, 4, 4, 4),
+    PZL_CV(768, 64, 1, true, 12, 4, 4, 4),
+    PZL_CV(1024, 128, 2, true, 16, 16, 4),
+    PZL_CV(1024, 256, 2, true, 8, 8, 4, 4),
+    PZL_CV(1024, 64, 1, true, 16, 16, 4),
+    PZL_CV(1280, 128, 1, true, 16, 16, 5),
+    PZL_CV(1280, 256, 2, true, 16, 16, 5),
+    PZL_CV(2048, 128, 1, true, 16, 16, 8),
+    PZL_CV(2048, 256, 1, true, 8, 8, 8, 4),
+    PZL_CV(2048, 256, 2, false, 16, 16, 8),
+    PZL_CV(4096, 256, 1, false, 16, 16, 16),
+    PZL_CV(4096, 512, 1, false, 8, 8, 8, 8),
+};
+static const int g_ncorr_variants = (int)(sizeof(g_corr_variants) / sizeof(g_corr_variants[0]));
+
+static const CorrVariant *corr_variant(int n, int v)
 {
-    switch (n) {
-    case 128: case 192: case 256: case 384: case 512: case 768: case 1024: case 1280: case 2048: case 4096:
-        return 1;
-    default:
-        return 0;
-    }
+    int k = 0;
+    for (int i = 0; i < g_ncorr_variants; i++)
+        if (g_corr_variants[i].n == n) {
+            if (k == v) return &g_corr_variants[i];
+            k++;
+        }
+    return nullptr;
 }
 
-extern "C" hipError_t pzl_launch_corr(int N, const PzlCorrArgs *a, int nblocks, hipStream_t st)
+extern "C" int pzl_corr_supported(int n) { return corr_variant(n, 0) != nullptr; }
+
+extern "C" int pzl_corr_num_variants(int n)
 {
-    switch (N) {
-    case 128: return launch_corr_t<128, 16, 256, 8, 4, 4>(a, nblocks, st);
-    case 192: return launch_corr_t<192, 16, 256, 3, 4, 4, 4>(a, nblocks, st);
-    case 256: return launch_corr_t<256, 8, 256, 8, 8, 4>(a, nblocks, st);
-    case 384: return launch_corr_t<384, 8, 256, 6, 4, 4, 4>(a, nblocks, st);
-    case 512: return launch_corr_t<512, 4, 256, 8, 8, 8>(a, nblocks, st);
-    case 768: return launch_corr_t<768, 4, 256, 3, 4, 4, 4, 4>(a, nblocks, st);
-    case 1024: return launch_corr_t<1024, 2, 256, 8, 8, 4, 4>(a, nblocks, st);
-    case 1280: return launch_corr_t<1280, 4, 256, 5, 4, 4, 4, 4>(a, nblocks, st);
-    case 2048: return launch_corr_t<2048, 1, 256, 8, 8, 8, 4>(a, nblocks, st);
-    case 4096: return launch_corr_t<4096, 1, 512, 8, 8, 8, 8>(a, nblocks, st);
-    default: return hipErrorInvalidValue;
-    }
+    int k = 0;
+    for (int i = 0; i < g_ncorr_variants; i++) k += g_corr_variants[i].n == n;
+    return k;
+}
+
+extern "C" const char *pzl_corr_variant_name(int n, int v)
+{
+    const CorrVariant *cv = corr_variant(n, v);
+    return cv ? cv->name : "";
+}
+
+extern "C" hipError_t pzl_launch_corr(int N, int variant, const PzlCorrArgs *a, const int *class_nrows,
+                                      hipStream_t st)
+{
+    const CorrVariant *cv = corr_variant(N, variant);
+    if (!cv) return hipErrorInvalidValue;
+    return cv->fn(a, class_nrows, st);
 }
 
 extern "C" hipError_t pzl_launch_harmsum(const PzlSumArgs *a, hipStream_t st)
 {
     const int chunks = (a->corr_uselen + SUM_NT - 1) / SUM_NT;
-    const long long nblk = (long long)a->nb * a->numw * a->numz * chunks;
-    const dim3 g((unsigned)nblk), t(SUM_NT);
+    const long long nblk = (long long)a->nb * a->numw * ((a->numz + SUM_ZB - 1) / SUM_ZB) * chunks;
+    const dim3 g((unsigned)(8 * ((nblk + 7) / 8))), t(SUM_NT);
     switch (a->nstages) {
     case 1: hipLaunchKernelGGL(k_harmsum<1>, g, t, 0, st, *a); break;
     case 2: hipLaunchKernelGGL(k_harmsum<2>, g, t, 0, st, *a); break;

@@ -25,7 +25,9 @@ extern thread_local std::string g_err;
 struct SizeClass {
     int fftlen;
     std::vector<int> harms;
+    std::vector<int> nrows;  // kernel rows of each harmonic (host copy for the launcher)
     int *d_harms = nullptr;
+    int variant = 0;         // correlation kernel variant (autotuned)
 };
 
 // create_accelobs (accel_utils.c:1636-2037) + the harmonic shapes of

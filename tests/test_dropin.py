@@ -249,7 +249,7 @@ def test_presto_loop_through_plane_api(tmp_path, config1_data):
     got = D.read_list(cands)
     ref_obs = O.create_accelobs(O.make_params(N, dt, len(fft), 20, 0, 4, 2.0, wmaxP=True))
     ctx = O.Ctx(ref_obs, O.FP32, nthreads=8)
-    exp, _hits = ctx.search(fft, lo, hi)
+    exp, _hits, _ = ctx.search(fft, lo, hi)
     ctx.close()
     print("plane API: %d candidates, oracle %d" % (len(got), len(exp)))
     assert [(c[2], c[3], c[4], c[5]) for c in got] == [(c[2], c[3], c[4], c[5]) for c in exp]
@@ -282,10 +282,10 @@ def test_driver_end_to_end(tmp_path, config1_data):
                        capture_output=True, text=True, timeout=300, cwd=str(tmp_path))
     print(r.stdout[-2000:])
     assert r.returncode == 0, r.stdout + r.stderr
-    got = read_pzlcands(root + "_ACCEL_20_JERK_0.pzlcands")
+    got = read_pzlcands(root + "_ACCEL_20.pzlcands")  # wmax 0: numw = 0, no _JERK_ (accel_utils.c:1873-1893)
     ref_obs = O.create_accelobs(O.make_params(N, dt, len(fft), 20, 0, 4, 2.0, wmaxP=True))
     ctx = O.Ctx(ref_obs, O.FP32, nthreads=8)
-    exp, _hits = ctx.search(fft)
+    exp, _hits, _ = ctx.search(fft)
     ctx.close()
     exp = O.eliminate_harmonics(O.sort_cands(exp))
     print("driver: %d candidates, oracle %d" % (len(got), len(exp)))

@@ -198,3 +198,53 @@ def test_batch_size_invariance(config1_data):
         res.append(c)
         plan.close()
     assert res[0] == res[1] == res[2]
+
+
+def _variant_shapes(config1_data):
+    from prestozl_amd import synth
+    fft1, N1, dt1, _ = config1_data
+    # numharm 32 at small z: FFT lengths 128 ... 2048 (six stages)
+    a = (fft1, dict(N=N1, dt=dt1, numbins=len(fft1), zmax=4, wmax=0, numharm=32, sigma=3.0, wmaxP=True,
+                    batchsize=4))
+    fft3, N3, dt3, opts, _ = synth.config3(N=1 << 21)
+    # zmax 200 / wmax 500: the 4096-point fundamental
+    b = (fft3, dict(N=N3, dt=dt3, numbins=len(fft3), zmax=200, wmax=500, numharm=16, sigma=5.0, wmaxP=True,
+                    batchsize=2))
+    return [a, b]
+
+
+@pytest.mark.parametrize("shape", [0, 1])
+def test_corr_variants(config1_data, shape):
+    """Every correlation-kernel variant of every FFT length gives the oracle's
+    planes (FP64 correlation, tolerance TOL_POW)."""
+    from prestozl_amd import binding as B
+    fft, kw = _variant_shapes(config1_data)[shape]
+    plan = _plan(kw, fft)
+    obs, ctx64 = _oracle(kw, O.FP64)
+    hl = O.harmonics(obs)
+    nr = plan.geom.nrsteps
+    rstep_bins = plan.geom.corr_uselen // 2
+    rstep = nr // 2
+    startr = obs.rlo + rstep * rstep_bins
+    by_len = {}
+    for idx in range(len(hl)):
+        by_len.setdefault(plan.harms[idx].fftlen, []).append(idx)
+    tested = 0
+    for n, idxs in sorted(by_len.items()):
+        # the largest and the smallest harmonic of the class
+        pick = sorted({idxs[0], idxs[-1]})
+        refs = {}
+        for idx in pick:
+            nh, hn = hl[idx]
+            refs[idx] = ctx64.subharm(fft, nh, hn, startr, startr + rstep_bins - 0.5)[0]
+        for v in range(B.lib().pzl_corr_num_variants(n)):
+            plan.set_variant(n, v)
+            for idx in pick:
+                pg, _ = plan.plane(rstep, idx)
+                e, _ = rel_err(pg, refs[idx], floor=1.0)
+                name = B.lib().pzl_corr_variant_name(n, v).decode()
+                print("fftlen %d variant %s harmonic %s: max rel err %.3g" % (n, name, hl[idx], e))
+                assert e < TOL_POW, name
+                tested += 1
+    assert tested >= 2 * len(by_len)
+    plan.close()
