@@ -315,3 +315,154 @@ def test_subharm_fderivs_vol_vs_direct_correlation():
         ref = np.abs(y[off:off + pows.shape[2]]) ** 2 / N ** 2
         assert np.abs(pows[wi, zi] - ref).max() < 1e-9 * max(ref.max(), 1.0)
     ctx.close()
+
+
+# ---------------- committed reference vectors (tests/golden) ---------------
+
+def _golden():
+    import json
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ref_vectors.json")) as f:
+        return json.load(f)
+
+
+def test_golden_fresnel_and_median():
+    g = _golden()
+    for x, s, c in g["fresnl"]:
+        os_, oc = O.fresnel(x)
+        assert abs(os_ - s) < 1e-14 and abs(oc - c) < 1e-14, x
+    for arr, m in g["median"]:
+        assert O.median(np.array(arr, dtype=np.float32)) == np.float32(m)
+
+
+def test_golden_power_for_sigma():
+    for sigma, numsum, ntrials, p in _golden()["power_for_sigma"]:
+        assert abs(O.power_for_sigma(sigma, numsum, ntrials) - p) < 2e-7 * p, (sigma, numsum)
+
+
+def test_golden_candidate_sigma():
+    g = _golden()["candidate_sigma"]
+    assert len(g) > 40
+    for power, numsum, ntrials, s in g:
+        assert abs(O.candidate_sigma(power, numsum, ntrials) - s) < 1e-9 * max(s, 1.0), (power, numsum)
+
+
+# ---------------- kernel placement (init_kernel, accel_utils.c:151-168) ------
+
+@pytest.mark.parametrize("nh,hn,zi,wi", [(1, 1, 0, 0), (1, 1, 7, 3), (2, 1, 5, 1), (4, 3, 0, 2)])
+def test_kernel_is_fft_of_wrap_placed_response(nh, hn, zi, wi):
+    """The FFT'd kernel's inverse is the w/z response of numkern = 4 * halfwidth
+    points with bin-zero response (index numkern/2) at index 0: the second
+    half at the start, the first half at the end, zeros between
+    (place_complex_kernel, corr_prep.c:92-114; forward FFT, sign -1)."""
+    obs = O.create_accelobs(O.make_params(1 << 20, 1e-4, 1 << 19, 40, 100, 4, 3.0))
+    h = O.harm_info(obs, nh, hn)
+    z = -h.zmax + 2 * zi
+    w = -h.wmax + 20 * wi
+    hw = O.w_resp_halfwidth(float(z), float(w))
+    numkern = 4 * hw
+    resp = O.response("w", 0.0, 2, float(z), float(w), numkern, mode=O.FP64)
+    kfft = O.kernel(obs, nh, hn, zi, wi, O.FP64)
+    data = np.fft.ifft(kfft)
+    N = h.fftlen
+    half = numkern // 2
+    exp = np.zeros(N, dtype=np.complex128)
+    exp[:half] = resp[half:]
+    exp[N - half:] = resp[:half]
+    assert np.abs(data - exp).max() < 1e-12 * np.abs(resp).max()
+
+
+# ---------------- harmonic summing and candidate insertion -----------------
+
+def _nearest_even_half(x):
+    """nearest multiple of 0.5, ties to the even multiple (rint of 2x)."""
+    return np.round(2.0 * x) / 2.0
+
+
+def _nearest_away(x, step):
+    """nearest multiple of 'step', ties away from zero (NEAREST_INT)."""
+    q = x / step
+    return np.sign(q) * np.floor(np.abs(q) + 0.5) * step
+
+
+def test_summed_plane_brute_force():
+    """The oracle's summed fundamental plane equals, bit for bit, float32
+    additions in the reference's order (stage 1..S, odd harmonics ascending,
+    accelsearch.c:199-211) of each subharmonic plane read at the grid point
+    nearest to h times the fundamental (r on the 0.5 grid with ties to even,
+    z on the 2 grid and w on the 20 grid with ties away from zero)."""
+    rng = np.random.default_rng(21)
+    nbins = 1 << 15
+    fft = (rng.standard_normal(nbins) + 1j * rng.standard_normal(nbins)).astype(np.complex64)
+    for b in (4000, 9000, 12345):
+        fft[b] += 30.0
+    obs = O.create_accelobs(O.make_params(2 * nbins, 1e-3, nbins, 12, 40, 8, 2.0))
+    ctx = O.Ctx(obs, O.FP32, 4)
+    rstep = 5
+    rbins = obs.corr_uselen // 2
+    fullrlo = obs.rlo + rstep * rbins
+    fullrhi = fullrlo + rbins - 0.5
+    fund, _, _, fmeta = ctx.subharm(fft.view(np.float32), 1, 1, fullrlo, fullrhi)
+    acc = fund[:, :, : obs.corr_uselen].astype(np.float32).copy()
+    h0 = O.harm_info(obs, 1, 1)
+    zf = -h0.zmax + 2.0 * np.arange(h0.numz)
+    wf = -h0.wmax + 20.0 * np.arange(h0.numw)
+    rf = fullrlo + 0.5 * np.arange(obs.corr_uselen)
+    for stage in range(1, obs.numharmstages):
+        for hn in range(1, 1 << stage, 2):
+            nh = 1 << stage
+            hf = hn / nh
+            sub, rinds, zinds, meta = ctx.subharm(fft.view(np.float32), nh, hn, fullrlo, fullrhi)
+            rlo_h, zlo_h, wlo_h = meta[0], meta[1], meta[2]
+            exp_r = ((_nearest_even_half(rf * hf) - rlo_h) * 2).astype(int)
+            exp_z = ((_nearest_away(zf * hf, 2.0) - zlo_h) / 2).astype(int)
+            exp_w = ((_nearest_away(wf * hf, 20.0) - wlo_h) / 20).astype(int)
+            assert np.array_equal(rinds[: obs.corr_uselen], exp_r), (nh, hn)
+            assert np.array_equal(zinds[: h0.numz], exp_z), (nh, hn)
+            acc = acc + sub[exp_w][:, exp_z][:, :, exp_r]
+        # after each stage the oracle's running sum is the next stage's input;
+        # the final one is checked below
+    _, _, sums = ctx.search(fft.view(np.float32), rstep, rstep + 1, sum_rstep=rstep)
+    assert sums.dtype == np.float32
+    assert np.array_equal(sums, acc)
+    ctx.close()
+
+
+def test_insertion_known_answer():
+    """A hit sequence worked by hand through insert_new_accelcand
+    (accel_utils.c:382-480): replace a weaker neighbour, keep a stronger one,
+    append past the tail, replace-and-remove, insert at the head."""
+    seq = [(100.0, 5.0), (110.0, 4.0), (112.0, 6.0), (130.0, 3.0), (120.0, 7.0), (50.0, 2.0)]
+    rr = [s[0] for s in seq]
+    sg = [s[1] for s in seq]
+    n = len(seq)
+    out = O.insert_cands(rr, sg, [1.0] * n, [1] * n, [0.0] * n, [0.0] * n)
+    assert [(c[3], c[1]) for c in out] == [(50.0, 2.0), (120.0, 7.0)]
+
+
+def test_insertion_invariants():
+    """Random hit streams: every candidate is one of the hits.  Hits more than
+    15 bins apart all survive (in r order); hits within 15 bins of one another
+    with rising sigma collapse onto the last; with falling sigma onto the first.
+    (Two candidates can end up closer than 15 bins: a hit may overwrite the
+    previous node while a stronger next node stays.)"""
+    rng = np.random.default_rng(5)
+    for trial in range(20):
+        n = 2000
+        rr = np.sort(rng.uniform(0, 3000, size=n)) if trial % 2 else rng.uniform(0, 3000, size=n)
+        rr = np.round(rr * 8) / 8
+        sg = rng.uniform(2, 9, size=n).astype(np.float32)
+        out = O.insert_cands(rr, sg, np.ones(n), np.ones(n, dtype=np.int32), np.zeros(n), np.zeros(n))
+        hits = set(zip(rr.tolist(), sg.tolist()))
+        for c in out:
+            assert (c[3], c[1]) in hits
+    far = rng.permutation(np.arange(40) * 16.5 + 3.0)
+    out = O.insert_cands(far, np.full(40, 5.0), np.ones(40), np.ones(40, dtype=np.int32), np.zeros(40),
+                         np.zeros(40))
+    assert [c[3] for c in out] == sorted(far.tolist())
+    near = 500.0 + np.arange(10) * 1.25
+    up = O.insert_cands(near, np.arange(10) + 3.0, np.ones(10), np.ones(10, dtype=np.int32), np.zeros(10),
+                        np.zeros(10))
+    assert [(c[3], c[1]) for c in up] == [(near[-1], 12.0)]
+    down = O.insert_cands(near, 12.0 - np.arange(10), np.ones(10), np.ones(10, dtype=np.int32), np.zeros(10),
+                          np.zeros(10))
+    assert [(c[3], c[1]) for c in down] == [(near[0], 12.0)]
