@@ -660,6 +660,9 @@ int pzl_plan_create(const PzlConfig *cfg, int device, void **plan_out)
     p->seg_stride = 0;
     for (auto &H : p->harms) p->seg_stride += H.fftlen / 2;
     p->plane_stride = p->g.plane_bytes_per_rstep / 4;
+    if (p->g.plane_bytes_per_rstep >= (1LL << 31))
+        return cleanup(fail(PZL_ERR_UNSUPPORTED, "the planes of one rstep need %lld bytes (at most 2 GiB)",
+                            p->g.plane_bytes_per_rstep));
     PCHK(hipMalloc(&p->d_segs, sizeof(float2) * p->seg_stride * p->B));
     PCHK(hipMalloc(&p->d_rinds, sizeof(uint16_t) * (size_t)p->B * nh * p->g.corr_uselen));
     PCHK(hipMalloc(&p->d_hr, sizeof(PzlHrTab) * (size_t)p->B * nh));

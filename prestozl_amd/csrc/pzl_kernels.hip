@@ -267,6 +267,22 @@ __global__ __launch_bounds__(SEG_NT) void k_segments(PzlBatchArgs a)
 
 __host__ __device__ constexpr int lds_pad(int p) { return p + (p >> 4); }
 
+// Word 3 of a raw (stride 0) buffer resource on gfx9-family parts: 32-bit
+// data format; with stride 0, an access at byte offset >= num_records is
+// dropped (stores) or returns 0 (loads).
+#define PZL_BUF_DWORD3 0x00020000
+
+// A buffer resource built from wave-uniform values (read from the first
+// lane, so that the compiler keeps it in scalar registers).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t uniform_rsrc(const void *p, int nbytes)
+{
+    const unsigned long long addr = (unsigned long long)p;
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)addr);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(addr >> 32));
+    const int n = __builtin_amdgcn_readfirstlane(nbytes);
+    return __builtin_amdgcn_make_buffer_rsrc((void *)(((unsigned long long)hi << 32) | lo), 0, n, PZL_BUF_DWORD3);
+}
+
 template <int N, int NT, int G, bool PING, int... Rs> struct CorrPlan {
     static constexpr int NP = sizeof...(Rs);
     static constexpr int RAD[NP] = {Rs...};
@@ -340,9 +356,11 @@ template <int N, int NT, int G, bool PING, int... Rs> struct CorrFFT {
             if (CP::full(P) || bf < CP::nbf(P)) {
                 int g, j;
                 where<P>(bf, g, j);
+                // stride NBP: with NBP a multiple of 16 the padded index is linear in r
+                const float2 *rp = src + lds_pad(g * N + j);
 #pragma unroll
                 for (int r = 0; r < R; r++) {
-                    float2 x = src[lds_pad(g * N + j + r * NBP)];
+                    float2 x = (NBP % 16 == 0) ? rp[r * lds_pad(NBP)] : src[lds_pad(g * N + j + r * NBP)];
                     if (r > 0) x = cmul(x, tw[CP::twoff(P) + i * (R - 1) + (r - 1)]);
                     v[i][r] = x;
                 }
@@ -358,8 +376,25 @@ template <int N, int NT, int G, bool PING, int... Rs> struct CorrFFT {
                 dftR<R, 1>(v[i]);
                 if constexpr (!last) {
                     const int idxD = (j / Ns) * Ns * R + (j % Ns);
+                    float2 *wp = dst + lds_pad(g * N + idxD);
 #pragma unroll
-                    for (int r = 0; r < R; r++) dst[lds_pad(g * N + idxD + r * Ns)] = v[i][r];
+                    for (int r = 0; r < R; r++) {
+                        if constexpr (Ns % 16 == 0) wp[r * lds_pad(Ns)] = v[i][r];
+                        else dst[lds_pad(g * N + idxD + r * Ns)] = v[i][r];
+                    }
+                } else if constexpr (NBP % 64 == 0) {
+                    // g is uniform in a wave: a buffer store whose range is the
+                    // row's numrs powers drops the outputs outside the r window
+                    // (negative offsets wrap to out of range) without branches
+                    const int gu = __builtin_amdgcn_readfirstlane(g);
+                    const __amdgpu_buffer_rsrc_t rs = uniform_rsrc(pbase + gu * rs_stride, gu < gvalid ? numrs * 4 : 0);
+#pragma unroll
+                    for (int r = 0; r < R; r++) {
+                        const int ro = j + r * NBP - offset;  // natural-order index minus offset
+                        const float2 y = v[i][r];
+                        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint((y.x * y.x + y.y * y.y) * norm), rs,
+                                                              ro * 4, 0, 0);
+                    }
                 } else {
                     if (g < gvalid) {
                         float *pl = pbase + g * rs_stride;
@@ -412,8 +447,10 @@ __global__ __launch_bounds__(NT) void k_corr(PzlCorrArgs a)
                 acc += cnt;
             }
         }
+        h = __builtin_amdgcn_readfirstlane(h);
+        row0 = __builtin_amdgcn_readfirstlane(row0);
         const PzlHarm &H = a.harms[h];
-        const int nrows = H.nrows;
+        const int nrows = __builtin_amdgcn_readfirstlane(H.nrows);
         // the kernel rows, at the positions pass 0 reads them
         float2 kv[CP::per(0)][R0];
 #pragma unroll
@@ -421,16 +458,54 @@ __global__ __launch_bounds__(NT) void k_corr(PzlCorrArgs a)
             const int bf = tid + i * NT;
             int g, j;
             F::template where<0>(bf, g, j);
-            const bool ok = (CP::full(0) || bf < CP::nbf(0)) && row0 + g < nrows;
-            const float2 *kr = a.bank + H.bank_off + (long long)(row0 + g) * N;
+            if constexpr (NB0 % 64 == 0 && CP::full(0)) {
+                // g is uniform in a wave: rows past the harmonic read as zeros
+                const int gu = __builtin_amdgcn_readfirstlane(g);
+                const __amdgpu_buffer_rsrc_t rs =
+                    uniform_rsrc(a.bank + H.bank_off + (long long)(row0 + gu) * N, row0 + gu < nrows ? N * 8 : 0);
 #pragma unroll
-            for (int r = 0; r < R0; r++) kv[i][r] = ok ? kr[j + r * NB0] : float2{0.f, 0.f};
+                for (int r = 0; r < R0; r++) {
+                    const int k = j + r * NB0;
+                    kv[i][r].x = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, k * 8, 0, 0));
+                    kv[i][r].y = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, k * 8 + 4, 0, 0));
+                }
+            } else {
+                const bool ok = (CP::full(0) || bf < CP::nbf(0)) && row0 + g < nrows;
+                const float2 *kr = a.bank + H.bank_off + (long long)(row0 + g) * N;
+#pragma unroll
+                for (int r = 0; r < R0; r++) kv[i][r] = ok ? kr[j + r * NB0] : float2{0.f, 0.f};
+            }
         }
         const int offset = 2 * H.halfwidth;
-        for (int b = 0; b < a.nb; b++) {
+        // segment values of the next rstep are loaded while this one is transformed
+        constexpr int NSV = R0 % 2 == 0 ? R0 / 2 : R0;
+        float2 sv[CP::per(0)][NSV];
+        auto load_seg = [&](int b) {
             const float2 *seg = a.segs + (long long)b * a.seg_stride + H.seg_off;
-            const int numrs = a.hr_tab[b * a.nh + h].numrs;
+#pragma unroll
+            for (int i = 0; i < CP::per(0); i++) {
+                const int bf = tid + i * NT;
+                int g, j;
+                F::template where<0>(bf, g, j);
+                if (CP::full(0) || bf < CP::nbf(0)) {
+#pragma unroll
+                    for (int r = 0; r < NSV; r++) {
+                        const int k = j + r * NB0;
+                        sv[i][r] = seg[k < N / 2 ? k : k - N / 2];
+                    }
+                }
+            }
+        };
+        load_seg(0);
+        for (int b = 0; b < a.nb; b++) {
+            const int numrs = __builtin_amdgcn_readfirstlane(a.hr_tab[b * a.nh + h].numrs);
             float *pbase = a.planes + (long long)b * a.plane_stride + H.plane_off + (long long)row0 * H.rs_stride;
+            float2 cur[CP::per(0)][NSV];
+#pragma unroll
+            for (int i = 0; i < CP::per(0); i++)
+#pragma unroll
+                for (int r = 0; r < NSV; r++) cur[i][r] = sv[i][r];
+            if (b + 1 < a.nb) load_seg(b + 1);
             // pass 0: P[k] = seg[k mod N/2] * conj(kern[k]); the spread segment repeats
             // with period N/2, so an even radix reads each segment value once
 #pragma unroll
@@ -443,7 +518,7 @@ __global__ __launch_bounds__(NT) void k_corr(PzlCorrArgs a)
                     if constexpr (R0 % 2 == 0) {
 #pragma unroll
                         for (int r = 0; r < R0 / 2; r++) {
-                            const float2 pd = seg[j + r * NB0];
+                            const float2 pd = cur[i][r];
                             const float2 k1 = kv[i][r], k2 = kv[i][r + R0 / 2];
                             // re = dr*kr + di*ki, im = di*kr - dr*ki (accel_utils.c:1262-1265)
                             v[r] = float2{pd.x * k1.x + pd.y * k1.y, pd.y * k1.x - pd.x * k1.y};
@@ -452,8 +527,7 @@ __global__ __launch_bounds__(NT) void k_corr(PzlCorrArgs a)
                     } else {
 #pragma unroll
                         for (int r = 0; r < R0; r++) {
-                            const int k = j + r * NB0;
-                            const float2 pd = seg[k < N / 2 ? k : k - N / 2];
+                            const float2 pd = cur[i][r];
                             const float2 kk = kv[i][r];
                             v[r] = float2{pd.x * kk.x + pd.y * kk.y, pd.y * kk.x - pd.x * kk.y};
                         }
@@ -477,26 +551,27 @@ __global__ __launch_bounds__(NT) void k_corr(PzlCorrArgs a)
 
 #define SUM_NT 256
 
-#define SUM_ZB 8  // fundamental z rows per workgroup
+#define SUM_ZB 32  // fundamental z rows per workgroup
 
 // One workgroup: SUM_NT consecutive r of SUM_ZB consecutive z rows of one
 // (rstep, w) of the fundamental.  The subharmonic r indices depend only on
 // (rstep, harmonic, r), so each thread computes them once (integer
 // round-half-even of m j / 2^s split into a uniform 64-bit part and a small
-// per-thread 32-bit part) and keeps them in registers; the row offsets of the
-// subharmonic z/w maps for the workgroup's rows are staged in LDS.  For every
-// cell all subharmonic powers are loaded first and then added in the
-// reference's order (stage 1..S, odd harmonics ascending).
+// per-thread 32-bit part) and keeps them in registers as byte offsets; the
+// row offsets of the subharmonic z/w maps for the workgroup's rows are staged
+// in LDS and used as the scalar offset of buffer loads over the rstep's
+// planes.  For every cell all subharmonic powers are loaded first and then
+// added in the reference's order (stage 1..S, odd harmonics ascending).
 //
 // Workgroup ids are dealt round-robin over the 8 XCDs; each XCD is given a
-// contiguous range of the (rstep, w, z block, r chunk) space so that the
-// subharmonic rows that neighbouring z/w rows share are fetched into one L2.
+// contiguous range of the (rstep, r chunk, z block, w) space, w fastest:
+// neighbouring w rows share subharmonic rows, which then meet in one L2.
 template <int NSTAGES>
 __global__ __launch_bounds__(SUM_NT) void k_harmsum(PzlSumArgs a)
 {
     constexpr int NSUB = (1 << (NSTAGES - 1)) - 1;
     constexpr int NQ = NSUB > 0 ? NSUB : 1;
-    __shared__ long long zoff[NQ][SUM_ZB];
+    __shared__ int zoff[NQ][SUM_ZB];
     const int tid = threadIdx.x;
     const int chunks = (a.corr_uselen + SUM_NT - 1) / SUM_NT;
     const int zblocks = (a.numz + SUM_ZB - 1) / SUM_ZB;
@@ -504,20 +579,22 @@ __global__ __launch_bounds__(SUM_NT) void k_harmsum(PzlSumArgs a)
     const long long per_xcd = (nblk + 7) / 8;
     const long long L = (long long)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
     if (L >= nblk) return;
-    const int chunk = (int)(L % chunks);
-    long long t = L / chunks;
+    const int w = (int)(L % a.numw);
+    long long t = L / a.numw;
     const int zb = (int)(t % zblocks);
     t /= zblocks;
-    const int w = (int)(t % a.numw);
-    const int b = (int)(t / a.numw);
+    const int chunk = (int)(t % chunks);
+    const int b = (int)(t / chunks);
     const int z0 = zb * SUM_ZB;
     const int nz = min(SUM_ZB, a.numz - z0);
     const int r = chunk * SUM_NT + tid;
     const bool active = r < a.corr_uselen;
     const int rr = active ? r : a.corr_uselen - 1;
     const long long rstep = a.rstep0 + b;
-    const float *base = a.planes + (long long)b * a.plane_stride;
     const PzlHarm F = a.harms[0];
+    // every plane of this rstep, as one buffer (byte offsets < 2^31)
+    const __amdgpu_buffer_rsrc_t planes =
+        uniform_rsrc(a.planes + (long long)b * a.plane_stride, (int)(a.plane_stride * 4));
 
     if constexpr (NSUB > 0) {
         for (int k = tid; k < NSUB * SUM_ZB; k += SUM_NT) {
@@ -525,17 +602,17 @@ __global__ __launch_bounds__(SUM_NT) void k_harmsum(PzlSumArgs a)
             const int h = q + 1;
             const PzlHarm &H = a.harms[h];
             const int z = min(z0 + zi, a.numz - 1);
-            zoff[q][zi] = H.plane_off +
-                          ((long long)a.winds[h * a.numw + w] * H.numz + a.zinds[h * a.numz + z]) * H.rs_stride;
+            zoff[q][zi] = (int)(4 * (H.plane_off + ((long long)a.winds[h * a.numw + w] * H.numz +
+                                                    a.zinds[h * a.numz + z]) * H.rs_stride));
         }
     }
-    int rind[NQ];
+    int rind[NQ];  // byte offsets within a subharmonic row
     if constexpr (NSUB > 0) {
-        const long long m0 = a.int_rlo ? (long long)(2.0 * a.startr[rstep]) : 0;
+        if (a.int_rlo) {
+            const long long m0 = (long long)(2.0 * a.startr[rstep]);
 #pragma unroll
-        for (int q = 0; q < NSUB; q++) {
-            const int h = q + 1;
-            if (a.int_rlo) {
+            for (int q = 0; q < NSUB; q++) {
+                const int h = q + 1;
                 // rint((m0 + rr) j / 2^s) - 2 rlo_h with m0 j split as q0 2^s + rem0
                 const int j = a.harms[h].harmnum, sh = a.harms[h].stage;
                 const long long p0 = m0 * j;
@@ -545,28 +622,31 @@ __global__ __launch_bounds__(SUM_NT) void k_harmsum(PzlSumArgs a)
                 const int tt = rem0 + rr * j;
                 int qq = tt >> sh;
                 const int rem = tt - (qq << sh);
-                if (sh > 0) {
-                    const int half = 1 << (sh - 1);
-                    // round half to even on the full quotient q0 + qq
-                    if (rem > half || (rem == half && ((q0 + qq) & 1))) qq++;
-                }
-                rind[q] = base_h + qq;
-            } else {
-                rind[q] = a.rinds[((long long)b * a.nh + h) * a.corr_uselen + rr];
+                const int half = (1 << sh) >> 1;  // sh >= 1 for every subharmonic
+                // round half to even on the full quotient q0 + qq
+                if (rem > half || (rem == half && ((q0 + qq) & 1))) qq++;
+                rind[q] = 4 * (base_h + qq);
             }
+        } else {
+#pragma unroll
+            for (int q = 0; q < NSUB; q++)
+                rind[q] = 4 * a.rinds[((long long)b * a.nh + q + 1) * a.corr_uselen + rr];
         }
     }
     __syncthreads();
 
     const unsigned long long lanemask_lt = (1ull << (__lane_id())) - 1ull;
-    const float *frow = base + F.plane_off + ((long long)w * a.numz + z0) * F.rs_stride;
+    const int frow = (int)(4 * (F.plane_off + ((long long)w * a.numz + z0) * F.rs_stride));
     for (int zi = 0; zi < nz; zi++) {
         const int z = z0 + zi;
-        float acc = frow[(long long)zi * F.rs_stride + rr];
+        float acc = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
+            planes, 4 * rr, __builtin_amdgcn_readfirstlane(frow + zi * 4 * (int)F.rs_stride), 0));
         float v[NQ];
         if constexpr (NSUB > 0) {
 #pragma unroll
-            for (int q = 0; q < NSUB; q++) v[q] = base[zoff[q][zi] + rind[q]];
+            for (int q = 0; q < NSUB; q++)
+                v[q] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
+                    planes, rind[q], __builtin_amdgcn_readfirstlane(zoff[q][zi]), 0));
         }
         int q = 0;
 #pragma unroll
@@ -856,6 +936,7 @@ extern "C" hipError_t pzl_launch_harmsum(const PzlSumArgs *a, hipStream_t st)
     const int chunks = (a->corr_uselen + SUM_NT - 1) / SUM_NT;
     const long long nblk = (long long)a->nb * a->numw * ((a->numz + SUM_ZB - 1) / SUM_ZB) * chunks;
     const dim3 g((unsigned)(8 * ((nblk + 7) / 8))), t(SUM_NT);
+    if (a->plane_stride * 4 >= (1LL << 31)) return hipErrorInvalidValue;  // buffer offsets are 32-bit
     switch (a->nstages) {
     case 1: hipLaunchKernelGGL(k_harmsum<1>, g, t, 0, st, *a); break;
     case 2: hipLaunchKernelGGL(k_harmsum<2>, g, t, 0, st, *a); break;
