@@ -61,6 +61,51 @@ def copy_calibrate(mib):
     print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "dispatches": 5}))
 
 
+def serve_bench(n, rate, max_inflight, seed=4):
+    """A served stream of the uneven-search workload: n requests arriving as
+    Poisson at 'rate' per second, lengths uniform in 2^21..2^24 samples at
+    dt = 1e-4 s, (zmax, wmax, numharm) uniform in (50, 0, 8), (200, 150, 16),
+    (200, 300, 16), sigma 5.  One synthetic spectrum per length (noise, plus a
+    jerked sinusoid for half of the lengths) is shared by the requests of that
+    length."""
+    from prestozl_amd import binding as B
+    from prestozl_amd import synth
+    rng = np.random.Generator(np.random.PCG64(seed))
+    spectra = {}
+    for lg in (21, 22, 23, 24):
+        sig = synth.random_signals(rng, 1) if rng.random() < 0.5 else []
+        spectra[lg] = synth.realfft_packed(synth.timeseries(1 << lg, 1e-4, sig, seed * 100 + lg))
+    shapes = [(50, 0, 8), (200, 150, 16), (200, 300, 16)]
+    reqs = [(int(rng.integers(21, 25)), shapes[int(rng.integers(0, 3))]) for _ in range(n)]
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n))
+    srv = B.Server(budget=0, max_inflight=max_inflight)
+    ids = []
+    t0 = time.perf_counter()
+    for k, (lg, (z, w, nh)) in enumerate(reqs):
+        dt_wait = t0 + arrivals[k] - time.perf_counter()
+        if dt_wait > 0:
+            time.sleep(dt_wait)
+        fft = spectra[lg]
+        cfg = B.make_config(N=1 << lg, dt=1e-4, numbins=len(fft), zmax=z, wmax=w, numharm=nh, sigma=5.0,
+                            wmaxP=w > 0, batchsize=8)
+        ids.append(srv.submit(cfg, fft))
+    res = [srv.wait(i) for i in ids]
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    lat = [r[1]["ms_total"] for r in res]
+    cells = sum(r[1]["cells"] for r in res)
+    return {"workload": "served stream: %d requests, Poisson %.2f/s, N in 2^21..2^24, (zmax,wmax,numharm) in "
+                        "(50,0,8),(200,150,16),(200,300,16), sigma 5" % (n, rate),
+            "throughput": cells / wall, "unit": "cells/s", "wall_s": wall,
+            "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                           "max": float(max(lat))},
+            "queue_ms_mean": float(np.mean([r[1]["ms_queue"] for r in res])),
+            "max_inflight": max_inflight, "peak_inflight": st["peak_inflight"],
+            "peak_admitted_GiB": st["peak_admitted_bytes"] / 2 ** 30,
+            "plans_reused": int(sum(r[1]["plan_reused"] for r in res))}
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of 'kernel' at this workload from the committed
     rocprofv3 --pmc measurement (FETCH_SIZE + WRITE_SIZE, copy-calibrated)."""
@@ -82,7 +127,10 @@ def main():
     ap.add_argument("--cpu-rsteps", type=int, default=24)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--plain", action="store_true", help="also time one-rstep-per-launch (plain GPU form)")
+    ap.add_argument("--no-plain", action="store_true", help="skip the one-rstep-per-launch (plain GPU form) search")
+    ap.add_argument("--serve", type=int, default=24, help="requests in the served-stream leg (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=1.5, help="Poisson arrival rate of the served stream (1/s)")
+    ap.add_argument("--serve-inflight", type=int, default=3)
     ap.add_argument("--no-tune", action="store_true", help="use the default correlation variants")
     ap.add_argument("--rsteps", type=int, default=0,
                     help="profiling runs only: search this many rsteps from the middle of the band")
@@ -176,7 +224,16 @@ def main():
         "candidates": len(cands),
         "hits": stats_all[0]["nhits"],
     }
-    if args.plain:
+    # the same search with every full batch replayed from a captured hipGraph
+    plan.set_graph(True)
+    plan.search(lo, hi, stats=False)  # captures
+    ts = time.perf_counter()
+    cg, _ = plan.search(lo, hi, stats=False)
+    tg = time.perf_counter() - ts
+    out["graph_replay"] = {"value": cells / tg, "unit": "cells/s", "ms_per_search": tg * 1e3,
+                           "same_candidates": cg == cands}
+    plan.set_graph(False)
+    if not args.no_plain and not args.rsteps:
         plan1 = B.Plan(B.make_config(batchsize=1, **kw))
         plan1.set_spectrum(fft)
         ts = time.perf_counter()
@@ -185,6 +242,8 @@ def main():
                             "same_candidates": c1 == cands}
         plan1.close()
     plan.close()
+    if args.serve and not args.rsteps:
+        out["serving"] = serve_bench(args.serve, args.serve_rate, args.serve_inflight)
     if not args.no_cpu:
         nthreads = args.cpu_threads or min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(fft, kw, args.cpu_rsteps, nthreads,

@@ -87,6 +87,8 @@ typedef struct PzlStats {
     double class_ms[16];          /* summed duration of that class's launches */
     long long class_launches[16];
     double class_bytes[16];       /* algorithmic bytes of that class's launches */
+    double ms_graph;              /* summed duration of graph-replayed batches */
+    long long graph_launches;
 } PzlStats;
 
 const char *pzl_last_error(void);
@@ -126,6 +128,11 @@ int pzl_search(void *plan, long long rstep_lo, long long rstep_hi, PzlCand *cand
  * same shape on the device.  report (optional) receives one line per variant.
  * Replaces the reference's per-shape cuFFT plan choice (accel_kernels.cu:96-158). */
 int pzl_autotune(void *plan, int reps, char *report, int report_len);
+/* Graph replay: every full batch of rsteps (segments, all correlation classes,
+ * harmonic sum) is launched as one captured hipGraph per hit slot, the batch
+ * index read from device memory.  Off by default.  Per-kernel timing is then
+ * reported per batch (ms_graph) instead of per kernel. */
+int pzl_plan_set_graph(void *plan, int on);
 /* The variant in use for an FFT length of the plan; set one explicitly. */
 int pzl_plan_variant(void *plan, int fftlen, int *variant);
 int pzl_plan_set_variant(void *plan, int fftlen, int variant);
@@ -155,6 +162,43 @@ int pzl_compute_sums(void *plan, long long rstep, float *out, long long maxfloat
 int pzl_insert_cands(const double *rr, const float *sigma, const float *power, const int *numharm,
                      const double *z, const double *w, long long n, PzlCand *out, long long maxout,
                      long long *nout);
+
+/* ---------------- request loop (continuous batching on one GPU) ----------
+ * Replaces the reference's multi-process pipeline
+ * (bin/accelsearch_pipeline_cu.py: pool_size processes of CPU1/GPU/CPU2
+ * threads per GPU).  Requests are admitted in arrival order when their HBM
+ * footprint (plan buffers + kernel bank) fits the budget beside the searches
+ * in flight and one of max_inflight execution slots is free; admitted
+ * searches run concurrently.  Plans are pooled per shape and reused; kernel
+ * banks are shared through the bank cache. */
+typedef struct PzlRequestStats {
+    double ms_queue, ms_run, ms_total;       /* submit->admit, admit->done, submit->done */
+    double t_submit_ms, t_admit_ms, t_end_ms; /* steady-clock time stamps */
+    long long footprint, nhits;
+    double cells;
+    int plan_reused;
+} PzlRequestStats;
+
+typedef struct PzlServerStats {
+    long long budget, admitted_bytes, peak_admitted_bytes, resident_bytes;
+    long long queued, completed, pooled_plans;
+    int inflight, peak_inflight;
+} PzlServerStats;
+
+/* hbm_budget <= 0: 80% of the device's free memory. */
+int pzl_server_create(int device, long long hbm_budget, int max_inflight, void **server);
+/* Queue a search; host_fft (numbins complex64) must stay valid until the
+ * request completes.  Fails if the request alone exceeds the budget. */
+int pzl_server_submit(void *server, const PzlConfig *cfg, const float *host_fft, long long numbins,
+                      long long *req_id);
+/* Block until the request is done; copy its candidates and timing and forget it. */
+int pzl_server_wait(void *server, long long req_id, PzlCand *cands, long long maxcands, long long *ncands,
+                    PzlRequestStats *stats);
+/* Ids of completed requests not yet waited for (n: how many there are). */
+int pzl_server_poll(void *server, long long *done_ids, int max_ids, int *n);
+int pzl_server_stats(void *server, PzlServerStats *stats);
+/* Finish every queued and running request, then free the pooled plans. */
+void pzl_server_destroy(void *server);
 
 #ifdef __cplusplus
 }

@@ -54,13 +54,32 @@ class Stats(C.Structure):
                 ("nrsteps", C.c_longlong), ("cells", C.c_double), ("bytes_alg", C.c_double),
                 ("bytes_alg_corr", C.c_double), ("nclass", C.c_int), ("class_fftlen", C.c_int * 16),
                 ("class_ms", C.c_double * 16), ("class_launches", C.c_longlong * 16),
-                ("class_bytes", C.c_double * 16)]
+                ("class_bytes", C.c_double * 16), ("ms_graph", C.c_double), ("graph_launches", C.c_longlong)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_ if not k.startswith("class_")}
         d["classes"] = [dict(fftlen=self.class_fftlen[i], ms=self.class_ms[i], launches=self.class_launches[i],
                              bytes=self.class_bytes[i]) for i in range(self.nclass)]
         return d
+
+
+class RequestStats(C.Structure):
+    _fields_ = [("ms_queue", C.c_double), ("ms_run", C.c_double), ("ms_total", C.c_double),
+                ("t_submit_ms", C.c_double), ("t_admit_ms", C.c_double), ("t_end_ms", C.c_double),
+                ("footprint", C.c_longlong), ("nhits", C.c_longlong), ("cells", C.c_double),
+                ("plan_reused", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("budget", C.c_longlong), ("admitted_bytes", C.c_longlong), ("peak_admitted_bytes", C.c_longlong),
+                ("resident_bytes", C.c_longlong), ("queued", C.c_longlong), ("completed", C.c_longlong),
+                ("pooled_plans", C.c_longlong), ("inflight", C.c_int), ("peak_inflight", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 _lib = None
@@ -97,7 +116,16 @@ def lib():
                                         P(C.c_int), P(C.c_longlong)]
         L.pzl_compute_sums.argtypes = [C.c_void_p, C.c_longlong, P(C.c_float), C.c_longlong]
         L.pzl_autotune.argtypes = [C.c_void_p, C.c_int, C.c_char_p, C.c_int]
+        L.pzl_server_create.argtypes = [C.c_int, C.c_longlong, C.c_int, P(C.c_void_p)]
+        L.pzl_server_submit.argtypes = [C.c_void_p, P(Config), P(C.c_float), C.c_longlong, P(C.c_longlong)]
+        L.pzl_server_wait.argtypes = [C.c_void_p, C.c_longlong, P(Cand), C.c_longlong, P(C.c_longlong),
+                                      P(RequestStats)]
+        L.pzl_server_poll.argtypes = [C.c_void_p, P(C.c_longlong), C.c_int, P(C.c_int)]
+        L.pzl_server_stats.argtypes = [C.c_void_p, P(ServerStats)]
+        L.pzl_server_destroy.argtypes = [C.c_void_p]
+        L.pzl_server_destroy.restype = None
         L.pzl_plan_variant.argtypes = [C.c_void_p, C.c_int, P(C.c_int)]
+        L.pzl_plan_set_graph.argtypes = [C.c_void_p, C.c_int]
         L.pzl_plan_set_variant.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.pzl_corr_num_variants.argtypes = [C.c_int]
         L.pzl_corr_variant_name.argtypes = [C.c_int, C.c_int]
@@ -189,6 +217,9 @@ class Plan:
         check(lib().pzl_autotune(self.h, int(reps), buf, len(buf)))
         return buf.value.decode().splitlines()
 
+    def set_graph(self, on=True):
+        check(lib().pzl_plan_set_graph(self.h, int(bool(on))))
+
     def variant(self, fftlen):
         v = C.c_int(0)
         check(lib().pzl_plan_variant(self.h, int(fftlen), C.byref(v)))
@@ -253,3 +284,58 @@ class Plan:
         out = np.zeros(n, dtype=np.float32)
         check(lib().pzl_compute_sums(self.h, rstep, _fptr(out), n))
         return out.reshape(h.numw, h.numz, self.geom.corr_uselen)
+
+
+class Server:
+    """The request loop (pzl_server_*): searches admitted by HBM footprint and
+    run concurrently on one GPU."""
+
+    def __init__(self, device=0, budget=0, max_inflight=2):
+        self.h = C.c_void_p()
+        check(lib().pzl_server_create(int(device), int(budget), int(max_inflight), C.byref(self.h)))
+        self._keep = {}
+
+    def submit(self, cfg, fft):
+        a = np.ascontiguousarray(fft)
+        if a.dtype == np.complex64:
+            a = a.view(np.float32)
+        assert a.dtype == np.float32
+        rid = C.c_longlong(0)
+        check(lib().pzl_server_submit(self.h, C.byref(cfg), _fptr(a), len(a) // 2, C.byref(rid)))
+        self._keep[rid.value] = a  # the spectrum stays valid until the request is done
+        return rid.value
+
+    def wait(self, rid, maxcands=1 << 16):
+        cands = (Cand * maxcands)()
+        n = C.c_longlong(0)
+        st = RequestStats()
+        try:
+            check(lib().pzl_server_wait(self.h, int(rid), cands, maxcands, C.byref(n), C.byref(st)))
+        finally:
+            self._keep.pop(rid, None)
+        out = [(cands[i].power, cands[i].sigma, cands[i].numharm, cands[i].r, cands[i].z, cands[i].w)
+               for i in range(min(n.value, maxcands))]
+        return out, st.as_dict()
+
+    def poll(self, maxn=1024):
+        ids = (C.c_longlong * maxn)()
+        n = C.c_int(0)
+        check(lib().pzl_server_poll(self.h, ids, maxn, C.byref(n)))
+        return [ids[i] for i in range(min(n.value, maxn))]
+
+    def stats(self):
+        st = ServerStats()
+        check(lib().pzl_server_stats(self.h, C.byref(st)))
+        return st.as_dict()
+
+    def close(self):
+        if self.h:
+            lib().pzl_server_destroy(self.h)
+            self.h = C.c_void_p()
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -128,6 +128,45 @@ int acquire_bank(Plan *p, bool *built)
     return 0;
 }
 
+// Batch size, hit-slot capacity and slot count of a plan (pzl_plan_create
+// uses them; the request loop uses them to size a request before admission).
+struct PlanSizes {
+    int B;
+    long long hitcap;
+    int nslots;
+    long long seg_stride;
+};
+
+PlanSizes plan_sizes(const PzlConfig &cfg, const PzlGeometry &g, const std::vector<PzlHarm> &harms)
+{
+    PlanSizes z;
+    z.B = cfg.batchsize > 0 ? cfg.batchsize : 8;
+    if (g.nrsteps > 0 && z.B > g.nrsteps) z.B = (int)g.nrsteps;
+    if (z.B < 1) z.B = 1;
+    const PzlHarm &F = harms[0];
+    // one rstep can never overflow a slot
+    const long long per_rstep = (long long)g.corr_uselen * F.numz * F.numw * g.numharmstages;
+    long long cap = cfg.maxhits > 0 ? cfg.maxhits : kDefaultHitCap;
+    cap = std::min<long long>(cap, per_rstep * z.B);
+    z.hitcap = std::max<long long>(cap, per_rstep);
+    const long long nbatches = (g.nrsteps + z.B - 1) / z.B;
+    z.nslots = (int)std::max<long long>(1, std::min<long long>(Plan::kSlots, nbatches));
+    z.seg_stride = 0;
+    for (auto &H : harms) z.seg_stride += H.fftlen / 2;
+    return z;
+}
+
+// HBM held by a plan: bank, spectrum, batch buffers, hit slots and ordering
+// buffers (sort scratch taken as its reported size, or estimated at 12 bytes
+// per hit when not yet known).
+long long plan_device_bytes(const PzlGeometry &g, const PlanSizes &z, int nh, long long sort_temp)
+{
+    if (sort_temp < 0) sort_temp = 12 * z.hitcap + (16LL << 20);
+    return g.bank_bytes + 8 * g.numbins + g.plane_bytes_per_rstep * z.B + (long long)sizeof(float2) * z.seg_stride * z.B +
+           (long long)(z.nslots * 12 + 12 + (long long)sizeof(PzlHostHit)) * z.hitcap + sort_temp +
+           (long long)sizeof(uint16_t) * z.B * nh * g.corr_uselen;
+}
+
 // Correlation variants chosen by pzl_autotune, per (device, FFT length,
 // batch size, row counts of the class): later plans of the same shape use them.
 std::mutex g_tune_mu;
@@ -176,8 +215,12 @@ void free_plan(Plan *p)
     (void)hipFree(p->d_spows);
     (void)hipFree(p->d_recs);
     (void)hipFree(p->d_sort_temp);
-    for (int s = 0; s < Plan::kSlots; s++)
+    for (int s = 0; s < Plan::kSlots; s++) {
         if (p->h_recs[s]) (void)hipHostFree(p->h_recs[s]);
+        if (p->gexec[s]) (void)hipGraphExecDestroy(p->gexec[s]);
+    }
+    (void)hipFree(p->d_r0);
+    if (p->h_r0) (void)hipHostFree(p->h_r0);
     (void)hipFree(p->d_sums);
     if (p->st) (void)hipStreamDestroy(p->st);
     if (p->st2) (void)hipStreamDestroy(p->st2);
@@ -189,7 +232,7 @@ struct Timers {
     std::vector<hipEvent_t> pool;
     size_t used = 0;
     struct Rec {
-        int kind;  // 0 segments, 1 correlation, 2 harmonic sum
+        int kind;  // 0 segments, 1 correlation, 2 harmonic sum, 3 graph-replayed batch
         int fftlen;
         hipEvent_t a, b;
     };
@@ -251,10 +294,12 @@ PzlCorrArgs corr_args(const Plan &p, const SizeClass &c, int nb)
 
 // Segments, correlations and the harmonic sum of rsteps [r0, r0 + nb) into
 // hit slot 'slot'.
-int launch_batch(Plan &p, long long r0, int nb, int slot, float *d_sums, Timers *tm, bool with_sum = true)
+int launch_batch(Plan &p, long long r0, int nb, int slot, float *d_sums, Timers *tm, bool with_sum = true,
+                 const long long *r0_dev = nullptr)
 {
     const int nh = (int)p.harms.size();
     PzlBatchArgs ba{};
+    ba.rstep0_dev = r0_dev;
     ba.harms = p.d_harms;
     ba.nh = nh;
     ba.rstep0 = r0;
@@ -286,6 +331,7 @@ int launch_batch(Plan &p, long long r0, int nb, int slot, float *d_sums, Timers 
     sa.nb = nb;
     sa.nstages = p.g.numharmstages;
     sa.rstep0 = r0;
+    sa.rstep0_dev = r0_dev;
     sa.numz = p.fnumz;
     sa.numw = p.fnumw;
     sa.corr_uselen = p.g.corr_uselen;
@@ -307,6 +353,39 @@ int launch_batch(Plan &p, long long r0, int nb, int slot, float *d_sums, Timers 
         Scope s(tm, 2, 0, p.st);
         PZL_HIPCHK(pzl_launch_harmsum(&sa, p.st));
     }
+    return 0;
+}
+
+// A full batch into 'slot' as one graph launch.  The graph of a slot
+// (segments, every correlation class, harmonic sum) is captured once, with the
+// batch's first rstep read from device memory, and replayed for every later
+// full batch of that slot; the first batch of each slot runs eagerly.
+int launch_batch_graph(Plan &p, long long r0, int slot, Timers *tm)
+{
+    if (!p.gexec[slot]) {
+        int rc = launch_batch(p, r0, p.B, slot, nullptr, tm);
+        if (rc) return rc;
+        hipGraph_t g = nullptr;
+        PZL_HIPCHK(hipStreamBeginCapture(p.st, hipStreamCaptureModeThreadLocal));
+        rc = launch_batch(p, 0, p.B, slot, nullptr, nullptr, true, p.d_r0 + slot);
+        const hipError_t e = hipStreamEndCapture(p.st, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return fail(PZL_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+        const hipError_t ei = hipGraphInstantiate(&p.gexec[slot], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) {
+            p.gexec[slot] = nullptr;
+            return fail(PZL_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(ei));
+        }
+        return 0;
+    }
+    p.h_r0[slot] = r0;  // the previous batch of this slot has consumed its value
+    PZL_HIPCHK(hipMemcpyAsync(p.d_r0 + slot, p.h_r0 + slot, sizeof(long long), hipMemcpyHostToDevice, p.st));
+    Scope s(tm, 3, 0, p.st);
+    PZL_HIPCHK(hipGraphLaunch(p.gexec[slot], p.st));
     return 0;
 }
 
@@ -509,7 +588,11 @@ int pzl_device_count(int *count)
 int pzl_geometry_from_config(const PzlConfig *cfg, PzlGeometry *g)
 {
     if (!cfg || !g) return fail(PZL_ERR_ARG, "null argument");
-    return compute_geometry(*cfg, *g, nullptr, nullptr);
+    std::vector<PzlHarm> harms;
+    int rc = compute_geometry(*cfg, *g, &harms, nullptr);
+    if (rc) return rc;
+    g->device_bytes = plan_device_bytes(*g, plan_sizes(*cfg, *g, harms), (int)harms.size(), -1);
+    return 0;
 }
 
 int pzl_plan_create(const PzlConfig *cfg, int device, void **plan_out)
@@ -538,9 +621,8 @@ int pzl_plan_create(const PzlConfig *cfg, int device, void **plan_out)
         p->max_fftlen = std::max(p->max_fftlen, H.fftlen);
     }
     for (int s = 0; s < p->g.numharmstages; s++) p->numindep_d[s] = (double)p->g.numindep[s];
-    p->B = cfg->batchsize > 0 ? cfg->batchsize : 8;
-    if (p->g.nrsteps > 0 && p->B > p->g.nrsteps) p->B = (int)p->g.nrsteps;
-    if (p->B < 1) p->B = 1;
+    const PlanSizes sizes = plan_sizes(*cfg, p->g, p->harms);
+    p->B = sizes.B;
     const int nh = (int)p->harms.size();
     const PzlHarm &F = p->harms[0];
     p->fnumz = F.numz;
@@ -610,12 +692,7 @@ int pzl_plan_create(const PzlConfig *cfg, int device, void **plan_out)
         c.variant = tuned_variant(*p, c);
         p->classes.push_back(c);
     }
-    // hit slots: one rstep can never overflow a slot
-    const long long per_rstep = (long long)p->g.corr_uselen * F.numz * F.numw * p->g.numharmstages;
-    long long cap = cfg->maxhits > 0 ? cfg->maxhits : kDefaultHitCap;
-    cap = std::min<long long>(cap, per_rstep * p->B);
-    cap = std::max<long long>(cap, per_rstep);
-    p->hitcap = cap;
+    p->hitcap = sizes.hitcap;
     const long long maxkey = std::max<long long>(p->g.nrsteps, 1) * p->g.numharmstages * F.numw * F.numz *
                              (long long)p->g.corr_uselen;
     p->key_bits = 1;
@@ -634,10 +711,7 @@ int pzl_plan_create(const PzlConfig *cfg, int device, void **plan_out)
     PCHK(hipSetDevice(device));
     PCHK(hipStreamCreateWithFlags(&p->st, hipStreamNonBlocking));
     PCHK(hipStreamCreateWithFlags(&p->st2, hipStreamNonBlocking));
-    {
-        const long long nbatches = (p->g.nrsteps + p->B - 1) / p->B;
-        p->nslots = (int)std::max<long long>(1, std::min<long long>(Plan::kSlots, nbatches));
-    }
+    p->nslots = sizes.nslots;
     for (int s = 0; s < p->nslots; s++) {
         PCHK(hipEventCreateWithFlags(&p->ev_count[s], hipEventDisableTiming));
         PCHK(hipEventCreateWithFlags(&p->ev_free[s], hipEventDisableTiming));
@@ -672,6 +746,8 @@ int pzl_plan_create(const PzlConfig *cfg, int device, void **plan_out)
         PCHK(hipMalloc(&p->d_hpows[s], sizeof(float) * p->hitcap));
     }
     PCHK(hipMalloc(&p->d_counts, 2 * Plan::kSlots * sizeof(unsigned long long)));
+    PCHK(hipMalloc(&p->d_r0, Plan::kSlots * sizeof(long long)));
+    PCHK(hipHostMalloc((void **)&p->h_r0, Plan::kSlots * sizeof(long long), hipHostMallocDefault));
     PCHK(hipMemset(p->d_counts, 0, 2 * Plan::kSlots * sizeof(unsigned long long)));
     PCHK(hipHostMalloc((void **)&p->h_counts, 2 * Plan::kSlots * sizeof(unsigned long long), hipHostMallocDefault));
     PCHK(hipMalloc(&p->d_skeys, sizeof(long long) * p->hitcap));
@@ -688,11 +764,7 @@ int pzl_plan_create(const PzlConfig *cfg, int device, void **plan_out)
         if (brc) return cleanup(brc);
         p->bank_built = built;
     }
-    p->g.device_bytes = p->g.bank_bytes + (long long)sizeof(float) * p->plane_stride * p->B +
-                        (long long)sizeof(float2) * p->seg_stride * p->B +
-                        (long long)(p->nslots * 12 + 12 + (long long)sizeof(PzlHostHit)) * p->hitcap +
-                        (long long)p->sort_temp_bytes +
-                        (long long)sizeof(uint16_t) * p->B * nh * p->g.corr_uselen;
+    p->g.device_bytes = plan_device_bytes(p->g, sizes, nh, (long long)p->sort_temp_bytes);
 #undef PCHK
     *plan_out = p;
     return 0;
@@ -790,7 +862,8 @@ int pzl_search(void *plan, long long rstep_lo, long long rstep_hi, PzlCand *cand
             if (rc) return rc;
         }
         if (i >= p->nslots) PZL_HIPCHK(hipStreamWaitEvent(p->st, p->ev_free[slot], 0));
-        int rc = launch_batch(*p, r0, nb, slot, nullptr, &tm);
+        int rc = (p->use_graph && nb == p->B) ? launch_batch_graph(*p, r0, slot, &tm)
+                                              : launch_batch(*p, r0, nb, slot, nullptr, &tm);
         if (rc) return rc;
         PZL_HIPCHK(hipMemcpyAsync(p->h_counts + 2 * slot, p->d_counts + 2 * slot, 2 * sizeof(unsigned long long),
                                   hipMemcpyDeviceToHost, p->st));
@@ -835,8 +908,11 @@ int pzl_search(void *plan, long long rstep_lo, long long rstep_hi, PzlCand *cand
                 per_class[r.fftlen] += ms;
                 per_class_n[r.fftlen]++;
                 stats->corr_launches++;
-            } else {
+            } else if (r.kind == 2) {
                 stats->ms_sum += ms;
+            } else {
+                stats->ms_graph += ms;
+                stats->graph_launches++;
             }
         }
         (void)hipEventElapsedTime(&ms, ev_start, ev_end);
@@ -933,6 +1009,14 @@ int pzl_autotune(void *plan, int reps, char *report, int report_len)
     if (report && report_len > 0) {
         snprintf(report, (size_t)report_len, "%s", rep.c_str());
     }
+    return 0;
+}
+
+int pzl_plan_set_graph(void *plan, int on)
+{
+    Plan *p = (Plan *)plan;
+    if (!p) return fail(PZL_ERR_ARG, "null plan");
+    p->use_graph = on != 0;
     return 0;
 }
 

@@ -29,7 +29,8 @@ struct PzlHostHit {
 struct PzlBatchArgs {
     const PzlHarm *harms;
     int nh;                // harmonics (fundamental first, then stage order)
-    long long rstep0;      // global index of the batch's first rstep
+    long long rstep0;      // global index of the batch's first rstep ...
+    const long long *rstep0_dev;  // ... or, when set, read from device memory (graph replay)
     int nb;                // rsteps in this batch
     const double *startr;  // fundamental startr of every rstep of the search
     int rstep_bins;        // corr_uselen / 2
@@ -61,6 +62,7 @@ struct PzlSumArgs {
     const PzlHarm *harms;
     int nh, nb, nstages;
     long long rstep0;
+    const long long *rstep0_dev;  // when set, the batch's first rstep is read from here
     int numz, numw, corr_uselen;
     const float *planes;
     long long plane_stride;

@@ -183,7 +183,7 @@ __global__ __launch_bounds__(SEG_NT) void k_segments(PzlBatchArgs a)
     const int tid = threadIdx.x;
     const int b = blockIdx.x / a.nh, h = blockIdx.x % a.nh;
     const PzlHarm H = a.harms[h];
-    const long long rstep = a.rstep0 + b;
+    const long long rstep = (a.rstep0_dev ? *a.rstep0_dev : a.rstep0) + b;
     const double fullrlo = a.startr[rstep];
     const double fullrhi = fullrlo + a.rstep_bins - PZL_DR;
     const PzlHarmR hr = pzl_harm_r(H.hf, h == 0, fullrlo, fullrhi, a.corr_uselen);
@@ -590,7 +590,7 @@ __global__ __launch_bounds__(SUM_NT) void k_harmsum(PzlSumArgs a)
     const int r = chunk * SUM_NT + tid;
     const bool active = r < a.corr_uselen;
     const int rr = active ? r : a.corr_uselen - 1;
-    const long long rstep = a.rstep0 + b;
+    const long long rstep = (a.rstep0_dev ? *a.rstep0_dev : a.rstep0) + b;
     const PzlHarm F = a.harms[0];
     // every plane of this rstep, as one buffer (byte offsets < 2^31)
     const __amdgpu_buffer_rsrc_t planes =

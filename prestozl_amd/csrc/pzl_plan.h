@@ -87,6 +87,11 @@ struct Plan {
     hipEvent_t ev_count[kSlots] = {};
     hipEvent_t ev_free[kSlots] = {};
     float *d_sums = nullptr;
+    // graph replay of full batches (one executable graph per hit slot)
+    bool use_graph = false;
+    hipGraphExec_t gexec[kSlots] = {};
+    long long *d_r0 = nullptr;  // [kSlots] first rstep of the batch in each slot
+    long long *h_r0 = nullptr;  // pinned staging of those
     // results of the last search
     std::vector<PzlHostHit> last_hits;
     std::vector<PzlCand> last_cands;

@@ -248,3 +248,24 @@ def test_corr_variants(config1_data, shape):
                 tested += 1
     assert tested >= 2 * len(by_len)
     plan.close()
+
+
+@pytest.mark.parametrize("which", ["c1", "c2"])
+def test_graph_replay_identical(config1_data, which):
+    """Batches replayed from captured graphs give the eager path's hits and
+    candidates exactly."""
+    fft, kw = _c1(config1_data) if which == "c1" else _c2_small()
+    plan = _plan(kw, fft)
+    lo, hi = (0, -1) if which == "c1" else (40, 52)
+    c_eager, _ = plan.search(lo, hi, stats=False)
+    h_eager = plan.hits()
+    plan.set_graph(True)
+    c_graph, st = plan.search(lo, hi, stats=True)
+    h_graph = plan.hits()
+    c_again, _ = plan.search(lo, hi, stats=False)  # replay of the already captured graphs
+    plan.close()
+    print("graph launches %d, eager candidates %d" % (st["graph_launches"], len(c_eager)))
+    assert st["graph_launches"] > 0
+    assert c_graph == c_eager and c_again == c_eager
+    assert np.array_equal(h_graph[1], h_eager[1])
+    assert all(np.array_equal(a, b) for a, b in zip(h_graph[0], h_eager[0]))
