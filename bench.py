@@ -106,6 +106,36 @@ def serve_bench(n, rate, max_inflight, seed=4):
             "plans_reused": int(sum(r[1]["plan_reused"] for r in res))}
 
 
+def dm_trials_bench(n, max_inflight):
+    """The DM-trial batch after dedispersion: n files of 262,144 samples at
+    dt = 6.4e-4 s with a common jerked sinusoid (37.2 Hz, z = 40, w = -80)
+    whose amplitude peaks at trial 282; zmax 200, wmax 300, numharm 8, sigma 5,
+    all served through the request loop (one kernel bank for every trial)."""
+    from prestozl_amd import binding as B
+    from prestozl_amd import synth
+    files = [synth.dm_trial(i)[0] for i in range(n)]
+    srv = B.Server(budget=0, max_inflight=max_inflight)
+    t0 = time.perf_counter()
+    ids = []
+    for fft in files:
+        cfg = B.make_config(N=262144, dt=6.4e-4, numbins=len(fft), zmax=200, wmax=300, numharm=8, sigma=5.0,
+                            wmaxP=True, batchsize=8)
+        ids.append(srv.submit(cfg, fft))
+    res = [srv.wait(i) for i in ids]
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    lat = [r[1]["ms_total"] for r in res]
+    best = max(range(n), key=lambda i: max([c[1] for c in res[i][0]] or [0.0]))
+    return {"workload": "DM-trial batch: %d files of N=2^18, dt=6.4e-4 s, zmax 200, wmax 300, numharm 8, sigma 5, "
+                        "served" % n,
+            "throughput": sum(r[1]["cells"] for r in res) / wall, "unit": "cells/s", "wall_s": wall,
+            "files_per_s": n / wall, "latency_ms": {"p50": float(np.percentile(lat, 50)),
+                                                     "p99": float(np.percentile(lat, 99))},
+            "peak_inflight": st["peak_inflight"], "plans_created": int(n - sum(r[1]["plan_reused"] for r in res)),
+            "strongest_trial": best}
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of 'kernel' at this workload from the committed
     rocprofv3 --pmc measurement (FETCH_SIZE + WRITE_SIZE, copy-calibrated)."""
@@ -131,6 +161,7 @@ def main():
     ap.add_argument("--serve", type=int, default=24, help="requests in the served-stream leg (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=1.5, help="Poisson arrival rate of the served stream (1/s)")
     ap.add_argument("--serve-inflight", type=int, default=3)
+    ap.add_argument("--dm-trials", type=int, default=564, help="files in the DM-trial leg (0: skip)")
     ap.add_argument("--no-tune", action="store_true", help="use the default correlation variants")
     ap.add_argument("--rsteps", type=int, default=0,
                     help="profiling runs only: search this many rsteps from the middle of the band")
@@ -244,6 +275,8 @@ def main():
     plan.close()
     if args.serve and not args.rsteps:
         out["serving"] = serve_bench(args.serve, args.serve_rate, args.serve_inflight)
+    if args.dm_trials and not args.rsteps:
+        out["dm_trials"] = dm_trials_bench(args.dm_trials, args.serve_inflight)
     if not args.no_cpu:
         nthreads = args.cpu_threads or min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(fft, kw, args.cpu_rsteps, nthreads,

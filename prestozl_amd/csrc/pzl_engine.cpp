@@ -162,9 +162,10 @@ PlanSizes plan_sizes(const PzlConfig &cfg, const PzlGeometry &g, const std::vect
 long long plan_device_bytes(const PzlGeometry &g, const PlanSizes &z, int nh, long long sort_temp)
 {
     if (sort_temp < 0) sort_temp = 12 * z.hitcap + (16LL << 20);
-    return g.bank_bytes + 8 * g.numbins + g.plane_bytes_per_rstep * z.B + (long long)sizeof(float2) * z.seg_stride * z.B +
-           (long long)(z.nslots * 12 + 12 + (long long)sizeof(PzlHostHit)) * z.hitcap + sort_temp +
-           (long long)sizeof(uint16_t) * z.B * nh * g.corr_uselen;
+    return g.bank_bytes + 8 * g.numbins +
+           Plan::kSets * (g.plane_bytes_per_rstep * z.B + (long long)sizeof(float2) * z.seg_stride * z.B +
+                          (long long)sizeof(uint16_t) * z.B * nh * g.corr_uselen) +
+           (long long)(z.nslots * 12 + 12 + (long long)sizeof(PzlHostHit)) * z.hitcap + sort_temp;
 }
 
 // Correlation variants chosen by pzl_autotune, per (device, FFT length,
@@ -191,6 +192,7 @@ void free_plan(Plan *p)
     if (!p) return;
     (void)hipSetDevice(p->device);
     if (p->st) (void)hipStreamSynchronize(p->st);
+    if (p->st3) (void)hipStreamSynchronize(p->st3);
     if (p->st2) (void)hipStreamSynchronize(p->st2);
     for (auto &c : p->classes) (void)hipFree(c.d_harms);
     (void)hipFree(p->d_harms);
@@ -199,10 +201,14 @@ void free_plan(Plan *p)
     (void)hipFree(p->d_zinds);
     (void)hipFree(p->d_winds);
     (void)hipFree(p->d_fft_own);
-    (void)hipFree(p->d_segs);
-    (void)hipFree(p->d_rinds);
-    (void)hipFree(p->d_hr);
-    (void)hipFree(p->d_planes);
+    for (int k = 0; k < Plan::kSets; k++) {
+        (void)hipFree(p->d_segs[k]);
+        (void)hipFree(p->d_rinds[k]);
+        (void)hipFree(p->d_hr[k]);
+        (void)hipFree(p->d_planes[k]);
+        if (p->ev_corr[k]) (void)hipEventDestroy(p->ev_corr[k]);
+        if (p->ev_sum[k]) (void)hipEventDestroy(p->ev_sum[k]);
+    }
     for (int s = 0; s < Plan::kSlots; s++) {
         (void)hipFree(p->d_hkeys[s]);
         (void)hipFree(p->d_hpows[s]);
@@ -215,15 +221,16 @@ void free_plan(Plan *p)
     (void)hipFree(p->d_spows);
     (void)hipFree(p->d_recs);
     (void)hipFree(p->d_sort_temp);
-    for (int s = 0; s < Plan::kSlots; s++) {
+    for (int s = 0; s < Plan::kSlots; s++)
         if (p->h_recs[s]) (void)hipHostFree(p->h_recs[s]);
-        if (p->gexec[s]) (void)hipGraphExecDestroy(p->gexec[s]);
-    }
+    for (int k = 0; k < Plan::kSets; k++)
+        if (p->gexec[k]) (void)hipGraphExecDestroy(p->gexec[k]);
     (void)hipFree(p->d_r0);
     if (p->h_r0) (void)hipHostFree(p->h_r0);
     (void)hipFree(p->d_sums);
     if (p->st) (void)hipStreamDestroy(p->st);
     if (p->st2) (void)hipStreamDestroy(p->st2);
+    if (p->st3) (void)hipStreamDestroy(p->st3);
     delete p;
 }
 
@@ -275,7 +282,7 @@ struct Scope {
     }
 };
 
-PzlCorrArgs corr_args(const Plan &p, const SizeClass &c, int nb)
+PzlCorrArgs corr_args(const Plan &p, const SizeClass &c, int nb, int set)
 {
     PzlCorrArgs ca{};
     ca.harms = p.d_harms;
@@ -283,25 +290,23 @@ PzlCorrArgs corr_args(const Plan &p, const SizeClass &c, int nb)
     ca.nb = nb;
     ca.class_harms = c.d_harms;
     ca.nclass = (int)c.harms.size();
-    ca.segs = p.d_segs;
+    ca.segs = p.d_segs[set];
     ca.seg_stride = p.seg_stride;
     ca.bank = p.d_bank;
-    ca.planes = p.d_planes;
+    ca.planes = p.d_planes[set];
     ca.plane_stride = p.plane_stride;
-    ca.hr_tab = p.d_hr;
+    ca.hr_tab = p.d_hr[set];
     return ca;
 }
 
-// Segments, correlations and the harmonic sum of rsteps [r0, r0 + nb) into
-// hit slot 'slot'.
-int launch_batch(Plan &p, long long r0, int nb, int slot, float *d_sums, Timers *tm, bool with_sum = true,
-                 const long long *r0_dev = nullptr)
+// Segments and correlations of rsteps [r0, r0 + nb) into buffer set 'set',
+// on the correlation stream (r0_dev: read the first rstep from there).
+int launch_corr_part(Plan &p, long long r0, int nb, int set, Timers *tm, const long long *r0_dev)
 {
-    const int nh = (int)p.harms.size();
     PzlBatchArgs ba{};
     ba.rstep0_dev = r0_dev;
     ba.harms = p.d_harms;
-    ba.nh = nh;
+    ba.nh = (int)p.harms.size();
     ba.rstep0 = r0;
     ba.nb = nb;
     ba.startr = p.d_startr;
@@ -311,33 +316,39 @@ int launch_batch(Plan &p, long long r0, int nb, int slot, float *d_sums, Timers 
     ba.numbins = p.fft_bins;
     ba.file_lobin = p.g.lobin;
     ba.nph = p.cfg.nph;
-    ba.segs = p.d_segs;
+    ba.segs = p.d_segs[set];
     ba.seg_stride = p.seg_stride;
-    ba.rinds = p.d_rinds;
-    ba.hr_tab = p.d_hr;
+    ba.rinds = p.d_rinds[set];
+    ba.hr_tab = p.d_hr[set];
     {
         Scope s(tm, 0, 0, p.st);
         PZL_HIPCHK(pzl_launch_segments(&ba, p.max_fftlen, p.st));
     }
     for (auto &c : p.classes) {
-        const PzlCorrArgs ca = corr_args(p, c, nb);
+        const PzlCorrArgs ca = corr_args(p, c, nb, set);
         Scope s(tm, 1, c.fftlen, p.st);
         PZL_HIPCHK(pzl_launch_corr(c.fftlen, c.variant, &ca, c.nrows.data(), p.st));
     }
-    if (!with_sum) return 0;
+    return 0;
+}
+
+// Harmonic sum of the batch held in buffer set 'set' into hit slot 'slot',
+// on the summing stream.
+int launch_sum_part(Plan &p, long long r0, int nb, int set, int slot, float *d_sums, Timers *tm)
+{
     PzlSumArgs sa{};
     sa.harms = p.d_harms;
-    sa.nh = nh;
+    sa.nh = (int)p.harms.size();
     sa.nb = nb;
     sa.nstages = p.g.numharmstages;
     sa.rstep0 = r0;
-    sa.rstep0_dev = r0_dev;
+    sa.rstep0_dev = nullptr;
     sa.numz = p.fnumz;
     sa.numw = p.fnumw;
     sa.corr_uselen = p.g.corr_uselen;
-    sa.planes = p.d_planes;
+    sa.planes = p.d_planes[set];
     sa.plane_stride = p.plane_stride;
-    sa.rinds = p.d_rinds;
+    sa.rinds = p.d_rinds[set];
     sa.zinds = p.d_zinds;
     sa.winds = p.d_winds;
     for (int s = 0; s < 6; s++) sa.powcut[s] = p.g.powcut[s];
@@ -347,45 +358,79 @@ int launch_batch(Plan &p, long long r0, int nb, int slot, float *d_sums, Timers 
     sa.maxhits = (unsigned long long)p.hitcap;
     sa.sums_out = d_sums;
     sa.startr = p.d_startr;
-    sa.hr_tab = p.d_hr;
+    sa.hr_tab = p.d_hr[set];
     sa.int_rlo = p.int_rlo;
-    {
-        Scope s(tm, 2, 0, p.st);
-        PZL_HIPCHK(pzl_launch_harmsum(&sa, p.st));
-    }
+    Scope s(tm, 2, 0, p.st3);
+    PZL_HIPCHK(pzl_launch_harmsum(&sa, p.st3));
     return 0;
 }
 
-// A full batch into 'slot' as one graph launch.  The graph of a slot
-// (segments, every correlation class, harmonic sum) is captured once, with the
+// The correlation part of a full batch as one graph launch: the graph of a
+// buffer set (segments and every correlation class) is captured once, with the
 // batch's first rstep read from device memory, and replayed for every later
-// full batch of that slot; the first batch of each slot runs eagerly.
-int launch_batch_graph(Plan &p, long long r0, int slot, Timers *tm)
+// full batch in that set; the first batch of each set runs eagerly.
+int launch_corr_graph(Plan &p, long long r0, int set, Timers *tm)
 {
-    if (!p.gexec[slot]) {
-        int rc = launch_batch(p, r0, p.B, slot, nullptr, tm);
+    if (!p.gexec[set]) {
+        int rc = launch_corr_part(p, r0, p.B, set, tm, nullptr);
         if (rc) return rc;
         hipGraph_t g = nullptr;
         PZL_HIPCHK(hipStreamBeginCapture(p.st, hipStreamCaptureModeThreadLocal));
-        rc = launch_batch(p, 0, p.B, slot, nullptr, nullptr, true, p.d_r0 + slot);
+        rc = launch_corr_part(p, 0, p.B, set, nullptr, p.d_r0 + set);
         const hipError_t e = hipStreamEndCapture(p.st, &g);
         if (rc) {
             if (g) (void)hipGraphDestroy(g);
             return rc;
         }
         if (e != hipSuccess) return fail(PZL_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
-        const hipError_t ei = hipGraphInstantiate(&p.gexec[slot], g, nullptr, nullptr, 0);
+        const hipError_t ei = hipGraphInstantiate(&p.gexec[set], g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
         if (ei != hipSuccess) {
-            p.gexec[slot] = nullptr;
+            p.gexec[set] = nullptr;
             return fail(PZL_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(ei));
         }
         return 0;
     }
-    p.h_r0[slot] = r0;  // the previous batch of this slot has consumed its value
-    PZL_HIPCHK(hipMemcpyAsync(p.d_r0 + slot, p.h_r0 + slot, sizeof(long long), hipMemcpyHostToDevice, p.st));
+    // staged through a ring longer than the batches the host can run ahead
+    // (it finalizes batch i - nslots before launching batch i)
+    long long *h = p.h_r0 + (p.r0_ring++ % Plan::kR0Ring);
+    *h = r0;
+    PZL_HIPCHK(hipMemcpyAsync(p.d_r0 + set, h, sizeof(long long), hipMemcpyHostToDevice, p.st));
     Scope s(tm, 3, 0, p.st);
-    PZL_HIPCHK(hipGraphLaunch(p.gexec[slot], p.st));
+    PZL_HIPCHK(hipGraphLaunch(p.gexec[set], p.st));
+    return 0;
+}
+
+// Rsteps [r0, r0 + nb) in buffer set 'set', hits into slot 'slot'.  The
+// correlation (stream st) waits until the previous harmonic sum of the set
+// has read its planes; the harmonic sum (stream st3) waits for this set's
+// correlation, so the correlation of one batch overlaps the sum of the
+// previous one.
+int launch_batch(Plan &p, long long r0, int nb, int set, int slot, float *d_sums, Timers *tm, bool with_sum = true,
+                 bool graph = false)
+{
+    PZL_HIPCHK(hipStreamWaitEvent(p.st, p.ev_sum[set], 0));
+    const int rc = (graph && nb == p.B) ? launch_corr_graph(p, r0, set, tm)
+                                        : launch_corr_part(p, r0, nb, set, tm, nullptr);
+    if (rc) return rc;
+    PZL_HIPCHK(hipEventRecord(p.ev_corr[set], p.st));
+    if (!with_sum) {
+        PZL_HIPCHK(hipEventRecord(p.ev_sum[set], p.st));
+        return 0;
+    }
+    PZL_HIPCHK(hipStreamWaitEvent(p.st3, p.ev_corr[set], 0));
+    const int rs = launch_sum_part(p, r0, nb, set, slot, d_sums, tm);
+    if (rs) return rs;
+    PZL_HIPCHK(hipEventRecord(p.ev_sum[set], p.st3));
+    return 0;
+}
+
+// Wait for every stream of the plan.
+int sync_all(Plan &p)
+{
+    PZL_HIPCHK(hipStreamSynchronize(p.st));
+    PZL_HIPCHK(hipStreamSynchronize(p.st3));
+    PZL_HIPCHK(hipStreamSynchronize(p.st2));
     return 0;
 }
 
@@ -523,15 +568,19 @@ int finalize(Plan &p, int slot, long long r0, int nb, SearchState &ss, Timers *t
         if (rc) return rc;
         return reset_slot(p, slot, p.st2);
     }
-    // overflow: rerun behind whatever the search stream holds
+    // overflow: let every batch in flight finish, then search this batch
+    // again one rstep at a time
+    int rc = sync_all(p);
+    if (rc) return rc;
     for (int b = 0; b < nb; b++) {
-        int rc = reset_slot(p, slot, p.st);
+        rc = reset_slot(p, slot, p.st3);
         if (rc) return rc;
-        rc = launch_batch(p, r0 + b, 1, slot, nullptr, tm);
+        rc = launch_batch(p, r0 + b, 1, 0, slot, nullptr, tm);
         if (rc) return rc;
         PZL_HIPCHK(hipMemcpyAsync(p.h_counts + 2 * slot, p.d_counts + 2 * slot, 2 * sizeof(unsigned long long),
-                                  hipMemcpyDeviceToHost, p.st));
-        PZL_HIPCHK(hipStreamSynchronize(p.st));
+                                  hipMemcpyDeviceToHost, p.st3));
+        rc = sync_all(p);
+        if (rc) return rc;
         const unsigned long long n1 = p.h_counts[2 * slot];
         if ((long long)n1 > p.hitcap)
             return fail(PZL_ERR_OVERFLOW, "hit buffer overflow: %llu hits in one rstep", n1);
@@ -711,6 +760,7 @@ int pzl_plan_create(const PzlConfig *cfg, int device, void **plan_out)
     PCHK(hipSetDevice(device));
     PCHK(hipStreamCreateWithFlags(&p->st, hipStreamNonBlocking));
     PCHK(hipStreamCreateWithFlags(&p->st2, hipStreamNonBlocking));
+    PCHK(hipStreamCreateWithFlags(&p->st3, hipStreamNonBlocking));
     p->nslots = sizes.nslots;
     for (int s = 0; s < p->nslots; s++) {
         PCHK(hipEventCreateWithFlags(&p->ev_count[s], hipEventDisableTiming));
@@ -737,17 +787,24 @@ int pzl_plan_create(const PzlConfig *cfg, int device, void **plan_out)
     if (p->g.plane_bytes_per_rstep >= (1LL << 31))
         return cleanup(fail(PZL_ERR_UNSUPPORTED, "the planes of one rstep need %lld bytes (at most 2 GiB)",
                             p->g.plane_bytes_per_rstep));
-    PCHK(hipMalloc(&p->d_segs, sizeof(float2) * p->seg_stride * p->B));
-    PCHK(hipMalloc(&p->d_rinds, sizeof(uint16_t) * (size_t)p->B * nh * p->g.corr_uselen));
-    PCHK(hipMalloc(&p->d_hr, sizeof(PzlHrTab) * (size_t)p->B * nh));
-    PCHK(hipMalloc(&p->d_planes, sizeof(float) * (size_t)p->plane_stride * p->B));
+    for (int k = 0; k < Plan::kSets; k++) {
+        PCHK(hipMalloc(&p->d_segs[k], sizeof(float2) * p->seg_stride * p->B));
+        PCHK(hipMalloc(&p->d_rinds[k], sizeof(uint16_t) * (size_t)p->B * nh * p->g.corr_uselen));
+        PCHK(hipMalloc(&p->d_hr[k], sizeof(PzlHrTab) * (size_t)p->B * nh));
+        PCHK(hipMalloc(&p->d_planes[k], sizeof(float) * (size_t)p->plane_stride * p->B));
+        PCHK(hipEventCreateWithFlags(&p->ev_corr[k], hipEventDisableTiming));
+        PCHK(hipEventCreateWithFlags(&p->ev_sum[k], hipEventDisableTiming));
+        // recorded once so that the first waits on them are satisfied
+        PCHK(hipEventRecord(p->ev_corr[k], p->st));
+        PCHK(hipEventRecord(p->ev_sum[k], p->st));
+    }
     for (int s = 0; s < p->nslots; s++) {
         PCHK(hipMalloc(&p->d_hkeys[s], sizeof(long long) * p->hitcap));
         PCHK(hipMalloc(&p->d_hpows[s], sizeof(float) * p->hitcap));
     }
     PCHK(hipMalloc(&p->d_counts, 2 * Plan::kSlots * sizeof(unsigned long long)));
-    PCHK(hipMalloc(&p->d_r0, Plan::kSlots * sizeof(long long)));
-    PCHK(hipHostMalloc((void **)&p->h_r0, Plan::kSlots * sizeof(long long), hipHostMallocDefault));
+    PCHK(hipMalloc(&p->d_r0, Plan::kSets * sizeof(long long)));
+    PCHK(hipHostMalloc((void **)&p->h_r0, Plan::kR0Ring * sizeof(long long), hipHostMallocDefault));
     PCHK(hipMemset(p->d_counts, 0, 2 * Plan::kSlots * sizeof(unsigned long long)));
     PCHK(hipHostMalloc((void **)&p->h_counts, 2 * Plan::kSlots * sizeof(unsigned long long), hipHostMallocDefault));
     PCHK(hipMalloc(&p->d_skeys, sizeof(long long) * p->hitcap));
@@ -861,13 +918,13 @@ int pzl_search(void *plan, long long rstep_lo, long long rstep_hi, PzlCand *cand
             int rc = finalize(*p, q.slot, q.r0, q.nb, ss, &tm);
             if (rc) return rc;
         }
-        if (i >= p->nslots) PZL_HIPCHK(hipStreamWaitEvent(p->st, p->ev_free[slot], 0));
-        int rc = (p->use_graph && nb == p->B) ? launch_batch_graph(*p, r0, slot, &tm)
-                                              : launch_batch(*p, r0, nb, slot, nullptr, &tm);
+        // the harmonic sum writes into the slot once the host has drained it
+        if (i >= p->nslots) PZL_HIPCHK(hipStreamWaitEvent(p->st3, p->ev_free[slot], 0));
+        int rc = launch_batch(*p, r0, nb, i % Plan::kSets, slot, nullptr, &tm, true, p->use_graph);
         if (rc) return rc;
         PZL_HIPCHK(hipMemcpyAsync(p->h_counts + 2 * slot, p->d_counts + 2 * slot, 2 * sizeof(unsigned long long),
-                                  hipMemcpyDeviceToHost, p->st));
-        PZL_HIPCHK(hipEventRecord(p->ev_count[slot], p->st));
+                                  hipMemcpyDeviceToHost, p->st3));
+        PZL_HIPCHK(hipEventRecord(p->ev_count[slot], p->st3));
         pend.push_back({slot, nb, r0});
         // apply whatever the GPU has already finished
         while (head + 1 < pend.size() && hipEventQuery(p->ev_count[pend[head].slot]) == hipSuccess) {
@@ -878,15 +935,17 @@ int pzl_search(void *plan, long long rstep_lo, long long rstep_hi, PzlCand *cand
     }
     if (tm.on) {
         ev_end = tm.get();
-        PZL_HIPCHK(hipEventRecord(ev_end, p->st));
+        PZL_HIPCHK(hipEventRecord(ev_end, p->st3));  // the last harmonic sum follows the last correlation
     }
     while (head < pend.size()) {
         const Pending q = pend[head++];
         int rc = finalize(*p, q.slot, q.r0, q.nb, ss, &tm);
         if (rc) return rc;
     }
-    PZL_HIPCHK(hipStreamSynchronize(p->st));
-    PZL_HIPCHK(hipStreamSynchronize(p->st2));
+    {
+        int rc = sync_all(*p);
+        if (rc) return rc;
+    }
     ss.finish();
     p->last_cands = ss.cl.items();
     const auto &items = p->last_cands;
@@ -962,16 +1021,19 @@ int pzl_autotune(void *plan, int reps, char *report, int report_len)
     const int nb = (int)std::min<long long>(p->B, nr - r0);
     if (nb < 1) return 0;
     // the segments of one batch from the middle of the band
-    int rc = launch_batch(*p, r0, nb, 0, nullptr, nullptr, false);
+    int rc = launch_batch(*p, r0, nb, 0, 0, nullptr, nullptr, false);
     if (rc) return rc;
-    PZL_HIPCHK(hipStreamSynchronize(p->st));
+    {
+        int src = sync_all(*p);
+        if (src) return src;
+    }
     hipEvent_t e0, e1;
     PZL_HIPCHK(hipEventCreate(&e0));
     PZL_HIPCHK(hipEventCreate(&e1));
     std::string rep;
     char line[256];
     for (auto &c : p->classes) {
-        const PzlCorrArgs ca = corr_args(*p, c, nb);
+        const PzlCorrArgs ca = corr_args(*p, c, nb, 0);
         const int nv = pzl_corr_num_variants(c.fftlen);
         int best = c.variant;
         float best_ms = 1e30f;
@@ -1091,18 +1153,21 @@ int pzl_compute_plane(void *plan, long long rstep, int idx, float *out, long lon
     if (!p->d_fft) return fail(PZL_ERR_NOSPECTRUM, "no spectrum set");
     PZL_HIPCHK(hipSetDevice(p->device));
     PZL_HIPCHK(hipMemsetAsync(p->d_counts, 0, 4 * sizeof(unsigned long long), p->st));
-    int rc = launch_batch(*p, rstep, 1, 0, nullptr, nullptr);
+    int rc = launch_batch(*p, rstep, 1, 0, 0, nullptr, nullptr);
     if (rc) return rc;
-    PZL_HIPCHK(hipStreamSynchronize(p->st));
+    {
+        int src = sync_all(*p);
+        if (src) return src;
+    }
     PzlHrTab hr;
-    PZL_HIPCHK(hipMemcpy(&hr, p->d_hr + idx, sizeof hr, hipMemcpyDeviceToHost));
+    PZL_HIPCHK(hipMemcpy(&hr, p->d_hr[0] + idx, sizeof hr, hipMemcpyDeviceToHost));
     const PzlHarm &H = p->harms[idx];
     if (numrs) *numrs = hr.numrs;
     if (rlo) *rlo = hr.rlo;
     if (out) {
         const long long need = (long long)H.nrows * hr.numrs;
         if (need > maxfloats) return fail(PZL_ERR_ARG, "output too small (%lld floats needed)", need);
-        PZL_HIPCHK(hipMemcpy2D(out, sizeof(float) * hr.numrs, p->d_planes + H.plane_off,
+        PZL_HIPCHK(hipMemcpy2D(out, sizeof(float) * hr.numrs, p->d_planes[0] + H.plane_off,
                                sizeof(float) * H.rs_stride, sizeof(float) * hr.numrs, H.nrows,
                                hipMemcpyDeviceToHost));
     }
@@ -1119,18 +1184,21 @@ int pzl_compute_plane_at(void *plan, double fullrlo, int idx, float *out, long l
     PZL_HIPCHK(hipSetDevice(p->device));
     const long long spare = p->g.nrsteps;
     PZL_HIPCHK(hipMemcpyAsync(p->d_startr + spare, &fullrlo, sizeof(double), hipMemcpyHostToDevice, p->st));
-    int rc = launch_batch(*p, spare, 1, 0, nullptr, nullptr, false);
+    int rc = launch_batch(*p, spare, 1, 0, 0, nullptr, nullptr, false);
     if (rc) return rc;
-    PZL_HIPCHK(hipStreamSynchronize(p->st));
+    {
+        int src = sync_all(*p);
+        if (src) return src;
+    }
     PzlHrTab hr;
-    PZL_HIPCHK(hipMemcpy(&hr, p->d_hr + idx, sizeof hr, hipMemcpyDeviceToHost));
+    PZL_HIPCHK(hipMemcpy(&hr, p->d_hr[0] + idx, sizeof hr, hipMemcpyDeviceToHost));
     const PzlHarm &H = p->harms[idx];
     if (numrs) *numrs = hr.numrs;
     if (rlo) *rlo = hr.rlo;
     if (out) {
         const long long need = (long long)H.nrows * hr.numrs;
         if (need > maxfloats) return fail(PZL_ERR_ARG, "output too small (%lld floats needed)", need);
-        PZL_HIPCHK(hipMemcpy2D(out, sizeof(float) * hr.numrs, p->d_planes + H.plane_off,
+        PZL_HIPCHK(hipMemcpy2D(out, sizeof(float) * hr.numrs, p->d_planes[0] + H.plane_off,
                                sizeof(float) * H.rs_stride, sizeof(float) * hr.numrs, H.nrows,
                                hipMemcpyDeviceToHost));
     }
@@ -1147,9 +1215,12 @@ int pzl_compute_sums(void *plan, long long rstep, float *out, long long maxfloat
     PZL_HIPCHK(hipSetDevice(p->device));
     if (!p->d_sums) PZL_HIPCHK(hipMalloc(&p->d_sums, sizeof(float) * need));
     PZL_HIPCHK(hipMemsetAsync(p->d_counts, 0, 4 * sizeof(unsigned long long), p->st));
-    int rc = launch_batch(*p, rstep, 1, 0, p->d_sums, nullptr);
+    int rc = launch_batch(*p, rstep, 1, 0, 0, p->d_sums, nullptr);
     if (rc) return rc;
-    PZL_HIPCHK(hipStreamSynchronize(p->st));
+    {
+        int src = sync_all(*p);
+        if (src) return src;
+    }
     PZL_HIPCHK(hipMemcpy(out, p->d_sums, sizeof(float) * need, hipMemcpyDeviceToHost));
     PZL_HIPCHK(hipMemset(p->d_counts, 0, 4 * sizeof(unsigned long long)));
     return 0;

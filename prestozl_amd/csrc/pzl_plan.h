@@ -48,8 +48,9 @@ struct Plan {
     int int_rlo = 1;  // every startr is an integer (integer r maps on the GPU)
     int B = 8;
     int device = 0;
-    hipStream_t st = nullptr;   // search stream
-    hipStream_t st2 = nullptr;  // hit ordering stream
+    hipStream_t st = nullptr;   // segments and correlations
+    hipStream_t st2 = nullptr;  // hit ordering
+    hipStream_t st3 = nullptr;  // harmonic sums
     // resident
     PzlHarm *d_harms = nullptr;
     float2 *d_bank = nullptr;  // shared, owned by the bank cache
@@ -61,13 +62,17 @@ struct Plan {
     const float2 *d_fft = nullptr;
     float2 *d_fft_own = nullptr;
     long long fft_bins = 0;
-    // per batch
-    float2 *d_segs = nullptr;
+    // per batch, two sets: the correlation of one batch fills one set while
+    // the harmonic sum of the previous batch reads the other
+    static constexpr int kSets = 2;
+    float2 *d_segs[kSets] = {};
     long long seg_stride = 0;
-    uint16_t *d_rinds = nullptr;
-    PzlHrTab *d_hr = nullptr;
-    float *d_planes = nullptr;
+    uint16_t *d_rinds[kSets] = {};
+    PzlHrTab *d_hr[kSets] = {};
+    float *d_planes[kSets] = {};
     long long plane_stride = 0;
+    hipEvent_t ev_corr[kSets] = {};  // the set's planes are written
+    hipEvent_t ev_sum[kSets] = {};   // the set's planes have been summed
     // hits: a ring of unordered slots (one per batch in flight), one ordered copy
     static constexpr int kSlots = 4;
     int nslots = 2;
@@ -89,9 +94,11 @@ struct Plan {
     float *d_sums = nullptr;
     // graph replay of full batches (one executable graph per hit slot)
     bool use_graph = false;
-    hipGraphExec_t gexec[kSlots] = {};
-    long long *d_r0 = nullptr;  // [kSlots] first rstep of the batch in each slot
-    long long *h_r0 = nullptr;  // pinned staging of those
+    hipGraphExec_t gexec[kSets] = {};  // correlation part, per buffer set
+    long long *d_r0 = nullptr;  // [kSets] first rstep of the batch in each set
+    static constexpr int kR0Ring = 2 * kSlots;
+    long long *h_r0 = nullptr;  // pinned staging ring of those values
+    unsigned long long r0_ring = 0;
     // results of the last search
     std::vector<PzlHostHit> last_hits;
     std::vector<PzlCand> last_cands;
