@@ -154,7 +154,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batchsize", type=int, default=8)
     ap.add_argument("--log2n", type=int, default=24)
-    ap.add_argument("--cpu-rsteps", type=int, default=24)
+    ap.add_argument("--cpu-rsteps", type=int, default=120)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the one-rstep-per-launch (plain GPU form) search")
@@ -255,7 +255,8 @@ def main():
         "candidates": len(cands),
         "hits": stats_all[0]["nhits"],
     }
-    # the same search with every full batch replayed from a captured hipGraph
+    # variants of the execution: graph replay of every full batch (on top of the
+    # default overlap), and everything in one stream without overlap
     plan.set_graph(True)
     plan.search(lo, hi, stats=False)  # captures
     ts = time.perf_counter()
@@ -264,9 +265,18 @@ def main():
     out["graph_replay"] = {"value": cells / tg, "unit": "cells/s", "ms_per_search": tg * 1e3,
                            "same_candidates": cg == cands}
     plan.set_graph(False)
+    plan.set_overlap(False)
+    ts = time.perf_counter()
+    cs, _ = plan.search(lo, hi, stats=False)
+    tsr = time.perf_counter() - ts
+    out["serial_one_stream"] = {"value": cells / tsr, "unit": "cells/s", "ms_per_search": tsr * 1e3,
+                                "same_candidates": cs == cands}
+    plan.set_overlap(True)
     if not args.no_plain and not args.rsteps:
         plan1 = B.Plan(B.make_config(batchsize=1, **kw))
         plan1.set_spectrum(fft)
+        for c in stats_all[0]["classes"]:  # the same kernel variants: identical arithmetic
+            plan1.set_variant(c["fftlen"], plan.variant(c["fftlen"]))
         ts = time.perf_counter()
         c1, _ = plan1.search(stats=False)
         out["plain_gpu"] = {"value": cells / (time.perf_counter() - ts), "unit": "cells/s", "batchsize": 1,

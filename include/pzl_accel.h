@@ -133,6 +133,10 @@ int pzl_autotune(void *plan, int reps, char *report, int report_len);
  * index read from device memory.  Off by default.  Per-kernel timing is then
  * reported per batch (ms_graph) instead of per kernel. */
 int pzl_plan_set_graph(void *plan, int on);
+/* Overlap: the harmonic sum of a batch runs on its own stream, concurrently
+ * with the correlation of the next batch (two plane sets).  On by default;
+ * off runs every kernel of the search in one stream. */
+int pzl_plan_set_overlap(void *plan, int on);
 /* The variant in use for an FFT length of the plan; set one explicitly. */
 int pzl_plan_variant(void *plan, int fftlen, int *variant);
 int pzl_plan_set_variant(void *plan, int fftlen, int variant);

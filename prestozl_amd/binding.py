@@ -126,6 +126,7 @@ def lib():
         L.pzl_server_destroy.restype = None
         L.pzl_plan_variant.argtypes = [C.c_void_p, C.c_int, P(C.c_int)]
         L.pzl_plan_set_graph.argtypes = [C.c_void_p, C.c_int]
+        L.pzl_plan_set_overlap.argtypes = [C.c_void_p, C.c_int]
         L.pzl_plan_set_variant.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.pzl_corr_num_variants.argtypes = [C.c_int]
         L.pzl_corr_variant_name.argtypes = [C.c_int, C.c_int]
@@ -219,6 +220,9 @@ class Plan:
 
     def set_graph(self, on=True):
         check(lib().pzl_plan_set_graph(self.h, int(bool(on))))
+
+    def set_overlap(self, on=True):
+        check(lib().pzl_plan_set_overlap(self.h, int(bool(on))))
 
     def variant(self, fftlen):
         v = C.c_int(0)

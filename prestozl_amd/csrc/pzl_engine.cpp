@@ -360,8 +360,8 @@ int launch_sum_part(Plan &p, long long r0, int nb, int set, int slot, float *d_s
     sa.startr = p.d_startr;
     sa.hr_tab = p.d_hr[set];
     sa.int_rlo = p.int_rlo;
-    Scope s(tm, 2, 0, p.st3);
-    PZL_HIPCHK(pzl_launch_harmsum(&sa, p.st3));
+    Scope s(tm, 2, 0, p.sum_stream());
+    PZL_HIPCHK(pzl_launch_harmsum(&sa, p.sum_stream()));
     return 0;
 }
 
@@ -418,10 +418,10 @@ int launch_batch(Plan &p, long long r0, int nb, int set, int slot, float *d_sums
         PZL_HIPCHK(hipEventRecord(p.ev_sum[set], p.st));
         return 0;
     }
-    PZL_HIPCHK(hipStreamWaitEvent(p.st3, p.ev_corr[set], 0));
+    PZL_HIPCHK(hipStreamWaitEvent(p.sum_stream(), p.ev_corr[set], 0));
     const int rs = launch_sum_part(p, r0, nb, set, slot, d_sums, tm);
     if (rs) return rs;
-    PZL_HIPCHK(hipEventRecord(p.ev_sum[set], p.st3));
+    PZL_HIPCHK(hipEventRecord(p.ev_sum[set], p.sum_stream()));
     return 0;
 }
 
@@ -573,12 +573,12 @@ int finalize(Plan &p, int slot, long long r0, int nb, SearchState &ss, Timers *t
     int rc = sync_all(p);
     if (rc) return rc;
     for (int b = 0; b < nb; b++) {
-        rc = reset_slot(p, slot, p.st3);
+        rc = reset_slot(p, slot, p.sum_stream());
         if (rc) return rc;
         rc = launch_batch(p, r0 + b, 1, 0, slot, nullptr, tm);
         if (rc) return rc;
         PZL_HIPCHK(hipMemcpyAsync(p.h_counts + 2 * slot, p.d_counts + 2 * slot, 2 * sizeof(unsigned long long),
-                                  hipMemcpyDeviceToHost, p.st3));
+                                  hipMemcpyDeviceToHost, p.sum_stream()));
         rc = sync_all(p);
         if (rc) return rc;
         const unsigned long long n1 = p.h_counts[2 * slot];
@@ -919,12 +919,12 @@ int pzl_search(void *plan, long long rstep_lo, long long rstep_hi, PzlCand *cand
             if (rc) return rc;
         }
         // the harmonic sum writes into the slot once the host has drained it
-        if (i >= p->nslots) PZL_HIPCHK(hipStreamWaitEvent(p->st3, p->ev_free[slot], 0));
+        if (i >= p->nslots) PZL_HIPCHK(hipStreamWaitEvent(p->sum_stream(), p->ev_free[slot], 0));
         int rc = launch_batch(*p, r0, nb, i % Plan::kSets, slot, nullptr, &tm, true, p->use_graph);
         if (rc) return rc;
         PZL_HIPCHK(hipMemcpyAsync(p->h_counts + 2 * slot, p->d_counts + 2 * slot, 2 * sizeof(unsigned long long),
-                                  hipMemcpyDeviceToHost, p->st3));
-        PZL_HIPCHK(hipEventRecord(p->ev_count[slot], p->st3));
+                                  hipMemcpyDeviceToHost, p->sum_stream()));
+        PZL_HIPCHK(hipEventRecord(p->ev_count[slot], p->sum_stream()));
         pend.push_back({slot, nb, r0});
         // apply whatever the GPU has already finished
         while (head + 1 < pend.size() && hipEventQuery(p->ev_count[pend[head].slot]) == hipSuccess) {
@@ -935,7 +935,7 @@ int pzl_search(void *plan, long long rstep_lo, long long rstep_hi, PzlCand *cand
     }
     if (tm.on) {
         ev_end = tm.get();
-        PZL_HIPCHK(hipEventRecord(ev_end, p->st3));  // the last harmonic sum follows the last correlation
+        PZL_HIPCHK(hipEventRecord(ev_end, p->sum_stream()));  // the last harmonic sum follows the last correlation
     }
     while (head < pend.size()) {
         const Pending q = pend[head++];
@@ -1071,6 +1071,16 @@ int pzl_autotune(void *plan, int reps, char *report, int report_len)
     if (report && report_len > 0) {
         snprintf(report, (size_t)report_len, "%s", rep.c_str());
     }
+    return 0;
+}
+
+int pzl_plan_set_overlap(void *plan, int on)
+{
+    Plan *p = (Plan *)plan;
+    if (!p) return fail(PZL_ERR_ARG, "null plan");
+    PZL_HIPCHK(hipStreamSynchronize(p->st));
+    PZL_HIPCHK(hipStreamSynchronize(p->st3));
+    p->overlap_sum = on != 0;
     return 0;
 }
 

@@ -56,6 +56,7 @@ struct PzlCorrArgs {
     float *planes;
     long long plane_stride;  // floats per rstep
     const PzlHrTab *hr_tab;
+    const float2 *twtab;     // the variant's twiddle table (set by the launcher)
 };
 
 struct PzlSumArgs {

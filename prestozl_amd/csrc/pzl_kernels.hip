@@ -15,6 +15,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
+
 #include <hipcub/hipcub.hpp>
 
 #include "pzl_fft.h"
@@ -308,7 +310,10 @@ template <int N, int NT, int G, bool PING, int... Rs> struct CorrPlan {
     static_assert(ns(NP) == N, "radices do not multiply to N");
 };
 
-template <int N, int NT, int G, bool PING, int... Rs> struct CorrFFT {
+// N: FFT length, G: FFTs per workgroup.  SH = 1: each kernel row is done as
+// two N-point FFTs (its even and odd outputs), FFT g holding row g >> 1,
+// output parity g & 1.
+template <int N, int NT, int G, bool PING, int SH, int... Rs> struct CorrFFT {
     using CP = CorrPlan<N, NT, G, PING, Rs...>;
     static constexpr int NP = CP::NP;
 
@@ -387,20 +392,24 @@ template <int N, int NT, int G, bool PING, int... Rs> struct CorrFFT {
                     // row's numrs powers drops the outputs outside the r window
                     // (negative offsets wrap to out of range) without branches
                     const int gu = __builtin_amdgcn_readfirstlane(g);
-                    const __amdgpu_buffer_rsrc_t rs = uniform_rsrc(pbase + gu * rs_stride, gu < gvalid ? numrs * 4 : 0);
+                    const int row = gu >> SH, par = gu & ((1 << SH) - 1);
+                    const __amdgpu_buffer_rsrc_t rs =
+                        uniform_rsrc(pbase + row * rs_stride, row < gvalid ? numrs * 4 : 0);
 #pragma unroll
                     for (int r = 0; r < R; r++) {
-                        const int ro = j + r * NBP - offset;  // natural-order index minus offset
+                        // natural-order output index (of the row) minus offset
+                        const int ro = ((j + r * NBP) << SH) + par - offset;
                         const float2 y = v[i][r];
                         __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint((y.x * y.x + y.y * y.y) * norm), rs,
                                                               ro * 4, 0, 0);
                     }
                 } else {
-                    if (g < gvalid) {
-                        float *pl = pbase + g * rs_stride;
+                    const int row = g >> SH, par = g & ((1 << SH) - 1);
+                    if (row < gvalid) {
+                        float *pl = pbase + row * rs_stride;
 #pragma unroll
                         for (int r = 0; r < R; r++) {
-                            const int ro = j + r * NBP - offset;  // natural-order index minus offset
+                            const int ro = ((j + r * NBP) << SH) + par - offset;
                             if (ro >= 0 && ro < numrs) {
                                 const float2 y = v[i][r];
                                 pl[ro] = (y.x * y.x + y.y * y.y) * norm;
@@ -417,19 +426,66 @@ template <int N, int NT, int G, bool PING, int... Rs> struct CorrFFT {
     }
 };
 
-template <int N, int NT, int G, bool PING, int... Rs>
+// The twiddles of every thread of a correlation variant, [TWN][NT]: computed
+// once per device in double precision and read by each workgroup of k_corr.
+template <int N, int NT, int G, bool PING, int SH, int... Rs>
+__global__ __launch_bounds__(NT) void k_corr_twiddles(float2 *table)
+{
+    using F = CorrFFT<(N >> SH), NT, (G << SH), PING, SH, Rs...>;
+    using CP = typename F::CP;
+    float2 tw[CP::TWN > 0 ? CP::TWN : 1];
+    F::template init_tw<1>(tw, threadIdx.x);
+#pragma unroll
+    for (int t = 0; t < CP::TWN; t++) table[t * NT + threadIdx.x] = tw[t];
+}
+
+// exp(-2 pi i m / R) for m < R, as compile-time constants
+template <int R> struct ExpTab {
+    float c[R], s[R];
+    constexpr ExpTab() : c(), s()
+    {
+        for (int m = 0; m < R; m++) {
+            c[m] = (float)ct_cos2pi(-m, R);
+            s[m] = (float)ct_sin2pi(-m, R);
+        }
+    }
+};
+
+// N: kernel-row (correlation) length, G: kernel rows per workgroup, SH: 1 to
+// transform each row as two N/2-point FFTs.  Because the segment spectrum
+// repeats with period N/2, y[2m + e] = sum_{k<N/2} P_e[k] exp(2 pi i k m/(N/2))
+// with P_0 = pdata (K*[k] + K*[k+N/2]) and P_1 = pdata (K*[k] - K*[k+N/2])
+// exp(2 pi i k/N): the first radix-2 stage of the N-point transform done on the
+// kernel row once per batch instead of per rstep.
+template <int N, int NT, int G, bool PING, int SH, int... Rs>
 __global__ __launch_bounds__(NT) void k_corr(PzlCorrArgs a)
 {
-    using F = CorrFFT<N, NT, G, PING, Rs...>;
+    constexpr int NF = N >> SH;  // FFT length
+    constexpr int GF = G << SH;  // FFTs per workgroup
+    using F = CorrFFT<NF, NT, GF, PING, SH, Rs...>;
     using CP = typename F::CP;
-    constexpr int R0 = CP::RAD[0], NB0 = N / R0;
+    constexpr int R0 = CP::RAD[0], NB0 = NF / R0;
     extern __shared__ float2 smem[];
     float2 *b0 = smem, *b1 = PING ? smem + CP::BUF : smem;
     const int tid = threadIdx.x;
     const float norm = (float)(1.0 / ((double)N * (double)N));
 
+    // twiddles of this thread, from the variant's table ([TWN][NT], see k_corr_twiddles)
     float2 tw[CP::TWN > 0 ? CP::TWN : 1];
-    F::template init_tw<1>(tw, tid);
+#pragma unroll
+    for (int t = 0; t < CP::TWN; t++) tw[t] = a.twtab[t * NT + tid];
+    // SH = 1: exp(-2 pi i j / N) of each pass-0 butterfly (the odd half's twist)
+    float2 wj[CP::per(0)];
+    if constexpr (SH == 1) {
+#pragma unroll
+        for (int i = 0; i < CP::per(0); i++) {
+            int g, j;
+            F::template where<0>(tid + i * NT, g, j);
+            double sn, cs;
+            sincospi(-2.0 * (double)j / (double)N, &sn, &cs);
+            wj[i] = float2{(float)cs, (float)sn};
+        }
+    }
 
     for (long long unit = blockIdx.x; unit < a.total_items; unit += gridDim.x) {
         // the unit's harmonic and first row (units: per harmonic of the class, ceil(nrows / G))
@@ -451,7 +507,7 @@ __global__ __launch_bounds__(NT) void k_corr(PzlCorrArgs a)
         row0 = __builtin_amdgcn_readfirstlane(row0);
         const PzlHarm &H = a.harms[h];
         const int nrows = __builtin_amdgcn_readfirstlane(H.nrows);
-        // the kernel rows, at the positions pass 0 reads them
+        // the kernel rows (conjugated in the product below), at the positions pass 0 reads them
         float2 kv[CP::per(0)][R0];
 #pragma unroll
         for (int i = 0; i < CP::per(0); i++) {
@@ -461,15 +517,33 @@ __global__ __launch_bounds__(NT) void k_corr(PzlCorrArgs a)
             if constexpr (NB0 % 64 == 0 && CP::full(0)) {
                 // g is uniform in a wave: rows past the harmonic read as zeros
                 const int gu = __builtin_amdgcn_readfirstlane(g);
+                const int row = gu >> SH;
                 const __amdgpu_buffer_rsrc_t rs =
-                    uniform_rsrc(a.bank + H.bank_off + (long long)(row0 + gu) * N, row0 + gu < nrows ? N * 8 : 0);
+                    uniform_rsrc(a.bank + H.bank_off + (long long)(row0 + row) * N, row0 + row < nrows ? N * 8 : 0);
+                auto ld = [&](int k) {
+                    return float2{__uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, k * 8, 0, 0)),
+                                  __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, k * 8 + 4, 0, 0))};
+                };
+                if constexpr (SH == 0) {
 #pragma unroll
-                for (int r = 0; r < R0; r++) {
-                    const int k = j + r * NB0;
-                    kv[i][r].x = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, k * 8, 0, 0));
-                    kv[i][r].y = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, k * 8 + 4, 0, 0));
+                    for (int r = 0; r < R0; r++) kv[i][r] = ld(j + r * NB0);
+                } else {
+                    const int par = gu & 1;
+                    constexpr ExpTab<2 * R0> et{};  // exp(-2 pi i r / (2 R0))
+#pragma unroll
+                    for (int r = 0; r < R0; r++) {
+                        const int k = j + r * NB0;
+                        const float2 k1 = ld(k), k2 = ld(k + NF);
+                        if (par == 0) {
+                            kv[i][r] = cadd(k1, k2);
+                        } else {
+                            // (K[k] - K[k+N/2]) exp(-2 pi i k/N), k = j + r N/(2 R0)
+                            kv[i][r] = cmul(cmul(csub(k1, k2), wj[i]), float2{et.c[r], et.s[r]});
+                        }
+                    }
                 }
             } else {
+                static_assert(SH == 0, "split rows need wave-uniform rows in pass 0");
                 const bool ok = (CP::full(0) || bf < CP::nbf(0)) && row0 + g < nrows;
                 const float2 *kr = a.bank + H.bank_off + (long long)(row0 + g) * N;
 #pragma unroll
@@ -477,8 +551,11 @@ __global__ __launch_bounds__(NT) void k_corr(PzlCorrArgs a)
             }
         }
         const int offset = 2 * H.halfwidth;
-        // segment values of the next rstep are loaded while this one is transformed
-        constexpr int NSV = R0 % 2 == 0 ? R0 / 2 : R0;
+        // segment values of the next rstep are loaded while this one is transformed;
+        // without the split, the spread segment's period N/2 lets an even radix
+        // read each value once
+        constexpr bool HALFPER = SH == 0 && R0 % 2 == 0;
+        constexpr int NSV = HALFPER ? R0 / 2 : R0;
         float2 sv[CP::per(0)][NSV];
         auto load_seg = [&](int b) {
             const float2 *seg = a.segs + (long long)b * a.seg_stride + H.seg_off;
@@ -506,8 +583,7 @@ __global__ __launch_bounds__(NT) void k_corr(PzlCorrArgs a)
 #pragma unroll
                 for (int r = 0; r < NSV; r++) cur[i][r] = sv[i][r];
             if (b + 1 < a.nb) load_seg(b + 1);
-            // pass 0: P[k] = seg[k mod N/2] * conj(kern[k]); the spread segment repeats
-            // with period N/2, so an even radix reads each segment value once
+            // pass 0: P[k] = seg[k mod N/2] * conj(kern[k])
 #pragma unroll
             for (int i = 0; i < CP::per(0); i++) {
                 const int bf = tid + i * NT;
@@ -515,26 +591,16 @@ __global__ __launch_bounds__(NT) void k_corr(PzlCorrArgs a)
                     int g, j;
                     F::template where<0>(bf, g, j);
                     float2 v[R0];
-                    if constexpr (R0 % 2 == 0) {
 #pragma unroll
-                        for (int r = 0; r < R0 / 2; r++) {
-                            const float2 pd = cur[i][r];
-                            const float2 k1 = kv[i][r], k2 = kv[i][r + R0 / 2];
-                            // re = dr*kr + di*ki, im = di*kr - dr*ki (accel_utils.c:1262-1265)
-                            v[r] = float2{pd.x * k1.x + pd.y * k1.y, pd.y * k1.x - pd.x * k1.y};
-                            v[r + R0 / 2] = float2{pd.x * k2.x + pd.y * k2.y, pd.y * k2.x - pd.x * k2.y};
-                        }
-                    } else {
-#pragma unroll
-                        for (int r = 0; r < R0; r++) {
-                            const float2 pd = cur[i][r];
-                            const float2 kk = kv[i][r];
-                            v[r] = float2{pd.x * kk.x + pd.y * kk.y, pd.y * kk.x - pd.x * kk.y};
-                        }
+                    for (int r = 0; r < R0; r++) {
+                        const float2 pd = cur[i][HALFPER ? r % (R0 / 2) : r];
+                        const float2 kk = kv[i][r];
+                        // re = dr*kr + di*ki, im = di*kr - dr*ki (accel_utils.c:1262-1265)
+                        v[r] = float2{pd.x * kk.x + pd.y * kk.y, pd.y * kk.x - pd.x * kk.y};
                     }
                     dftR<R0, 1>(v);
 #pragma unroll
-                    for (int r = 0; r < R0; r++) b0[lds_pad(g * N + j * R0 + r)] = v[r];
+                    for (int r = 0; r < R0; r++) b0[lds_pad(g * NF + j * R0 + r)] = v[r];
                 }
             }
             __syncthreads();
@@ -834,27 +900,47 @@ extern "C" hipError_t pzl_launch_segments(const PzlBatchArgs *a, int max_fftlen,
 // One correlation variant: FFT length N, NT threads, G rows per workgroup,
 // ping-pong LDS or not, radices.  The grid is persistent: as many workgroups
 // as fit on the device at once (or fewer when there are fewer units).
-template <int N, int NT, int G, bool PING, int... Rs>
+// One correlation variant: kernel-row length N, NT threads, G rows per
+// workgroup, ping-pong LDS or not, split rows or not, radices.  One workgroup
+// per G rows of a harmonic (the grid is not persistent, so that kernels of
+// concurrent searches interleave); its twiddles come from a table built once
+// per device.
+template <int N, int NT, int G, bool PING, int SH, int... Rs>
 static hipError_t launch_corr_t(const PzlCorrArgs *a, const int *class_nrows, hipStream_t st)
 {
-    using CP = CorrPlan<N, NT, G, PING, Rs...>;
-    static int max_grid = 0;
-    if (max_grid == 0) {
-        int dev = 0, cus = 0, per_cu = 0;
-        hipError_t e = hipGetDevice(&dev);
-        if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        if (e == hipSuccess)
-            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_corr<N, NT, G, PING, Rs...>, NT, CP::smem());
-        if (e != hipSuccess) return e;
-        max_grid = std::max(1, cus * std::max(1, per_cu));
+    using CP = CorrPlan<(N >> SH), NT, (G << SH), PING, Rs...>;
+    static std::mutex mu;
+    static float2 *tables[64] = {};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= 64) return hipErrorInvalidDevice;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        if (!tables[dev]) {
+            // built on first use (never inside a stream capture: the first
+            // batch of every plan runs eagerly), kept until the process ends
+            float2 *t = nullptr;
+            e = hipMalloc(&t, sizeof(float2) * (size_t)(CP::TWN > 0 ? CP::TWN : 1) * NT);
+            if (e != hipSuccess) return e;
+            hipLaunchKernelGGL((k_corr_twiddles<N, NT, G, PING, SH, Rs...>), dim3(1), dim3(NT), 0, st, t);
+            e = hipGetLastError();
+            if (e == hipSuccess) e = hipStreamSynchronize(st);
+            if (e != hipSuccess) {
+                (void)hipFree(t);
+                return e;
+            }
+            tables[dev] = t;
+        }
     }
     long long units = 0;
     for (int c = 0; c < a->nclass; c++) units += (class_nrows[c] + G - 1) / G;
     if (units < 1) return hipSuccess;
+    if (units > 0x7fffffffLL) return hipErrorInvalidValue;
     PzlCorrArgs a2 = *a;
     a2.total_items = units;
-    const int grid = (int)std::min<long long>(units, max_grid);
-    hipLaunchKernelGGL((k_corr<N, NT, G, PING, Rs...>), dim3(grid), dim3(NT), CP::smem(), st, a2);
+    a2.twtab = tables[dev];
+    hipLaunchKernelGGL((k_corr<N, NT, G, PING, SH, Rs...>), dim3((unsigned)units), dim3(NT), CP::smem(), st, a2);
     return hipGetLastError();
 }
 
@@ -866,7 +952,10 @@ struct CorrVariant {
 };
 
 #define PZL_CV(N, NT, G, PING, ...) \
-    {N, launch_corr_t<N, NT, G, PING, __VA_ARGS__>, #N ":" #NT "t:" #G "r:" #PING ":" #__VA_ARGS__}
+    {N, launch_corr_t<N, NT, G, PING, 0, __VA_ARGS__>, #N ":" #NT "t:" #G "r:" #PING ":" #__VA_ARGS__}
+// the same with each row split into its even and odd outputs (two N/2-point FFTs)
+#define PZL_CVS(N, NT, G, PING, ...) \
+    {N, launch_corr_t<N, NT, G, PING, 1, __VA_ARGS__>, #N ":" #NT "t:" #G "r:" #PING ":split:" #__VA_ARGS__}
 
 // Candidates per FFT length; the first of each length is the default, the
 // autotuner may pick another on the device.
@@ -887,13 +976,17 @@ static const CorrVariant g_corr_variants[] = {
     PZL_CV(1024, 128, 2, true, 16, 16, 4),
     PZL_CV(1024, 256, 2, true, 8, 8, 4, 4),
     PZL_CV(1024, 64, 1, true, 16, 16, 4),
+    PZL_CVS(1024, 128, 1, true, 8, 8, 8),
     PZL_CV(1280, 128, 1, true, 16, 16, 5),
     PZL_CV(1280, 256, 2, true, 16, 16, 5),
     PZL_CV(2048, 128, 1, true, 16, 16, 8),
     PZL_CV(2048, 256, 1, true, 8, 8, 8, 4),
     PZL_CV(2048, 256, 2, false, 16, 16, 8),
+    PZL_CVS(2048, 128, 1, true, 16, 16, 4),
+    PZL_CVS(2048, 256, 2, false, 16, 16, 4),
     PZL_CV(4096, 256, 1, false, 16, 16, 16),
     PZL_CV(4096, 512, 1, false, 8, 8, 8, 8),
+    PZL_CVS(4096, 256, 1, false, 16, 16, 8),
 };
 static const int g_ncorr_variants = (int)(sizeof(g_corr_variants) / sizeof(g_corr_variants[0]));
 

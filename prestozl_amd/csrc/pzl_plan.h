@@ -50,7 +50,8 @@ struct Plan {
     int device = 0;
     hipStream_t st = nullptr;   // segments and correlations
     hipStream_t st2 = nullptr;  // hit ordering
-    hipStream_t st3 = nullptr;  // harmonic sums
+    hipStream_t st3 = nullptr;  // harmonic sums (when overlapped)
+    bool overlap_sum = true;    // harmonic sums on st3, concurrent with the next batch's correlation
     // resident
     PzlHarm *d_harms = nullptr;
     float2 *d_bank = nullptr;  // shared, owned by the bank cache
@@ -99,6 +100,7 @@ struct Plan {
     static constexpr int kR0Ring = 2 * kSlots;
     long long *h_r0 = nullptr;  // pinned staging ring of those values
     unsigned long long r0_ring = 0;
+    hipStream_t sum_stream() const { return overlap_sum ? st3 : st; }
     // results of the last search
     std::vector<PzlHostHit> last_hits;
     std::vector<PzlCand> last_cands;

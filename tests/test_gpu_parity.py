@@ -52,6 +52,23 @@ def _c2_small():
     return fft, kw
 
 
+def _c3_small():
+    """zmax 200 / wmax 500 / 16 harmonics (4096-point fundamental) at a reduced length."""
+    from prestozl_amd import synth
+    fft, N, dt, opts, sigs = synth.config3(N=1 << 21)
+    kw = dict(N=N, dt=dt, numbins=len(fft), zmax=200, wmax=500, numharm=16, sigma=5.0, wmaxP=True, batchsize=2)
+    return fft, kw, sigs
+
+
+def _shape(config1_data, which):
+    if which == "c1":
+        return _c1(config1_data)
+    if which == "c2":
+        return _c2_small()
+    fft, kw, _ = _c3_small()
+    return fft, kw
+
+
 def test_geometry_matches_oracle(config1_data):
     fft, kw = _c1(config1_data)
     plan = _plan(kw, fft)
@@ -115,18 +132,25 @@ def test_planes(config1_data, which):
     plan.close()
 
 
-@pytest.mark.parametrize("which", ["c1", "c2"])
+@pytest.mark.parametrize("which", ["c1", "c2", "c3"])
 def test_summed_powers(config1_data, which):
-    fft, kw = _c1(config1_data) if which == "c1" else _c2_small()
+    """Final summed fundamental plane vs the FP32 oracle (tolerance TOL_POW)
+    and the FP64 correlation-and-sum oracle (max and RMS error reported)."""
+    fft, kw = _shape(config1_data, which)
     plan = _plan(kw, fft)
     obs, ctx32 = _oracle(kw, O.FP32)
+    _, ctx64 = _oracle(kw, O.FP64)
     nr = plan.geom.nrsteps
     for rstep in (0, nr // 3):
         sg = plan.sums(rstep)
         _, _, s32 = ctx32.search(fft, rstep, rstep + 1, sum_rstep=rstep)
+        _, _, s64 = ctx64.search(fft, rstep, rstep + 1, sum_rstep=rstep)
         e, rms = rel_err(sg, s32, floor=1.0)
-        print("rstep %d summed powers max rel err %.3g rms %.3g" % (rstep, e, rms))
+        e64, rms64 = rel_err(sg, s64, floor=1.0)
+        print("%s rstep %d summed powers vs fp32: max rel err %.3g rms %.3g; vs fp64: max %.3g rms %.3g"
+              % (which, rstep, e, rms, e64, rms64))
         assert e < TOL_POW
+        assert e64 < TOL_POW
     plan.close()
 
 
@@ -184,6 +208,41 @@ def test_search_config2_band():
     print("config2 band: %d gpu hits, %d oracle hits, %d near, err %.3g, cands %d/%d"
           % (len(hg), len(ho), near, worst, len(cg), len(co)))
     assert worst < TOL_POW
+    _assert_same_cands(cg, co, near)
+    plan.close()
+
+
+def _assert_same_cands(cg, co, near):
+    """(r, z, w, numharm) identical and sigma within 1e-4, unless hits near the
+    threshold differ (then only the count is reported)."""
+    key = lambda c: (round(c[3], 6), round(c[4], 6), round(c[5], 6), c[2])
+    if near == 0:
+        assert [key(c) for c in cg] == [key(c) for c in co]
+        for a, b in zip(cg, co):
+            assert abs(a[1] - b[1]) <= 1e-4 * max(b[1], 1.0)
+
+
+def test_search_config3_band():
+    """Config-3 geometry (zmax 200, wmax 500: 4096-point fundamental) on the
+    rsteps holding an injected signal."""
+    fft, kw, sigs = _c3_small()
+    kw["sigma"] = 3.0
+    plan = _plan(kw, fft)
+    obs, ctx32 = _oracle(kw, O.FP32)
+    T = kw["N"] * kw["dt"]
+    amp, f0, z, w = sigs[0]
+    r_mean = f0 * T + z / 2.0 + w / 6.0  # mean Fourier frequency over the observation
+    k = int((r_mean - obs.rlo) // (obs.corr_uselen // 2))
+    lo, hi = max(k - 1, 0), k + 2
+    cg, st = plan.search(lo, hi)
+    hg, pg = plan.hits()
+    co, ho, _ = ctx32.search(fft, lo, hi)
+    near, worst, common = _compare_hits(plan, hg, pg, ho, list(obs.powcut))
+    print("config3 band [%d, %d): %d gpu hits, %d oracle hits, %d near, err %.3g, cands %d/%d"
+          % (lo, hi, len(hg), len(ho), near, worst, len(cg), len(co)))
+    assert common > 0
+    assert worst < TOL_POW
+    _assert_same_cands(cg, co, near)
     plan.close()
 
 
