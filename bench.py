@@ -163,6 +163,8 @@ def main():
     ap.add_argument("--serve-inflight", type=int, default=3)
     ap.add_argument("--dm-trials", type=int, default=564, help="files in the DM-trial leg (0: skip)")
     ap.add_argument("--no-tune", action="store_true", help="use the default correlation variants")
+    ap.add_argument("--no-legs", action="store_true",
+                    help="only the timed searches (no graph-replay / one-stream / plain / serving / DM-trial legs)")
     ap.add_argument("--rsteps", type=int, default=0,
                     help="profiling runs only: search this many rsteps from the middle of the band")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
@@ -257,21 +259,24 @@ def main():
     }
     # variants of the execution: graph replay of every full batch (on top of the
     # default overlap), and everything in one stream without overlap
-    plan.set_graph(True)
-    plan.search(lo, hi, stats=False)  # captures
-    ts = time.perf_counter()
-    cg, _ = plan.search(lo, hi, stats=False)
-    tg = time.perf_counter() - ts
-    out["graph_replay"] = {"value": cells / tg, "unit": "cells/s", "ms_per_search": tg * 1e3,
-                           "same_candidates": cg == cands}
-    plan.set_graph(False)
-    plan.set_overlap(False)
-    ts = time.perf_counter()
-    cs, _ = plan.search(lo, hi, stats=False)
-    tsr = time.perf_counter() - ts
-    out["serial_one_stream"] = {"value": cells / tsr, "unit": "cells/s", "ms_per_search": tsr * 1e3,
-                                "same_candidates": cs == cands}
-    plan.set_overlap(True)
+    if args.no_legs:
+        args.no_plain, args.serve, args.dm_trials = True, 0, 0
+    plan.set_graph(not args.no_legs)
+    if not args.no_legs:
+        plan.search(lo, hi, stats=False)  # captures
+        ts = time.perf_counter()
+        cg, _ = plan.search(lo, hi, stats=False)
+        tg = time.perf_counter() - ts
+        out["graph_replay"] = {"value": cells / tg, "unit": "cells/s", "ms_per_search": tg * 1e3,
+                               "same_candidates": cg == cands}
+        plan.set_graph(False)
+        plan.set_overlap(False)
+        ts = time.perf_counter()
+        cs, _ = plan.search(lo, hi, stats=False)
+        tsr = time.perf_counter() - ts
+        out["serial_one_stream"] = {"value": cells / tsr, "unit": "cells/s", "ms_per_search": tsr * 1e3,
+                                    "same_candidates": cs == cands}
+        plan.set_overlap(True)
     if not args.no_plain and not args.rsteps:
         plan1 = B.Plan(B.make_config(batchsize=1, **kw))
         plan1.set_spectrum(fft)

@@ -267,7 +267,9 @@ __global__ __launch_bounds__(SEG_NT) void k_segments(PzlBatchArgs a)
 // them once (double-precision sincospi) and keeps them in registers for all of
 // its (row, rstep) pairs (the grid is persistent).
 
-__host__ __device__ constexpr int lds_pad(int p) { return p + (p >> 4); }
+// LDS index padding: one element after every 2^S (breaks bank conflicts of
+// strided exchanges)
+template <int S> __host__ __device__ constexpr int lds_pad(int p) { return p + (p >> S); }
 
 // Word 3 of a raw (stride 0) buffer resource on gfx9-family parts: 32-bit
 // data format; with stride 0, an access at byte offset >= num_records is
@@ -285,7 +287,7 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t uniform_rsrc(const void *p, in
     return __builtin_amdgcn_make_buffer_rsrc((void *)(((unsigned long long)hi << 32) | lo), 0, n, PZL_BUF_DWORD3);
 }
 
-template <int N, int NT, int G, bool PING, int... Rs> struct CorrPlan {
+template <int N, int NT, int G, bool PING, int PADSH, int... Rs> struct CorrPlan {
     static constexpr int NP = sizeof...(Rs);
     static constexpr int RAD[NP] = {Rs...};
     static constexpr int ns(int p)
@@ -305,7 +307,7 @@ template <int N, int NT, int G, bool PING, int... Rs> struct CorrPlan {
     }
     static constexpr int TWN = twoff(NP);
     static constexpr int PPT = per(0) * RAD[0];  // kernel values held per thread
-    static constexpr int BUF = lds_pad(G * N) + 1;
+    static constexpr int BUF = lds_pad<PADSH>(G * N) + 1;
     static constexpr size_t smem() { return (size_t)(PING ? 2 : 1) * BUF * sizeof(float2); }
     static_assert(ns(NP) == N, "radices do not multiply to N");
 };
@@ -313,8 +315,8 @@ template <int N, int NT, int G, bool PING, int... Rs> struct CorrPlan {
 // N: FFT length, G: FFTs per workgroup.  SH = 1: each kernel row is done as
 // two N-point FFTs (its even and odd outputs), FFT g holding row g >> 1,
 // output parity g & 1.
-template <int N, int NT, int G, bool PING, int SH, int... Rs> struct CorrFFT {
-    using CP = CorrPlan<N, NT, G, PING, Rs...>;
+template <int N, int NT, int G, bool PING, int SH, int PADSH, int... Rs> struct CorrFFT {
+    using CP = CorrPlan<N, NT, G, PING, PADSH, Rs...>;
     static constexpr int NP = CP::NP;
 
     // butterfly bf of pass P: FFT g, index j within that FFT's N / R butterflies
@@ -362,10 +364,12 @@ template <int N, int NT, int G, bool PING, int SH, int... Rs> struct CorrFFT {
                 int g, j;
                 where<P>(bf, g, j);
                 // stride NBP: with NBP a multiple of 16 the padded index is linear in r
-                const float2 *rp = src + lds_pad(g * N + j);
+                constexpr int PM = 1 << PADSH;
+                const float2 *rp = src + lds_pad<PADSH>(g * N + j);
 #pragma unroll
                 for (int r = 0; r < R; r++) {
-                    float2 x = (NBP % 16 == 0) ? rp[r * lds_pad(NBP)] : src[lds_pad(g * N + j + r * NBP)];
+                    float2 x = (NBP % PM == 0) ? rp[r * lds_pad<PADSH>(NBP)]
+                                               : src[lds_pad<PADSH>(g * N + j + r * NBP)];
                     if (r > 0) x = cmul(x, tw[CP::twoff(P) + i * (R - 1) + (r - 1)]);
                     v[i][r] = x;
                 }
@@ -381,11 +385,11 @@ template <int N, int NT, int G, bool PING, int SH, int... Rs> struct CorrFFT {
                 dftR<R, 1>(v[i]);
                 if constexpr (!last) {
                     const int idxD = (j / Ns) * Ns * R + (j % Ns);
-                    float2 *wp = dst + lds_pad(g * N + idxD);
+                    float2 *wp = dst + lds_pad<PADSH>(g * N + idxD);
 #pragma unroll
                     for (int r = 0; r < R; r++) {
-                        if constexpr (Ns % 16 == 0) wp[r * lds_pad(Ns)] = v[i][r];
-                        else dst[lds_pad(g * N + idxD + r * Ns)] = v[i][r];
+                        if constexpr (Ns % (1 << PADSH) == 0) wp[r * lds_pad<PADSH>(Ns)] = v[i][r];
+                        else dst[lds_pad<PADSH>(g * N + idxD + r * Ns)] = v[i][r];
                     }
                 } else if constexpr (NBP % 64 == 0) {
                     // g is uniform in a wave: a buffer store whose range is the
@@ -428,10 +432,10 @@ template <int N, int NT, int G, bool PING, int SH, int... Rs> struct CorrFFT {
 
 // The twiddles of every thread of a correlation variant, [TWN][NT]: computed
 // once per device in double precision and read by each workgroup of k_corr.
-template <int N, int NT, int G, bool PING, int SH, int... Rs>
+template <int N, int NT, int G, bool PING, int SH, int PADSH, int... Rs>
 __global__ __launch_bounds__(NT) void k_corr_twiddles(float2 *table)
 {
-    using F = CorrFFT<(N >> SH), NT, (G << SH), PING, SH, Rs...>;
+    using F = CorrFFT<(N >> SH), NT, (G << SH), PING, SH, PADSH, Rs...>;
     using CP = typename F::CP;
     float2 tw[CP::TWN > 0 ? CP::TWN : 1];
     F::template init_tw<1>(tw, threadIdx.x);
@@ -457,12 +461,12 @@ template <int R> struct ExpTab {
 // with P_0 = pdata (K*[k] + K*[k+N/2]) and P_1 = pdata (K*[k] - K*[k+N/2])
 // exp(2 pi i k/N): the first radix-2 stage of the N-point transform done on the
 // kernel row once per batch instead of per rstep.
-template <int N, int NT, int G, bool PING, int SH, int... Rs>
+template <int N, int NT, int G, bool PING, int SH, int PADSH, int... Rs>
 __global__ __launch_bounds__(NT) void k_corr(PzlCorrArgs a)
 {
     constexpr int NF = N >> SH;  // FFT length
     constexpr int GF = G << SH;  // FFTs per workgroup
-    using F = CorrFFT<NF, NT, GF, PING, SH, Rs...>;
+    using F = CorrFFT<NF, NT, GF, PING, SH, PADSH, Rs...>;
     using CP = typename F::CP;
     constexpr int R0 = CP::RAD[0], NB0 = NF / R0;
     extern __shared__ float2 smem[];
@@ -600,7 +604,7 @@ __global__ __launch_bounds__(NT) void k_corr(PzlCorrArgs a)
                     }
                     dftR<R0, 1>(v);
 #pragma unroll
-                    for (int r = 0; r < R0; r++) b0[lds_pad(g * NF + j * R0 + r)] = v[r];
+                    for (int r = 0; r < R0; r++) b0[lds_pad<PADSH>(g * NF + j * R0 + r)] = v[r];
                 }
             }
             __syncthreads();
@@ -905,10 +909,10 @@ extern "C" hipError_t pzl_launch_segments(const PzlBatchArgs *a, int max_fftlen,
 // per G rows of a harmonic (the grid is not persistent, so that kernels of
 // concurrent searches interleave); its twiddles come from a table built once
 // per device.
-template <int N, int NT, int G, bool PING, int SH, int... Rs>
+template <int N, int NT, int G, bool PING, int SH, int PADSH, int... Rs>
 static hipError_t launch_corr_t(const PzlCorrArgs *a, const int *class_nrows, hipStream_t st)
 {
-    using CP = CorrPlan<(N >> SH), NT, (G << SH), PING, Rs...>;
+    using CP = CorrPlan<(N >> SH), NT, (G << SH), PING, PADSH, Rs...>;
     static std::mutex mu;
     static float2 *tables[64] = {};
     int dev = 0;
@@ -923,7 +927,7 @@ static hipError_t launch_corr_t(const PzlCorrArgs *a, const int *class_nrows, hi
             float2 *t = nullptr;
             e = hipMalloc(&t, sizeof(float2) * (size_t)(CP::TWN > 0 ? CP::TWN : 1) * NT);
             if (e != hipSuccess) return e;
-            hipLaunchKernelGGL((k_corr_twiddles<N, NT, G, PING, SH, Rs...>), dim3(1), dim3(NT), 0, st, t);
+            hipLaunchKernelGGL((k_corr_twiddles<N, NT, G, PING, SH, PADSH, Rs...>), dim3(1), dim3(NT), 0, st, t);
             e = hipGetLastError();
             if (e == hipSuccess) e = hipStreamSynchronize(st);
             if (e != hipSuccess) {
@@ -940,7 +944,8 @@ static hipError_t launch_corr_t(const PzlCorrArgs *a, const int *class_nrows, hi
     PzlCorrArgs a2 = *a;
     a2.total_items = units;
     a2.twtab = tables[dev];
-    hipLaunchKernelGGL((k_corr<N, NT, G, PING, SH, Rs...>), dim3((unsigned)units), dim3(NT), CP::smem(), st, a2);
+    hipLaunchKernelGGL((k_corr<N, NT, G, PING, SH, PADSH, Rs...>), dim3((unsigned)units), dim3(NT), CP::smem(), st,
+                       a2);
     return hipGetLastError();
 }
 
@@ -952,10 +957,14 @@ struct CorrVariant {
 };
 
 #define PZL_CV(N, NT, G, PING, ...) \
-    {N, launch_corr_t<N, NT, G, PING, 0, __VA_ARGS__>, #N ":" #NT "t:" #G "r:" #PING ":" #__VA_ARGS__}
+    {N, launch_corr_t<N, NT, G, PING, 0, 4, __VA_ARGS__>, #N ":" #NT "t:" #G "r:" #PING ":" #__VA_ARGS__}
 // the same with each row split into its even and odd outputs (two N/2-point FFTs)
 #define PZL_CVS(N, NT, G, PING, ...) \
-    {N, launch_corr_t<N, NT, G, PING, 1, __VA_ARGS__>, #N ":" #NT "t:" #G "r:" #PING ":split:" #__VA_ARGS__}
+    {N, launch_corr_t<N, NT, G, PING, 1, 4, __VA_ARGS__>, #N ":" #NT "t:" #G "r:" #PING ":split:" #__VA_ARGS__}
+// the default with LDS padding of one element per 2^PADSH
+#define PZL_CVP(N, NT, G, PING, PADSH, ...)                                                                   \
+    {N, launch_corr_t<N, NT, G, PING, 0, PADSH, __VA_ARGS__>,                                                 \
+     #N ":" #NT "t:" #G "r:" #PING ":pad" #PADSH ":" #__VA_ARGS__}
 
 // Candidates per FFT length; the first of each length is the default, the
 // autotuner may pick another on the device.
@@ -979,9 +988,13 @@ static const CorrVariant g_corr_variants[] = {
     PZL_CVS(1024, 128, 1, true, 8, 8, 8),
     PZL_CV(1280, 128, 1, true, 16, 16, 5),
     PZL_CV(1280, 256, 2, true, 16, 16, 5),
+    PZL_CVP(1280, 128, 1, true, 3, 16, 16, 5),
     PZL_CV(2048, 128, 1, true, 16, 16, 8),
     PZL_CV(2048, 256, 1, true, 8, 8, 8, 4),
     PZL_CV(2048, 256, 2, false, 16, 16, 8),
+    PZL_CVP(2048, 256, 1, true, 3, 8, 8, 8, 4),
+    PZL_CVP(2048, 256, 1, true, 5, 8, 8, 8, 4),
+    PZL_CVP(2048, 128, 1, true, 3, 16, 16, 8),
     PZL_CVS(2048, 128, 1, true, 16, 16, 4),
     PZL_CVS(2048, 256, 2, false, 16, 16, 4),
     PZL_CV(4096, 256, 1, false, 16, 16, 16),
