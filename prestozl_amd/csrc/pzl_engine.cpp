@@ -193,6 +193,7 @@ void free_plan(Plan *p)
     (void)hipSetDevice(p->device);
     if (p->st) (void)hipStreamSynchronize(p->st);
     if (p->st3) (void)hipStreamSynchronize(p->st3);
+    if (p->st4) (void)hipStreamSynchronize(p->st4);
     if (p->st2) (void)hipStreamSynchronize(p->st2);
     for (auto &c : p->classes) (void)hipFree(c.d_harms);
     (void)hipFree(p->d_harms);
@@ -208,6 +209,7 @@ void free_plan(Plan *p)
         (void)hipFree(p->d_planes[k]);
         if (p->ev_corr[k]) (void)hipEventDestroy(p->ev_corr[k]);
         if (p->ev_sum[k]) (void)hipEventDestroy(p->ev_sum[k]);
+        if (p->ev_seg[k]) (void)hipEventDestroy(p->ev_seg[k]);
     }
     for (int s = 0; s < Plan::kSlots; s++) {
         (void)hipFree(p->d_hkeys[s]);
@@ -231,11 +233,13 @@ void free_plan(Plan *p)
     if (p->st) (void)hipStreamDestroy(p->st);
     if (p->st2) (void)hipStreamDestroy(p->st2);
     if (p->st3) (void)hipStreamDestroy(p->st3);
+    if (p->st4) (void)hipStreamDestroy(p->st4);
     delete p;
 }
 
 struct Timers {
     bool on = false;
+    bool active = true;  // record this batch (searches sample every kSampleEvery-th batch)
     std::vector<hipEvent_t> pool;
     size_t used = 0;
     struct Rec {
@@ -267,14 +271,14 @@ struct Scope {
     hipEvent_t a = nullptr;
     Scope(Timers *t, int k, int f, hipStream_t s) : tm(t), kind(k), fftlen(f), st(s)
     {
-        if (tm && tm->on) {
+        if (tm && tm->on && tm->active) {
             a = tm->get();
             (void)hipEventRecord(a, st);
         }
     }
     ~Scope()
     {
-        if (tm && tm->on) {
+        if (a) {
             hipEvent_t b = tm->get();
             (void)hipEventRecord(b, st);
             tm->recs.push_back({kind, fftlen, a, b});
@@ -301,7 +305,8 @@ PzlCorrArgs corr_args(const Plan &p, const SizeClass &c, int nb, int set)
 
 // Segments and correlations of rsteps [r0, r0 + nb) into buffer set 'set',
 // on the correlation stream (r0_dev: read the first rstep from there).
-int launch_corr_part(Plan &p, long long r0, int nb, int set, Timers *tm, const long long *r0_dev)
+int launch_corr_part(Plan &p, long long r0, int nb, int set, Timers *tm, const long long *r0_dev,
+                     bool seg_stream = false)
 {
     PzlBatchArgs ba{};
     ba.rstep0_dev = r0_dev;
@@ -320,9 +325,17 @@ int launch_corr_part(Plan &p, long long r0, int nb, int set, Timers *tm, const l
     ba.seg_stride = p.seg_stride;
     ba.rinds = p.d_rinds[set];
     ba.hr_tab = p.d_hr[set];
+    // the segments of a batch can run ahead on their own stream, beside the
+    // correlation of the previous batch, once the set's last sum is done
+    hipStream_t ss = seg_stream ? p.st4 : p.st;
+    if (seg_stream) PZL_HIPCHK(hipStreamWaitEvent(ss, p.ev_sum[set], 0));
     {
-        Scope s(tm, 0, 0, p.st);
-        PZL_HIPCHK(pzl_launch_segments(&ba, p.max_fftlen, p.st));
+        Scope s(tm, 0, 0, ss);
+        PZL_HIPCHK(pzl_launch_segments(&ba, p.max_fftlen, ss));
+    }
+    if (seg_stream) {
+        PZL_HIPCHK(hipEventRecord(p.ev_seg[set], ss));
+        PZL_HIPCHK(hipStreamWaitEvent(p.st, p.ev_seg[set], 0));
     }
     for (auto &c : p.classes) {
         const PzlCorrArgs ca = corr_args(p, c, nb, set);
@@ -407,11 +420,11 @@ int launch_corr_graph(Plan &p, long long r0, int set, Timers *tm)
 // correlation, so the correlation of one batch overlaps the sum of the
 // previous one.
 int launch_batch(Plan &p, long long r0, int nb, int set, int slot, float *d_sums, Timers *tm, bool with_sum = true,
-                 bool graph = false)
+                 bool graph = false, bool seg_ahead = false)
 {
     PZL_HIPCHK(hipStreamWaitEvent(p.st, p.ev_sum[set], 0));
     const int rc = (graph && nb == p.B) ? launch_corr_graph(p, r0, set, tm)
-                                        : launch_corr_part(p, r0, nb, set, tm, nullptr);
+                                        : launch_corr_part(p, r0, nb, set, tm, nullptr, seg_ahead);
     if (rc) return rc;
     PZL_HIPCHK(hipEventRecord(p.ev_corr[set], p.st));
     if (!with_sum) {
@@ -428,6 +441,7 @@ int launch_batch(Plan &p, long long r0, int nb, int set, int slot, float *d_sums
 // Wait for every stream of the plan.
 int sync_all(Plan &p)
 {
+    PZL_HIPCHK(hipStreamSynchronize(p.st4));
     PZL_HIPCHK(hipStreamSynchronize(p.st));
     PZL_HIPCHK(hipStreamSynchronize(p.st3));
     PZL_HIPCHK(hipStreamSynchronize(p.st2));
@@ -761,6 +775,7 @@ int pzl_plan_create(const PzlConfig *cfg, int device, void **plan_out)
     PCHK(hipStreamCreateWithFlags(&p->st, hipStreamNonBlocking));
     PCHK(hipStreamCreateWithFlags(&p->st2, hipStreamNonBlocking));
     PCHK(hipStreamCreateWithFlags(&p->st3, hipStreamNonBlocking));
+    PCHK(hipStreamCreateWithFlags(&p->st4, hipStreamNonBlocking));
     p->nslots = sizes.nslots;
     for (int s = 0; s < p->nslots; s++) {
         PCHK(hipEventCreateWithFlags(&p->ev_count[s], hipEventDisableTiming));
@@ -794,6 +809,8 @@ int pzl_plan_create(const PzlConfig *cfg, int device, void **plan_out)
         PCHK(hipMalloc(&p->d_planes[k], sizeof(float) * (size_t)p->plane_stride * p->B));
         PCHK(hipEventCreateWithFlags(&p->ev_corr[k], hipEventDisableTiming));
         PCHK(hipEventCreateWithFlags(&p->ev_sum[k], hipEventDisableTiming));
+        PCHK(hipEventCreateWithFlags(&p->ev_seg[k], hipEventDisableTiming));
+        PCHK(hipEventRecord(p->ev_seg[k], p->st));
         // recorded once so that the first waits on them are satisfied
         PCHK(hipEventRecord(p->ev_corr[k], p->st));
         PCHK(hipEventRecord(p->ev_sum[k], p->st));
@@ -910,9 +927,15 @@ int pzl_search(void *plan, long long rstep_lo, long long rstep_hi, PzlCand *cand
     std::vector<Pending> pend;
     size_t head = 0;
     int i = 0;
+    // per-kernel events on every kSampleEvery-th batch only: the averages stay
+    // representative and the event packets do not perturb the timed search
+    constexpr int kSampleEvery = 8;
+    long long sampled = 0;
     for (long long r0 = rstep_lo; r0 < rstep_hi; r0 += p->B, i++) {
         const int nb = (int)std::min<long long>(p->B, rstep_hi - r0);
         const int slot = i % p->nslots;
+        tm.active = (i % kSampleEvery) == 0;
+        sampled += tm.active;
         if (pend.size() - head == (size_t)p->nslots) {
             const Pending q = pend[head++];
             int rc = finalize(*p, q.slot, q.r0, q.nb, ss, &tm);
@@ -920,7 +943,9 @@ int pzl_search(void *plan, long long rstep_lo, long long rstep_hi, PzlCand *cand
         }
         // the harmonic sum writes into the slot once the host has drained it
         if (i >= p->nslots) PZL_HIPCHK(hipStreamWaitEvent(p->sum_stream(), p->ev_free[slot], 0));
-        int rc = launch_batch(*p, r0, nb, i % Plan::kSets, slot, nullptr, &tm, true, p->use_graph);
+        // (segments ahead on their own stream read only what is constant during
+        // the search: the spectrum and the startr table)
+        int rc = launch_batch(*p, r0, nb, i % Plan::kSets, slot, nullptr, &tm, true, p->use_graph, p->overlap_sum);
         if (rc) return rc;
         PZL_HIPCHK(hipMemcpyAsync(p->h_counts + 2 * slot, p->d_counts + 2 * slot, 2 * sizeof(unsigned long long),
                                   hipMemcpyDeviceToHost, p->sum_stream()));
@@ -933,6 +958,7 @@ int pzl_search(void *plan, long long rstep_lo, long long rstep_hi, PzlCand *cand
             if (rc) return rc;
         }
     }
+    tm.active = true;
     if (tm.on) {
         ev_end = tm.get();
         PZL_HIPCHK(hipEventRecord(ev_end, p->sum_stream()));  // the last harmonic sum follows the last correlation
@@ -987,6 +1013,16 @@ int pzl_search(void *plan, long long rstep_lo, long long rstep_hi, PzlCand *cand
         stats->bytes_alg = stats->nrsteps * (planes + seg_bytes) + (double)p->g.bank_bytes;
         stats->bytes_alg_corr = stats->nrsteps * (planes / 2 + seg_bytes) +
                                 (double)p->g.bank_bytes * std::ceil((double)stats->nrsteps / p->B);
+        // sampled batches -> estimates for the whole search (launch counts exact)
+        const double scale = sampled > 0 ? (double)i / (double)sampled : 0.0;
+        stats->ms_segments *= scale;
+        stats->ms_corr *= scale;
+        stats->ms_sum *= scale;
+        stats->ms_graph *= scale;
+        for (auto &kv : per_class) kv.second *= scale;
+        for (auto &kv : per_class_n) kv.second = i;
+        stats->corr_launches = (long long)i * (long long)p->classes.size();
+        stats->graph_launches = (long long)std::llround(stats->graph_launches * scale);
         for (auto &kv : per_class) stats->ms_corr_max_class = std::max(stats->ms_corr_max_class, kv.second);
         stats->nclass = 0;
         for (auto &c : p->classes) {
@@ -1078,8 +1114,8 @@ int pzl_plan_set_overlap(void *plan, int on)
 {
     Plan *p = (Plan *)plan;
     if (!p) return fail(PZL_ERR_ARG, "null plan");
-    PZL_HIPCHK(hipStreamSynchronize(p->st));
-    PZL_HIPCHK(hipStreamSynchronize(p->st3));
+    int rc = sync_all(*p);
+    if (rc) return rc;
     p->overlap_sum = on != 0;
     return 0;
 }

@@ -51,6 +51,7 @@ struct Plan {
     hipStream_t st = nullptr;   // segments and correlations
     hipStream_t st2 = nullptr;  // hit ordering
     hipStream_t st3 = nullptr;  // harmonic sums (when overlapped)
+    hipStream_t st4 = nullptr;  // segments (when overlapped), ahead of the correlation
     bool overlap_sum = true;    // harmonic sums on st3, concurrent with the next batch's correlation
     // resident
     PzlHarm *d_harms = nullptr;
@@ -74,6 +75,7 @@ struct Plan {
     long long plane_stride = 0;
     hipEvent_t ev_corr[kSets] = {};  // the set's planes are written
     hipEvent_t ev_sum[kSets] = {};   // the set's planes have been summed
+    hipEvent_t ev_seg[kSets] = {};   // the set's segments are ready
     // hits: a ring of unordered slots (one per batch in flight), one ordered copy
     static constexpr int kSlots = 4;
     int nslots = 2;
