@@ -277,6 +277,11 @@ def main():
         out["serial_one_stream"] = {"value": cells / tsr, "unit": "cells/s", "ms_per_search": tsr * 1e3,
                                     "same_candidates": cs == cands}
         plan.set_overlap(True)
+    if not args.no_legs and not args.rsteps:
+        # the autotuner's rsteps-per-launch choice, timed on a 64-rstep band
+        bt, ms = B.tune_batchsize(B.make_config(batchsize=args.batchsize, **kw), fft, candidates=(4, 8, 16, 32))
+        out["batchsize_tuning"] = {"best": bt, "ms_per_rstep": ms, "band_rsteps": 64,
+                                   "note": "value above uses the configured batchsize %d" % args.batchsize}
     if not args.no_plain and not args.rsteps:
         plan1 = B.Plan(B.make_config(batchsize=1, **kw))
         plan1.set_spectrum(fft)

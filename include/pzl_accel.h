@@ -137,6 +137,12 @@ int pzl_plan_set_graph(void *plan, int on);
  * with the correlation of the next batch (two plane sets).  On by default;
  * off runs every kernel of the search in one stream. */
 int pzl_plan_set_overlap(void *plan, int on);
+/* Rsteps per launch: for each candidate batch size, plan, autotune and time a
+ * band of band_rsteps rsteps from the middle of the search on this spectrum;
+ * *best gets the fastest, ms_per_rstep (optional, ncand entries) the times.
+ * Results do not depend on the batch size. */
+int pzl_tune_batchsize(const PzlConfig *cfg, int device, const float *host_fft, long long numbins,
+                       const int *candidates, int ncand, long long band_rsteps, int *best, double *ms_per_rstep);
 /* The variant in use for an FFT length of the plan; set one explicitly. */
 int pzl_plan_variant(void *plan, int fftlen, int *variant);
 int pzl_plan_set_variant(void *plan, int fftlen, int variant);

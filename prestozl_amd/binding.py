@@ -127,6 +127,8 @@ def lib():
         L.pzl_plan_variant.argtypes = [C.c_void_p, C.c_int, P(C.c_int)]
         L.pzl_plan_set_graph.argtypes = [C.c_void_p, C.c_int]
         L.pzl_plan_set_overlap.argtypes = [C.c_void_p, C.c_int]
+        L.pzl_tune_batchsize.argtypes = [P(Config), C.c_int, P(C.c_float), C.c_longlong, P(C.c_int), C.c_int,
+                                         C.c_longlong, P(C.c_int), P(C.c_double)]
         L.pzl_plan_set_variant.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.pzl_corr_num_variants.argtypes = [C.c_int]
         L.pzl_corr_variant_name.argtypes = [C.c_int, C.c_int]
@@ -288,6 +290,20 @@ class Plan:
         out = np.zeros(n, dtype=np.float32)
         check(lib().pzl_compute_sums(self.h, rstep, _fptr(out), n))
         return out.reshape(h.numw, h.numz, self.geom.corr_uselen)
+
+
+def tune_batchsize(cfg, fft, candidates=(4, 8, 16), band=64, device=0):
+    """The autotuner's choice of rsteps per launch for this search shape and
+    spectrum: (best, {batchsize: ms per rstep})."""
+    a = np.ascontiguousarray(fft)
+    if a.dtype == np.complex64:
+        a = a.view(np.float32)
+    cand = (C.c_int * len(candidates))(*candidates)
+    ms = (C.c_double * len(candidates))()
+    best = C.c_int(0)
+    check(lib().pzl_tune_batchsize(C.byref(cfg), int(device), _fptr(a), len(a) // 2, cand, len(candidates),
+                                   int(band), C.byref(best), ms))
+    return best.value, {int(c): ms[i] for i, c in enumerate(candidates)}
 
 
 class Server:

@@ -1128,6 +1128,42 @@ int pzl_plan_set_graph(void *plan, int on)
     return 0;
 }
 
+int pzl_tune_batchsize(const PzlConfig *cfg, int device, const float *host_fft, long long numbins,
+                       const int *candidates, int ncand, long long band_rsteps, int *best, double *ms_per_rstep)
+{
+    if (!cfg || !host_fft || !candidates || ncand < 1 || !best) return fail(PZL_ERR_ARG, "bad arguments");
+    double best_ms = 1e300;
+    *best = candidates[0];
+    for (int k = 0; k < ncand; k++) {
+        PzlConfig c = *cfg;
+        c.batchsize = candidates[k];
+        void *plan = nullptr;
+        int rc = pzl_plan_create(&c, device, &plan);
+        if (rc) return rc;
+        Plan *p = (Plan *)plan;
+        rc = pzl_set_spectrum(plan, host_fft, numbins);
+        const long long nr = p->g.nrsteps;
+        const long long band = std::max<long long>(1, std::min<long long>(band_rsteps, nr));
+        const long long lo = std::max<long long>(0, std::min<long long>(nr / 2, nr - band));
+        if (rc == 0) rc = pzl_autotune(plan, 2, nullptr, 0);
+        if (rc == 0) rc = pzl_search(plan, lo, lo + band, nullptr, 0, nullptr, nullptr);  // warm
+        double ms = 0;
+        if (rc == 0) {
+            const auto t0 = std::chrono::steady_clock::now();
+            rc = pzl_search(plan, lo, lo + band, nullptr, 0, nullptr, nullptr);
+            ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / band;
+        }
+        pzl_plan_destroy(plan);
+        if (rc) return rc;
+        if (ms_per_rstep) ms_per_rstep[k] = ms;
+        if (ms < best_ms) {
+            best_ms = ms;
+            *best = candidates[k];
+        }
+    }
+    return 0;
+}
+
 int pzl_plan_variant(void *plan, int fftlen, int *variant)
 {
     Plan *p = (Plan *)plan;

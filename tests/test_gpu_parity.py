@@ -328,3 +328,20 @@ def test_graph_replay_identical(config1_data, which):
     assert c_graph == c_eager and c_again == c_eager
     assert np.array_equal(h_graph[1], h_eager[1])
     assert all(np.array_equal(a, b) for a, b in zip(h_graph[0], h_eager[0]))
+
+
+def test_tune_batchsize(config1_data):
+    """The rsteps-per-launch autotuner returns one of the candidates, and the
+    search it would configure gives the same candidates as batch size 8."""
+    from prestozl_amd import binding as B
+    fft, kw = _c1(config1_data)
+    best, ms = B.tune_batchsize(B.make_config(**kw), fft, candidates=(2, 8, 16), band=48)
+    print("batch-size tuning:", best, ms)
+    assert best in (2, 8, 16) and all(v > 0 for v in ms.values())
+    p8 = _plan(kw, fft)
+    kw2 = dict(kw)
+    kw2["batchsize"] = best
+    pb = _plan(kw2, fft)
+    assert pb.search(stats=False)[0] == p8.search(stats=False)[0]
+    p8.close()
+    pb.close()
