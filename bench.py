@@ -136,6 +136,30 @@ def dm_trials_bench(n, max_inflight):
             "strongest_trial": best}
 
 
+def config3_bench(steps):
+    """The batched form at zmax 200, wmax 500, numharm 16 (4096-point
+    fundamental): N = 2^23 samples at dt = 1e-4 s with five injected jerked
+    sinusoids, sigma 5, 8 rsteps per launch, autotuned kernels."""
+    from prestozl_amd import binding as B
+    from prestozl_amd import synth
+    fft, N, dt, opts, sigs = synth.config3()
+    plan = B.Plan(B.make_config(N=N, dt=dt, numbins=len(fft), zmax=200, wmax=500, numharm=16, sigma=5.0,
+                                wmaxP=True, batchsize=8))
+    plan.set_spectrum(fft)
+    plan.autotune(reps=2)
+    plan.search(stats=False)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        cands, st = plan.search(stats=False)
+    dt_s = (time.perf_counter() - t0) / steps
+    g = plan.geom
+    cells = g.nrsteps * g.corr_uselen * g.numz * max(g.numw, 1)
+    plan.close()
+    return {"workload": "N=2^23, dt=1e-4 s, zmax=200, wmax=500, numharm=16, sigma=5, batchsize=8",
+            "value": cells / dt_s, "unit": "cells/s", "ms_per_search": dt_s * 1e3, "nrsteps": g.nrsteps,
+            "fftlen": g.fftlen, "candidates": len(cands)}
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of 'kernel' at this workload from the committed
     rocprofv3 --pmc measurement (FETCH_SIZE + WRITE_SIZE, copy-calibrated)."""
@@ -295,6 +319,8 @@ def main():
     plan.close()
     if args.serve and not args.rsteps:
         out["serving"] = serve_bench(args.serve, args.serve_rate, args.serve_inflight)
+    if not args.no_legs and not args.rsteps:
+        out["config3"] = config3_bench(1)
     if args.dm_trials and not args.rsteps:
         out["dm_trials"] = dm_trials_bench(args.dm_trials, args.serve_inflight)
     if not args.no_cpu:

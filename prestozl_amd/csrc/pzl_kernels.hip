@@ -986,14 +986,19 @@ static const CorrVariant g_corr_variants[] = {
     PZL_CV(1024, 256, 2, true, 8, 8, 4, 4),
     PZL_CV(1024, 64, 1, true, 16, 16, 4),
     PZL_CVS(1024, 128, 1, true, 8, 8, 8),
+    PZL_CVP(1024, 256, 2, true, 5, 8, 8, 4, 4),
     PZL_CV(1280, 128, 1, true, 16, 16, 5),
     PZL_CV(1280, 256, 2, true, 16, 16, 5),
     PZL_CVP(1280, 128, 1, true, 3, 16, 16, 5),
+    PZL_CVP(1280, 128, 1, true, 5, 16, 16, 5),
     PZL_CV(2048, 128, 1, true, 16, 16, 8),
     PZL_CV(2048, 256, 1, true, 8, 8, 8, 4),
     PZL_CV(2048, 256, 2, false, 16, 16, 8),
     PZL_CVP(2048, 256, 1, true, 3, 8, 8, 8, 4),
     PZL_CVP(2048, 256, 1, true, 5, 8, 8, 8, 4),
+    PZL_CVP(2048, 256, 1, true, 6, 8, 8, 8, 4),
+    PZL_CVP(2048, 256, 1, true, 5, 8, 8, 4, 8),
+    PZL_CVP(2048, 256, 1, true, 5, 4, 8, 8, 8),
     PZL_CVP(2048, 128, 1, true, 3, 16, 16, 8),
     PZL_CVS(2048, 128, 1, true, 16, 16, 4),
     PZL_CVS(2048, 256, 2, false, 16, 16, 4),
