@@ -373,6 +373,7 @@ int launch_sum_part(Plan &p, long long r0, int nb, int set, int slot, float *d_s
     sa.startr = p.d_startr;
     sa.hr_tab = p.d_hr[set];
     sa.int_rlo = p.int_rlo;
+    sa.order = p.sum_order;
     Scope s(tm, 2, 0, p.sum_stream());
     PZL_HIPCHK(pzl_launch_harmsum(&sa, p.sum_stream()));
     return 0;
@@ -1101,6 +1102,37 @@ int pzl_autotune(void *plan, int reps, char *report, int report_len)
         c.variant = best;
         std::lock_guard<std::mutex> lk(g_tune_mu);
         g_tuned[tune_key(*p, c)] = best;
+    }
+    // harmonic-sum workgroup order, on the planes the correlations just wrote
+    {
+        PZL_HIPCHK(hipMemsetAsync(p->d_counts, 0, 2 * sizeof(unsigned long long), p->st));
+        float best_ms = 1e30f;
+        int best = p->sum_order;
+        const bool ov = p->overlap_sum;
+        p->overlap_sum = false;  // on p->st, behind the correlations
+        for (int order = 0; order < 2; order++) {
+            p->sum_order = order;
+            int rc2 = launch_sum_part(*p, r0, nb, 0, 0, nullptr, nullptr);
+            hipError_t e = rc2 ? hipErrorUnknown : hipEventRecord(e0, p->st);
+            for (int k = 0; k < reps && e == hipSuccess; k++)
+                e = launch_sum_part(*p, r0, nb, 0, 0, nullptr, nullptr) ? hipErrorUnknown : hipSuccess;
+            if (e == hipSuccess) e = hipEventRecord(e1, p->st);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            float ms = 0;
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+            if (e != hipSuccess) continue;
+            ms /= reps;
+            snprintf(line, sizeof line, "harmonic sum order %d: %.4f ms\n", order, ms);
+            rep += line;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = order;
+            }
+        }
+        p->sum_order = best;
+        p->overlap_sum = ov;
+        PZL_HIPCHK(hipMemsetAsync(p->d_counts, 0, 2 * sizeof(unsigned long long), p->st));
+        PZL_HIPCHK(hipStreamSynchronize(p->st));
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);

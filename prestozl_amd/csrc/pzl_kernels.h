@@ -79,6 +79,7 @@ struct PzlSumArgs {
     const double *startr;    // fundamental startr of every rstep
     const PzlHrTab *hr_tab;  // [nb][nh] subharmonic rlo of the batch
     int int_rlo;             // startr are integers: r maps computed in integer arithmetic
+    int order;               // workgroup sweep: 0 (r chunk, z block, w fastest), 1 (z block, w, r chunk fastest)
 };
 
 #ifdef __cplusplus

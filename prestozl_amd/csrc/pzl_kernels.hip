@@ -634,8 +634,10 @@ __global__ __launch_bounds__(NT) void k_corr(PzlCorrArgs a)
 // added in the reference's order (stage 1..S, odd harmonics ascending).
 //
 // Workgroup ids are dealt round-robin over the 8 XCDs; each XCD is given a
-// contiguous range of the (rstep, r chunk, z block, w) space, w fastest:
-// neighbouring w rows share subharmonic rows, which then meet in one L2.
+// contiguous range of the (rstep, r chunk, z block, w) space, w fastest
+// (order 0), or of (rstep, z block, w, r chunk), r chunk fastest (order 1):
+// neighbouring w rows share subharmonic rows, neighbouring r chunks the cache
+// lines at their edges, which then meet in one L2.
 template <int NSTAGES>
 __global__ __launch_bounds__(SUM_NT) void k_harmsum(PzlSumArgs a)
 {
@@ -649,12 +651,22 @@ __global__ __launch_bounds__(SUM_NT) void k_harmsum(PzlSumArgs a)
     const long long per_xcd = (nblk + 7) / 8;
     const long long L = (long long)(blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
     if (L >= nblk) return;
-    const int w = (int)(L % a.numw);
-    long long t = L / a.numw;
-    const int zb = (int)(t % zblocks);
-    t /= zblocks;
-    const int chunk = (int)(t % chunks);
-    const int b = (int)(t / chunks);
+    int w, zb, chunk, b;
+    if (a.order == 0) {
+        w = (int)(L % a.numw);
+        long long t = L / a.numw;
+        zb = (int)(t % zblocks);
+        t /= zblocks;
+        chunk = (int)(t % chunks);
+        b = (int)(t / chunks);
+    } else {
+        chunk = (int)(L % chunks);
+        long long t = L / chunks;
+        w = (int)(t % a.numw);
+        t /= a.numw;
+        zb = (int)(t % zblocks);
+        b = (int)(t / zblocks);
+    }
     const int z0 = zb * SUM_ZB;
     const int nz = min(SUM_ZB, a.numz - z0);
     const int r = chunk * SUM_NT + tid;

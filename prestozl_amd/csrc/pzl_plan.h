@@ -103,6 +103,7 @@ struct Plan {
     long long *h_r0 = nullptr;  // pinned staging ring of those values
     unsigned long long r0_ring = 0;
     hipStream_t sum_stream() const { return overlap_sum ? st3 : st; }
+    int sum_order = 0;  // harmonic-sum workgroup order (autotuned)
     // results of the last search
     std::vector<PzlHostHit> last_hits;
     std::vector<PzlCand> last_cands;
