@@ -139,25 +139,40 @@ def dm_trials_bench(n, max_inflight):
 def config3_bench(steps):
     """The batched form at zmax 200, wmax 500, numharm 16 (4096-point
     fundamental): N = 2^23 samples at dt = 1e-4 s with five injected jerked
-    sinusoids, sigma 5, 8 rsteps per launch, autotuned kernels."""
+    sinusoids, sigma 5.  Rsteps per launch (8 to 64) from the batch-size tuner,
+    kernel variants from the autotuner, full batches replayed from hipGraphs;
+    the same search at 8 rsteps per launch beside it."""
     from prestozl_amd import binding as B
     from prestozl_amd import synth
     fft, N, dt, opts, sigs = synth.config3()
-    plan = B.Plan(B.make_config(N=N, dt=dt, numbins=len(fft), zmax=200, wmax=500, numharm=16, sigma=5.0,
-                                wmaxP=True, batchsize=8))
-    plan.set_spectrum(fft)
-    plan.autotune(reps=2)
-    plan.search(stats=False)
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        cands, st = plan.search(stats=False)
-    dt_s = (time.perf_counter() - t0) / steps
-    g = plan.geom
-    cells = g.nrsteps * g.corr_uselen * g.numz * max(g.numw, 1)
-    plan.close()
-    return {"workload": "N=2^23, dt=1e-4 s, zmax=200, wmax=500, numharm=16, sigma=5, batchsize=8",
-            "value": cells / dt_s, "unit": "cells/s", "ms_per_search": dt_s * 1e3, "nrsteps": g.nrsteps,
-            "fftlen": g.fftlen, "candidates": len(cands)}
+    kw = dict(N=N, dt=dt, numbins=len(fft), zmax=200, wmax=500, numharm=16, sigma=5.0, wmaxP=True)
+    bt, ms = B.tune_batchsize(B.make_config(batchsize=8, **kw), fft, candidates=(8, 16, 32, 64), band=128)
+    res = {}
+    variants = None
+    for bs, graph in ((bt, True), (8, False)):
+        plan = B.Plan(B.make_config(batchsize=bs, **kw))
+        plan.set_spectrum(fft)
+        if variants is None:
+            plan.autotune(reps=2)
+            variants = {f: plan.variant(f) for f in plan.fftlens()}
+        else:  # the tuned plan's kernel variants: the same arithmetic, so the same candidates
+            for f, v in variants.items():
+                plan.set_variant(f, v)
+        plan.set_graph(graph)
+        plan.search(stats=False)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            cands, st = plan.search(stats=False)
+        res[bs, graph] = ((time.perf_counter() - t0) / steps, cands)
+        g = plan.geom
+        cells = g.nrsteps * g.corr_uselen * g.numz * max(g.numw, 1)
+        plan.close()
+    t8, c8 = res[8, False]
+    tb, cb = res[bt, True]
+    return {"workload": "N=2^23, dt=1e-4 s, zmax=200, wmax=500, numharm=16, sigma=5",
+            "value": cells / tb, "unit": "cells/s", "ms_per_search": tb * 1e3, "batchsize": bt, "graph": True,
+            "batchsize_ms_per_rstep": ms, "at_batchsize_8": {"value": cells / t8, "ms_per_search": t8 * 1e3},
+            "same_candidates": cb == c8, "nrsteps": g.nrsteps, "fftlen": g.fftlen, "candidates": len(cb)}
 
 
 def measured_traffic(kernel):
@@ -300,6 +315,14 @@ def main():
         tsr = time.perf_counter() - ts
         out["serial_one_stream"] = {"value": cells / tsr, "unit": "cells/s", "ms_per_search": tsr * 1e3,
                                     "same_candidates": cs == cands}
+        plan.set_overlap(1)
+        ts = time.perf_counter()
+        co, _ = plan.search(lo, hi, stats=False)
+        tso = time.perf_counter() - ts
+        out["overlap_sum_only"] = {"value": cells / tso, "unit": "cells/s", "ms_per_search": tso * 1e3,
+                                   "same_candidates": co == cands,
+                                   "note": "harmonic sums and segments on their own streams, every FFT length "
+                                           "correlated on the main stream"}
         plan.set_overlap(True)
     if not args.no_legs and not args.rsteps:
         # the autotuner's rsteps-per-launch choice, timed on a 64-rstep band

@@ -134,8 +134,10 @@ int pzl_autotune(void *plan, int reps, char *report, int report_len);
  * reported per batch (ms_graph) instead of per kernel. */
 int pzl_plan_set_graph(void *plan, int on);
 /* Overlap: the harmonic sum of a batch runs on its own stream, concurrently
- * with the correlation of the next batch (two plane sets).  On by default;
- * off runs every kernel of the search in one stream. */
+ * with the correlation of the next batch (two plane sets), and the segments of
+ * the next batch on a third.  on = 1: that; on = 2 (the default): also the
+ * correlations of the shorter FFT lengths on the segments' stream, beside the
+ * longest length's; on = 0: every kernel of the search in one stream. */
 int pzl_plan_set_overlap(void *plan, int on);
 /* Rsteps per launch: for each candidate batch size, plan, autotune and time a
  * band of band_rsteps rsteps from the middle of the search on this spectrum;
@@ -143,6 +145,12 @@ int pzl_plan_set_overlap(void *plan, int on);
  * Results do not depend on the batch size. */
 int pzl_tune_batchsize(const PzlConfig *cfg, int device, const float *host_fft, long long numbins,
                        const int *candidates, int ncand, long long band_rsteps, int *best, double *ms_per_rstep);
+/* The harmonic-sum form in use (chosen by pzl_autotune): workgroup order
+ * (0: r chunk, z block, w fastest; 1: z block, w, r chunk fastest) and
+ * consecutive fundamental r per thread (1, 2 or 4).  Results do not depend on
+ * it. */
+int pzl_plan_sum_form(void *plan, int *order, int *rpt);
+int pzl_plan_set_sum_form(void *plan, int order, int rpt);
 /* The variant in use for an FFT length of the plan; set one explicitly. */
 int pzl_plan_variant(void *plan, int fftlen, int *variant);
 int pzl_plan_set_variant(void *plan, int fftlen, int variant);

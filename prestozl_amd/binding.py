@@ -130,6 +130,8 @@ def lib():
         L.pzl_tune_batchsize.argtypes = [P(Config), C.c_int, P(C.c_float), C.c_longlong, P(C.c_int), C.c_int,
                                          C.c_longlong, P(C.c_int), P(C.c_double)]
         L.pzl_plan_set_variant.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.pzl_plan_sum_form.argtypes = [C.c_void_p, P(C.c_int), P(C.c_int)]
+        L.pzl_plan_set_sum_form.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.pzl_corr_num_variants.argtypes = [C.c_int]
         L.pzl_corr_variant_name.argtypes = [C.c_int, C.c_int]
         L.pzl_corr_variant_name.restype = C.c_char_p
@@ -223,8 +225,25 @@ class Plan:
     def set_graph(self, on=True):
         check(lib().pzl_plan_set_graph(self.h, int(bool(on))))
 
-    def set_overlap(self, on=True):
-        check(lib().pzl_plan_set_overlap(self.h, int(bool(on))))
+    def set_overlap(self, on=2):
+        """0 / False: one stream; 1: harmonic sums and segments on their own
+        streams; 2 / True (the default): also the shorter FFT lengths'
+        correlations beside the longest's."""
+        mode = (2 if on else 0) if isinstance(on, bool) else int(on)
+        check(lib().pzl_plan_set_overlap(self.h, mode))
+
+    def sum_form(self):
+        """(workgroup order, r per thread) of the harmonic sum."""
+        o, r = C.c_int(0), C.c_int(0)
+        check(lib().pzl_plan_sum_form(self.h, C.byref(o), C.byref(r)))
+        return o.value, r.value
+
+    def set_sum_form(self, order, rpt):
+        check(lib().pzl_plan_set_sum_form(self.h, int(order), int(rpt)))
+
+    def fftlens(self):
+        """The FFT lengths of the plan's harmonics (one kernel variant each)."""
+        return sorted({h.fftlen for h in self.harms})
 
     def variant(self, fftlen):
         v = C.c_int(0)

@@ -163,7 +163,8 @@ long long plan_device_bytes(const PzlGeometry &g, const PlanSizes &z, int nh, lo
 {
     if (sort_temp < 0) sort_temp = 12 * z.hitcap + (16LL << 20);
     return g.bank_bytes + 8 * g.numbins +
-           Plan::kSets * (g.plane_bytes_per_rstep * z.B + (long long)sizeof(float2) * z.seg_stride * z.B +
+           Plan::kSets * (g.plane_bytes_per_rstep * z.B + (long long)sizeof(float) * PZL_PLANE_SLACK +
+                          (long long)sizeof(float2) * z.seg_stride * z.B +
                           (long long)sizeof(uint16_t) * z.B * nh * g.corr_uselen) +
            (long long)(z.nslots * 12 + 12 + (long long)sizeof(PzlHostHit)) * z.hitcap + sort_temp;
 }
@@ -210,6 +211,7 @@ void free_plan(Plan *p)
         if (p->ev_corr[k]) (void)hipEventDestroy(p->ev_corr[k]);
         if (p->ev_sum[k]) (void)hipEventDestroy(p->ev_sum[k]);
         if (p->ev_seg[k]) (void)hipEventDestroy(p->ev_seg[k]);
+        if (p->ev_side[k]) (void)hipEventDestroy(p->ev_side[k]);
     }
     for (int s = 0; s < Plan::kSlots; s++) {
         (void)hipFree(p->d_hkeys[s]);
@@ -337,10 +339,23 @@ int launch_corr_part(Plan &p, long long r0, int nb, int set, Timers *tm, const l
         PZL_HIPCHK(hipEventRecord(p.ev_seg[set], ss));
         PZL_HIPCHK(hipStreamWaitEvent(p.st, p.ev_seg[set], 0));
     }
-    for (auto &c : p.classes) {
+    // With the segments on their own stream, the shorter FFT lengths (small
+    // grids that alone leave most of the chip idle) can follow them there and
+    // run beside the longest length's correlation on the main stream.
+    const bool side = seg_stream && p.side_classes && p.classes.size() > 1;
+    size_t main_k = 0;
+    for (size_t k = 1; k < p.classes.size(); k++)
+        if (p.classes[k].fftlen > p.classes[main_k].fftlen) main_k = k;
+    for (size_t k = 0; k < p.classes.size(); k++) {
+        const SizeClass &c = p.classes[k];
+        hipStream_t cs = (side && k != main_k) ? ss : p.st;
         const PzlCorrArgs ca = corr_args(p, c, nb, set);
-        Scope s(tm, 1, c.fftlen, p.st);
-        PZL_HIPCHK(pzl_launch_corr(c.fftlen, c.variant, &ca, c.nrows.data(), p.st));
+        Scope s(tm, 1, c.fftlen, cs);
+        PZL_HIPCHK(pzl_launch_corr(c.fftlen, c.variant, &ca, c.nrows.data(), cs));
+    }
+    if (side) {
+        PZL_HIPCHK(hipEventRecord(p.ev_side[set], ss));
+        PZL_HIPCHK(hipStreamWaitEvent(p.st, p.ev_side[set], 0));
     }
     return 0;
 }
@@ -374,6 +389,7 @@ int launch_sum_part(Plan &p, long long r0, int nb, int set, int slot, float *d_s
     sa.hr_tab = p.d_hr[set];
     sa.int_rlo = p.int_rlo;
     sa.order = p.sum_order;
+    sa.rpt = p.sum_rpt;
     Scope s(tm, 2, 0, p.sum_stream());
     PZL_HIPCHK(pzl_launch_harmsum(&sa, p.sum_stream()));
     return 0;
@@ -807,11 +823,13 @@ int pzl_plan_create(const PzlConfig *cfg, int device, void **plan_out)
         PCHK(hipMalloc(&p->d_segs[k], sizeof(float2) * p->seg_stride * p->B));
         PCHK(hipMalloc(&p->d_rinds[k], sizeof(uint16_t) * (size_t)p->B * nh * p->g.corr_uselen));
         PCHK(hipMalloc(&p->d_hr[k], sizeof(PzlHrTab) * (size_t)p->B * nh));
-        PCHK(hipMalloc(&p->d_planes[k], sizeof(float) * (size_t)p->plane_stride * p->B));
+        PCHK(hipMalloc(&p->d_planes[k], sizeof(float) * ((size_t)p->plane_stride * p->B + PZL_PLANE_SLACK)));
         PCHK(hipEventCreateWithFlags(&p->ev_corr[k], hipEventDisableTiming));
         PCHK(hipEventCreateWithFlags(&p->ev_sum[k], hipEventDisableTiming));
         PCHK(hipEventCreateWithFlags(&p->ev_seg[k], hipEventDisableTiming));
         PCHK(hipEventRecord(p->ev_seg[k], p->st));
+        PCHK(hipEventCreateWithFlags(&p->ev_side[k], hipEventDisableTiming));
+        PCHK(hipEventRecord(p->ev_side[k], p->st));
         // recorded once so that the first waits on them are satisfied
         PCHK(hipEventRecord(p->ev_corr[k], p->st));
         PCHK(hipEventRecord(p->ev_sum[k], p->st));
@@ -1107,11 +1125,13 @@ int pzl_autotune(void *plan, int reps, char *report, int report_len)
     {
         PZL_HIPCHK(hipMemsetAsync(p->d_counts, 0, 2 * sizeof(unsigned long long), p->st));
         float best_ms = 1e30f;
-        int best = p->sum_order;
+        int best = p->sum_order, best_rpt = p->sum_rpt;
         const bool ov = p->overlap_sum;
         p->overlap_sum = false;  // on p->st, behind the correlations
-        for (int order = 0; order < 2; order++) {
+        for (int form = 0; form < 6; form++) {
+            const int order = form & 1, rpt = 1 << (form >> 1);
             p->sum_order = order;
+            p->sum_rpt = rpt;
             int rc2 = launch_sum_part(*p, r0, nb, 0, 0, nullptr, nullptr);
             hipError_t e = rc2 ? hipErrorUnknown : hipEventRecord(e0, p->st);
             for (int k = 0; k < reps && e == hipSuccess; k++)
@@ -1122,14 +1142,16 @@ int pzl_autotune(void *plan, int reps, char *report, int report_len)
             if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
             if (e != hipSuccess) continue;
             ms /= reps;
-            snprintf(line, sizeof line, "harmonic sum order %d: %.4f ms\n", order, ms);
+            snprintf(line, sizeof line, "harmonic sum order %d, %d r per thread: %.4f ms\n", order, rpt, ms);
             rep += line;
             if (ms < best_ms) {
                 best_ms = ms;
                 best = order;
+                best_rpt = rpt;
             }
         }
         p->sum_order = best;
+        p->sum_rpt = best_rpt;
         p->overlap_sum = ov;
         PZL_HIPCHK(hipMemsetAsync(p->d_counts, 0, 2 * sizeof(unsigned long long), p->st));
         PZL_HIPCHK(hipStreamSynchronize(p->st));
@@ -1149,6 +1171,7 @@ int pzl_plan_set_overlap(void *plan, int on)
     int rc = sync_all(*p);
     if (rc) return rc;
     p->overlap_sum = on != 0;
+    p->side_classes = on >= 2;
     return 0;
 }
 
@@ -1193,6 +1216,28 @@ int pzl_tune_batchsize(const PzlConfig *cfg, int device, const float *host_fft, 
             *best = candidates[k];
         }
     }
+    return 0;
+}
+
+int pzl_plan_sum_form(void *plan, int *order, int *rpt)
+{
+    Plan *p = (Plan *)plan;
+    if (!p) return fail(PZL_ERR_ARG, "null plan");
+    if (order) *order = p->sum_order;
+    if (rpt) *rpt = p->sum_rpt;
+    return 0;
+}
+
+int pzl_plan_set_sum_form(void *plan, int order, int rpt)
+{
+    Plan *p = (Plan *)plan;
+    if (!p) return fail(PZL_ERR_ARG, "null plan");
+    if ((order != 0 && order != 1) || (rpt != 1 && rpt != 2 && rpt != 4))
+        return fail(PZL_ERR_ARG, "harmonic-sum form: order 0 or 1, 1, 2 or 4 r per thread");
+    int rc = sync_all(*p);
+    if (rc) return rc;
+    p->sum_order = order;
+    p->sum_rpt = rpt;
     return 0;
 }
 

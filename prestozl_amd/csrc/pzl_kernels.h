@@ -80,7 +80,12 @@ struct PzlSumArgs {
     const PzlHrTab *hr_tab;  // [nb][nh] subharmonic rlo of the batch
     int int_rlo;             // startr are integers: r maps computed in integer arithmetic
     int order;               // workgroup sweep: 0 (r chunk, z block, w fastest), 1 (z block, w, r chunk fastest)
+    int rpt;                 // consecutive fundamental r per thread: 1, 2 or 4
 };
+
+// Floats allocated behind the last plane set of a batch buffer, so that the
+// harmonic sum's multi-float loads never reach past the buffer range.
+#define PZL_PLANE_SLACK 1024
 
 #ifdef __cplusplus
 extern "C" {

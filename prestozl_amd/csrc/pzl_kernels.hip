@@ -619,33 +619,63 @@ __global__ __launch_bounds__(NT) void k_corr(PzlCorrArgs a)
 // Harmonic sum, threshold, compaction
 // ---------------------------------------------------------------------------
 
-#define SUM_NT 256
-
 #define SUM_ZB 32  // fundamental z rows per workgroup
 
-// One workgroup: SUM_NT consecutive r of SUM_ZB consecutive z rows of one
-// (rstep, w) of the fundamental.  The subharmonic r indices depend only on
-// (rstep, harmonic, r), so each thread computes them once (integer
-// round-half-even of m j / 2^s split into a uniform 64-bit part and a small
-// per-thread 32-bit part) and keeps them in registers as byte offsets; the
-// row offsets of the subharmonic z/w maps for the workgroup's rows are staged
-// in LDS and used as the scalar offset of buffer loads over the rstep's
-// planes.  For every cell all subharmonic powers are loaded first and then
-// added in the reference's order (stage 1..S, odd harmonics ascending).
+// Threads per workgroup and fundamental r per workgroup row chunk, for RPT
+// consecutive r per thread.
+template <int RPT> struct SumShape {
+    static constexpr int NT = RPT == 4 ? 128 : 256;
+    static constexpr int RCH = NT * RPT;
+};
+
+// RPT consecutive floats at byte offset voff + soff of a buffer, one load.
+template <int RPT>
+__device__ __forceinline__ void load_run(__amdgpu_buffer_rsrc_t rs, int voff, int soff, float (&x)[RPT])
+{
+    if constexpr (RPT == 1) {
+        x[0] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, voff, soff, 0));
+    } else if constexpr (RPT == 2) {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b64(rs, voff, soff, 0);
+        x[0] = __uint_as_float(v[0]);
+        x[1] = __uint_as_float(v[1]);
+    } else {
+        static_assert(RPT == 4, "1, 2 or 4 r per thread");
+        const auto v = __builtin_amdgcn_raw_buffer_load_b128(rs, voff, soff, 0);
+        x[0] = __uint_as_float(v[0]);
+        x[1] = __uint_as_float(v[1]);
+        x[2] = __uint_as_float(v[2]);
+        x[3] = __uint_as_float(v[3]);
+    }
+}
+
+// One workgroup: RCH consecutive r of SUM_ZB consecutive z rows of one
+// (rstep, w) of the fundamental, RPT consecutive r per thread.  The
+// subharmonic r indices depend only on (rstep, harmonic, r), so each thread
+// computes them once (integer round-half-even of m j / 2^s split into a uniform
+// 64-bit part and a small per-thread 32-bit part).  A subharmonic's ratio j/2^s
+// is below 1, so the RPT indices of a thread's r are non-decreasing with steps
+// of 0 or 1: they lie in the RPT floats from the first one, which one load
+// fetches; the thread keeps the first index as a byte offset and the others as
+// 2-bit distances from it.  The row offsets of the subharmonic z/w maps for the
+// workgroup's rows are staged in LDS and used as the scalar offset of buffer
+// loads over the rstep's planes.  For every cell all subharmonic powers are
+// loaded first and then added in the reference's order (stage 1..S, odd
+// harmonics ascending).
 //
 // Workgroup ids are dealt round-robin over the 8 XCDs; each XCD is given a
 // contiguous range of the (rstep, r chunk, z block, w) space, w fastest
 // (order 0), or of (rstep, z block, w, r chunk), r chunk fastest (order 1):
 // neighbouring w rows share subharmonic rows, neighbouring r chunks the cache
 // lines at their edges, which then meet in one L2.
-template <int NSTAGES>
-__global__ __launch_bounds__(SUM_NT) void k_harmsum(PzlSumArgs a)
+template <int NSTAGES, int RPT>
+__global__ __launch_bounds__(SumShape<RPT>::NT) void k_harmsum(PzlSumArgs a)
 {
+    constexpr int NT = SumShape<RPT>::NT, RCH = SumShape<RPT>::RCH;
     constexpr int NSUB = (1 << (NSTAGES - 1)) - 1;
     constexpr int NQ = NSUB > 0 ? NSUB : 1;
     __shared__ int zoff[NQ][SUM_ZB];
     const int tid = threadIdx.x;
-    const int chunks = (a.corr_uselen + SUM_NT - 1) / SUM_NT;
+    const int chunks = (a.corr_uselen + RCH - 1) / RCH;
     const int zblocks = (a.numz + SUM_ZB - 1) / SUM_ZB;
     const long long nblk = (long long)a.nb * a.numw * zblocks * chunks;
     const long long per_xcd = (nblk + 7) / 8;
@@ -669,17 +699,23 @@ __global__ __launch_bounds__(SUM_NT) void k_harmsum(PzlSumArgs a)
     }
     const int z0 = zb * SUM_ZB;
     const int nz = min(SUM_ZB, a.numz - z0);
-    const int r = chunk * SUM_NT + tid;
-    const bool active = r < a.corr_uselen;
-    const int rr = active ? r : a.corr_uselen - 1;
+    const int rbase = chunk * RCH + tid * RPT;
+    bool active[RPT];
+    int rr[RPT];
+#pragma unroll
+    for (int i = 0; i < RPT; i++) {
+        active[i] = rbase + i < a.corr_uselen;
+        rr[i] = min(rbase + i, a.corr_uselen - 1);
+    }
     const long long rstep = (a.rstep0_dev ? *a.rstep0_dev : a.rstep0) + b;
     const PzlHarm F = a.harms[0];
-    // every plane of this rstep, as one buffer (byte offsets < 2^31)
-    const __amdgpu_buffer_rsrc_t planes =
-        uniform_rsrc(a.planes + (long long)b * a.plane_stride, (int)(a.plane_stride * 4));
+    // every plane of this rstep (and the slack behind the set's planes), as
+    // one buffer (byte offsets < 2^31)
+    const __amdgpu_buffer_rsrc_t planes = uniform_rsrc(a.planes + (long long)b * a.plane_stride,
+                                                       (int)(4 * (a.plane_stride + PZL_PLANE_SLACK)));
 
     if constexpr (NSUB > 0) {
-        for (int k = tid; k < NSUB * SUM_ZB; k += SUM_NT) {
+        for (int k = tid; k < NSUB * SUM_ZB; k += NT) {
             const int q = k / SUM_ZB, zi = k % SUM_ZB;
             const int h = q + 1;
             const PzlHarm &H = a.harms[h];
@@ -688,31 +724,41 @@ __global__ __launch_bounds__(SUM_NT) void k_harmsum(PzlSumArgs a)
                                                     a.zinds[h * a.numz + z]) * H.rs_stride));
         }
     }
-    int rind[NQ];  // byte offsets within a subharmonic row
+    int rind[NQ];      // byte offset of the first subharmonic index within its row
+    unsigned dsel[NQ];  // 2-bit distances of the others from it
     if constexpr (NSUB > 0) {
-        if (a.int_rlo) {
-            const long long m0 = (long long)(2.0 * a.startr[rstep]);
+        const long long m0 = a.int_rlo ? (long long)(2.0 * a.startr[rstep]) : 0;
 #pragma unroll
-            for (int q = 0; q < NSUB; q++) {
-                const int h = q + 1;
-                // rint((m0 + rr) j / 2^s) - 2 rlo_h with m0 j split as q0 2^s + rem0
+        for (int q = 0; q < NSUB; q++) {
+            const int h = q + 1;
+            int sidx[RPT];
+            if (a.int_rlo) {
+                // rint((m0 + r) j / 2^s) - 2 rlo_h with m0 j split as q0 2^s + rem0
                 const int j = a.harms[h].harmnum, sh = a.harms[h].stage;
                 const long long p0 = m0 * j;
                 const long long q0 = p0 >> sh;
                 const int rem0 = (int)(p0 - (q0 << sh));
                 const int base_h = (int)(q0 - 2 * a.hr_tab[b * a.nh + h].rlo);
-                const int tt = rem0 + rr * j;
-                int qq = tt >> sh;
-                const int rem = tt - (qq << sh);
                 const int half = (1 << sh) >> 1;  // sh >= 1 for every subharmonic
-                // round half to even on the full quotient q0 + qq
-                if (rem > half || (rem == half && ((q0 + qq) & 1))) qq++;
-                rind[q] = 4 * (base_h + qq);
-            }
-        } else {
 #pragma unroll
-            for (int q = 0; q < NSUB; q++)
-                rind[q] = 4 * a.rinds[((long long)b * a.nh + q + 1) * a.corr_uselen + rr];
+                for (int i = 0; i < RPT; i++) {
+                    const int tt = rem0 + rr[i] * j;
+                    int qq = tt >> sh;
+                    const int rem = tt - (qq << sh);
+                    // round half to even on the full quotient q0 + qq
+                    if (rem > half || (rem == half && ((q0 + qq) & 1))) qq++;
+                    sidx[i] = base_h + qq;
+                }
+            } else {
+#pragma unroll
+                for (int i = 0; i < RPT; i++)
+                    sidx[i] = a.rinds[((long long)b * a.nh + h) * a.corr_uselen + rr[i]];
+            }
+            rind[q] = 4 * sidx[0];
+            unsigned d = 0;
+#pragma unroll
+            for (int i = 1; i < RPT; i++) d |= (unsigned)min(max(sidx[i] - sidx[0], 0), RPT - 1) << (2 * i);
+            dsel[q] = d;
         }
     }
     __syncthreads();
@@ -721,42 +767,64 @@ __global__ __launch_bounds__(SUM_NT) void k_harmsum(PzlSumArgs a)
     const int frow = (int)(4 * (F.plane_off + ((long long)w * a.numz + z0) * F.rs_stride));
     for (int zi = 0; zi < nz; zi++) {
         const int z = z0 + zi;
-        float acc = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
-            planes, 4 * rr, __builtin_amdgcn_readfirstlane(frow + zi * 4 * (int)F.rs_stride), 0));
-        float v[NQ];
+        float acc[RPT];
+        load_run<RPT>(planes, 4 * rbase, __builtin_amdgcn_readfirstlane(frow + zi * 4 * (int)F.rs_stride), acc);
+        float x[NQ][RPT];
         if constexpr (NSUB > 0) {
 #pragma unroll
             for (int q = 0; q < NSUB; q++)
-                v[q] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
-                    planes, rind[q], __builtin_amdgcn_readfirstlane(zoff[q][zi]), 0));
+                load_run<RPT>(planes, rind[q], __builtin_amdgcn_readfirstlane(zoff[q][zi]), x[q]);
         }
         int q = 0;
 #pragma unroll
         for (int s = 0; s < NSTAGES; s++) {
             if (s > 0) {
 #pragma unroll
-                for (int k = 0; k < (1 << (s - 1)); k++, q++) acc += v[q];
-            }
-            const bool hit = active && (acc > a.powcut[s]);
-            const unsigned long long mask = __ballot(hit);
-            if (mask) {
-                const int cnt = __popcll(mask);
-                const int leader = __ffsll((long long)mask) - 1;
-                unsigned long long slot0 = 0;
-                if ((int)__lane_id() == leader) slot0 = atomicAdd(a.nhits, (unsigned long long)cnt);
-                slot0 = __shfl(slot0, leader);
-                if (hit) {
-                    const unsigned long long slot = slot0 + __popcll(mask & lanemask_lt);
-                    if (slot < a.maxhits) {
-                        a.hit_keys[slot] =
-                            ((((rstep * NSTAGES + s) * a.numw + w) * (long long)a.numz + z) * a.corr_uselen + r);
-                        a.hit_pows[slot] = acc;
+                for (int k = 0; k < (1 << (s - 1)); k++, q++) {
+#pragma unroll
+                    for (int i = 0; i < RPT; i++) {
+                        float v = x[q][0];
+                        const int d = (int)((dsel[q] >> (2 * i)) & 3u);
+#pragma unroll
+                        for (int e = 1; e <= i; e++) v = d >= e ? x[q][e] : v;
+                        acc[i] += v;
                     }
                 }
             }
+            bool hit[RPT];
+            unsigned long long mask[RPT];
+            int cnt = 0;
+#pragma unroll
+            for (int i = 0; i < RPT; i++) {
+                hit[i] = active[i] && (acc[i] > a.powcut[s]);
+                mask[i] = __ballot(hit[i]);
+                cnt += __popcll(mask[i]);
+            }
+            if (cnt) {
+                unsigned long long slot0 = 0;
+                if (__lane_id() == 0) slot0 = atomicAdd(a.nhits, (unsigned long long)cnt);
+                slot0 = __shfl(slot0, 0);
+#pragma unroll
+                for (int i = 0; i < RPT; i++) {
+                    if (hit[i]) {
+                        const unsigned long long slot = slot0 + __popcll(mask[i] & lanemask_lt);
+                        if (slot < a.maxhits) {
+                            a.hit_keys[slot] = ((((rstep * NSTAGES + s) * a.numw + w) * (long long)a.numz + z) *
+                                                    a.corr_uselen +
+                                                rbase + i);
+                            a.hit_pows[slot] = acc[i];
+                        }
+                    }
+                    slot0 += __popcll(mask[i]);
+                }
+            }
         }
-        if (a.sums_out && active)
-            a.sums_out[(((long long)b * a.numw + w) * a.numz + z) * a.corr_uselen + r] = acc;
+        if (a.sums_out) {
+#pragma unroll
+            for (int i = 0; i < RPT; i++)
+                if (active[i])
+                    a.sums_out[(((long long)b * a.numw + w) * a.numz + z) * a.corr_uselen + rbase + i] = acc[i];
+        }
     }
 }
 
@@ -1054,20 +1122,32 @@ extern "C" hipError_t pzl_launch_corr(int N, int variant, const PzlCorrArgs *a, 
     return cv->fn(a, class_nrows, st);
 }
 
-extern "C" hipError_t pzl_launch_harmsum(const PzlSumArgs *a, hipStream_t st)
+template <int RPT> static hipError_t launch_harmsum_t(const PzlSumArgs *a, hipStream_t st)
 {
-    const int chunks = (a->corr_uselen + SUM_NT - 1) / SUM_NT;
+    using SS = SumShape<RPT>;
+    const int chunks = (a->corr_uselen + SS::RCH - 1) / SS::RCH;
     const long long nblk = (long long)a->nb * a->numw * ((a->numz + SUM_ZB - 1) / SUM_ZB) * chunks;
-    const dim3 g((unsigned)(8 * ((nblk + 7) / 8))), t(SUM_NT);
-    if (a->plane_stride * 4 >= (1LL << 31)) return hipErrorInvalidValue;  // buffer offsets are 32-bit
+    const dim3 g((unsigned)(8 * ((nblk + 7) / 8))), t(SS::NT);
     switch (a->nstages) {
-    case 1: hipLaunchKernelGGL(k_harmsum<1>, g, t, 0, st, *a); break;
-    case 2: hipLaunchKernelGGL(k_harmsum<2>, g, t, 0, st, *a); break;
-    case 3: hipLaunchKernelGGL(k_harmsum<3>, g, t, 0, st, *a); break;
-    case 4: hipLaunchKernelGGL(k_harmsum<4>, g, t, 0, st, *a); break;
-    case 5: hipLaunchKernelGGL(k_harmsum<5>, g, t, 0, st, *a); break;
-    case 6: hipLaunchKernelGGL(k_harmsum<6>, g, t, 0, st, *a); break;
+    case 1: hipLaunchKernelGGL((k_harmsum<1, RPT>), g, t, 0, st, *a); break;
+    case 2: hipLaunchKernelGGL((k_harmsum<2, RPT>), g, t, 0, st, *a); break;
+    case 3: hipLaunchKernelGGL((k_harmsum<3, RPT>), g, t, 0, st, *a); break;
+    case 4: hipLaunchKernelGGL((k_harmsum<4, RPT>), g, t, 0, st, *a); break;
+    case 5: hipLaunchKernelGGL((k_harmsum<5, RPT>), g, t, 0, st, *a); break;
+    case 6: hipLaunchKernelGGL((k_harmsum<6, RPT>), g, t, 0, st, *a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+extern "C" hipError_t pzl_launch_harmsum(const PzlSumArgs *a, hipStream_t st)
+{
+    // buffer offsets are 32-bit; the planes of a set carry PZL_PLANE_SLACK floats behind them
+    if ((a->plane_stride + PZL_PLANE_SLACK) * 4 >= (1LL << 31)) return hipErrorInvalidValue;
+    switch (a->rpt) {
+    case 1: return launch_harmsum_t<1>(a, st);
+    case 2: return launch_harmsum_t<2>(a, st);
+    case 4: return launch_harmsum_t<4>(a, st);
+    default: return hipErrorInvalidValue;
+    }
 }

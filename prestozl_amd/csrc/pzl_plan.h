@@ -53,6 +53,7 @@ struct Plan {
     hipStream_t st3 = nullptr;  // harmonic sums (when overlapped)
     hipStream_t st4 = nullptr;  // segments (when overlapped), ahead of the correlation
     bool overlap_sum = true;    // harmonic sums on st3, concurrent with the next batch's correlation
+    bool side_classes = true;   // with overlap_sum: shorter FFT lengths correlated on st4, beside the longest
     // resident
     PzlHarm *d_harms = nullptr;
     float2 *d_bank = nullptr;  // shared, owned by the bank cache
@@ -76,6 +77,7 @@ struct Plan {
     hipEvent_t ev_corr[kSets] = {};  // the set's planes are written
     hipEvent_t ev_sum[kSets] = {};   // the set's planes have been summed
     hipEvent_t ev_seg[kSets] = {};   // the set's segments are ready
+    hipEvent_t ev_side[kSets] = {};  // the set's shorter FFT lengths are correlated (st4)
     // hits: a ring of unordered slots (one per batch in flight), one ordered copy
     static constexpr int kSlots = 4;
     int nslots = 2;
@@ -104,6 +106,7 @@ struct Plan {
     unsigned long long r0_ring = 0;
     hipStream_t sum_stream() const { return overlap_sum ? st3 : st; }
     int sum_order = 0;  // harmonic-sum workgroup order (autotuned)
+    int sum_rpt = 1;    // harmonic-sum fundamental r per thread (autotuned)
     // results of the last search
     std::vector<PzlHostHit> last_hits;
     std::vector<PzlCand> last_cands;

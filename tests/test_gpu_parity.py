@@ -259,6 +259,44 @@ def test_batch_size_invariance(config1_data):
     assert res[0] == res[1] == res[2]
 
 
+@pytest.mark.parametrize("which", ["c1", "c2", "h32"])
+def test_harmsum_forms_identical(config1_data, which):
+    """Every harmonic-sum form (workgroup order x 1, 2 or 4 consecutive r per
+    thread, with its wide loads and per-lane selection) gives the same hits,
+    summed powers and candidates, bit for bit, as the one-r-per-thread form;
+    that form is compared with the oracle by the tests above."""
+    if which == "h32":
+        fft, kw = _variant_shapes(config1_data)[0]  # six stages (numharm 32)
+        lo, hi, rs = 0, 40, 20
+    elif which == "c1":
+        fft, kw = _c1(config1_data)
+        lo, hi, rs = 0, -1, 100
+    else:
+        fft, kw = _c2_small()
+        kw["sigma"] = 3.0
+        lo, hi, rs = 58, 66, 61
+    plan = _plan(kw, fft)
+    ref = None
+    for order in (0, 1):
+        for rpt in (1, 2, 4):
+            plan.set_sum_form(order, rpt)
+            assert plan.sum_form() == (order, rpt)
+            c, _ = plan.search(lo, hi)
+            h, p = plan.hits()
+            idx = np.lexsort(h.T[::-1])
+            got = (c, h[idx], p[idx], plan.sums(rs))
+            if ref is None:
+                ref = got
+                assert len(h) > 0
+                continue
+            print("%s order %d rpt %d: %d hits, %d candidates" % (which, order, rpt, len(h), len(c)))
+            assert got[0] == ref[0]
+            assert np.array_equal(got[1], ref[1])
+            assert np.array_equal(got[2], ref[2])
+            assert np.array_equal(got[3], ref[3])
+    plan.close()
+
+
 def _variant_shapes(config1_data):
     from prestozl_amd import synth
     fft1, N1, dt1, _ = config1_data
