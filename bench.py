@@ -89,6 +89,9 @@ def serve_bench(n, rate, max_inflight, seed=4):
         cfg = B.make_config(N=1 << lg, dt=1e-4, numbins=len(fft), zmax=z, wmax=w, numharm=nh, sigma=5.0,
                             wmaxP=w > 0, batchsize=8)
         ids.append(srv.submit(cfg, fft))
+        if (k + 1) % 32 == 0:
+            print("served stream: %d of %d submitted at %.1f s" % (k + 1, n, time.perf_counter() - t0),
+                  file=sys.stderr, flush=True)
     res = [srv.wait(i) for i in ids]
     wall = time.perf_counter() - t0
     st = srv.stats()
@@ -206,6 +209,8 @@ def main():
                     help="only the timed searches (no graph-replay / one-stream / plain / serving / DM-trial legs)")
     ap.add_argument("--rsteps", type=int, default=0,
                     help="profiling runs only: search this many rsteps from the middle of the band")
+    ap.add_argument("--serve-only", action="store_true",
+                    help="run only the served-stream leg (--serve requests at --serve-rate) and print its line")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
                     help="run a plain device copy of MIB MiB five times and exit (HBM counter calibration)")
     args = ap.parse_args()
@@ -220,6 +225,9 @@ def main():
     from prestozl_amd import binding as B
     from prestozl_amd import synth
 
+    if args.serve_only:
+        print(json.dumps(serve_bench(args.serve, args.serve_rate, args.serve_inflight)), flush=True)
+        return
     fft, N, dt, opts, sigs = synth.config2(N=1 << args.log2n)
     kw = dict(N=N, dt=dt, numbins=len(fft), zmax=opts["zmax"], wmax=opts["wmax"], numharm=opts["numharm"],
               sigma=opts["sigma"], wmaxP=True)

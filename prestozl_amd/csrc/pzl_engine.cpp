@@ -1300,6 +1300,9 @@ int pzl_get_kernel(void *plan, int idx, int row, float *out)
     PZL_HIPCHK(hipSetDevice(p->device));
     PZL_HIPCHK(hipMemcpy(out, p->d_bank + H.bank_off + (long long)row * H.fftlen, sizeof(float2) * H.fftlen,
                          hipMemcpyDeviceToHost));
+    // power-of-two rows are held as K / fftlen (an exact scale): undo it
+    if ((H.fftlen & (H.fftlen - 1)) == 0)
+        for (int k = 0; k < 2 * H.fftlen; k++) out[k] *= (float)H.fftlen;
     return 0;
 }
 

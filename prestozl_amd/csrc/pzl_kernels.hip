@@ -169,8 +169,13 @@ __global__ __launch_bounds__(BANK_NT) void k_bank_build(const PzlHarm *__restric
     }
     __syncthreads();
     lds_fft_f64<-1>(sbuf, fftlen, tid, BANK_NT);
+    // For power-of-two lengths the bank holds K / fftlen: the correlation's
+    // 1 / fftlen^2 power normalisation then comes out of the product, exactly
+    // (a power-of-two scale commutes with every rounding on the way).
+    const double bscale = (fftlen & (fftlen - 1)) == 0 ? 1.0 / fftlen : 1.0;
     float2 *out = bank + H.bank_off + (long long)row * fftlen;
-    for (int k = tid; k < fftlen; k += BANK_NT) out[k] = float2{(float)sbuf[k].x, (float)sbuf[k].y};
+    for (int k = tid; k < fftlen; k += BANK_NT)
+        out[k] = float2{(float)(sbuf[k].x * bscale), (float)(sbuf[k].y * bscale)};
 }
 
 // ---------------------------------------------------------------------------
@@ -472,7 +477,10 @@ __global__ __launch_bounds__(NT) void k_corr(PzlCorrArgs a)
     extern __shared__ float2 smem[];
     float2 *b0 = smem, *b1 = PING ? smem + CP::BUF : smem;
     const int tid = threadIdx.x;
-    const float norm = (float)(1.0 / ((double)N * (double)N));
+    // power normalisation 1 / N^2 (accel_utils.c:1234, 1278-1282); for power-of-two N
+    // the bank rows carry 1 / N each, so the powers need no further scaling
+    constexpr bool POW2 = (N & (N - 1)) == 0;
+    const float norm = POW2 ? 1.0f : (float)(1.0 / ((double)N * (double)N));
 
     // twiddles of this thread, from the variant's table ([TWN][NT], see k_corr_twiddles)
     float2 tw[CP::TWN > 0 ? CP::TWN : 1];
@@ -981,9 +989,6 @@ extern "C" hipError_t pzl_launch_segments(const PzlBatchArgs *a, int max_fftlen,
     return hipGetLastError();
 }
 
-// One correlation variant: FFT length N, NT threads, G rows per workgroup,
-// ping-pong LDS or not, radices.  The grid is persistent: as many workgroups
-// as fit on the device at once (or fewer when there are fewer units).
 // One correlation variant: kernel-row length N, NT threads, G rows per
 // workgroup, ping-pong LDS or not, split rows or not, radices.  One workgroup
 // per G rows of a harmonic (the grid is not persistent, so that kernels of
@@ -1041,6 +1046,10 @@ struct CorrVariant {
 // the same with each row split into its even and odd outputs (two N/2-point FFTs)
 #define PZL_CVS(N, NT, G, PING, ...) \
     {N, launch_corr_t<N, NT, G, PING, 1, 4, __VA_ARGS__>, #N ":" #NT "t:" #G "r:" #PING ":split:" #__VA_ARGS__}
+// split rows with LDS padding of one element per 2^PADSH
+#define PZL_CVSP(N, NT, G, PING, PADSH, ...)                                                                  \
+    {N, launch_corr_t<N, NT, G, PING, 1, PADSH, __VA_ARGS__>,                                                 \
+     #N ":" #NT "t:" #G "r:" #PING ":split:pad" #PADSH ":" #__VA_ARGS__}
 // the default with LDS padding of one element per 2^PADSH
 #define PZL_CVP(N, NT, G, PING, PADSH, ...)                                                                   \
     {N, launch_corr_t<N, NT, G, PING, 0, PADSH, __VA_ARGS__>,                                                 \
@@ -1067,10 +1076,13 @@ static const CorrVariant g_corr_variants[] = {
     PZL_CV(1024, 64, 1, true, 16, 16, 4),
     PZL_CVS(1024, 128, 1, true, 8, 8, 8),
     PZL_CVP(1024, 256, 2, true, 5, 8, 8, 4, 4),
+    PZL_CVP(1024, 128, 1, true, 5, 8, 8, 4, 4),
+    PZL_CVP(1024, 256, 2, true, 5, 8, 4, 4, 8),
     PZL_CV(1280, 128, 1, true, 16, 16, 5),
     PZL_CV(1280, 256, 2, true, 16, 16, 5),
     PZL_CVP(1280, 128, 1, true, 3, 16, 16, 5),
     PZL_CVP(1280, 128, 1, true, 5, 16, 16, 5),
+    PZL_CVP(1280, 320, 2, true, 5, 8, 8, 4, 5),
     PZL_CV(2048, 128, 1, true, 16, 16, 8),
     PZL_CV(2048, 256, 1, true, 8, 8, 8, 4),
     PZL_CV(2048, 256, 2, false, 16, 16, 8),
@@ -1082,9 +1094,14 @@ static const CorrVariant g_corr_variants[] = {
     PZL_CVP(2048, 128, 1, true, 3, 16, 16, 8),
     PZL_CVS(2048, 128, 1, true, 16, 16, 4),
     PZL_CVS(2048, 256, 2, false, 16, 16, 4),
+    PZL_CVSP(2048, 256, 1, true, 5, 8, 8, 4, 4),
+    PZL_CVSP(2048, 256, 1, true, 5, 8, 4, 4, 8),
+    PZL_CVSP(2048, 256, 1, true, 5, 4, 8, 8, 4),
     PZL_CV(4096, 256, 1, false, 16, 16, 16),
     PZL_CV(4096, 512, 1, false, 8, 8, 8, 8),
     PZL_CVS(4096, 256, 1, false, 16, 16, 8),
+    PZL_CVSP(4096, 512, 1, true, 5, 8, 8, 8, 4),
+    PZL_CVP(4096, 512, 1, true, 5, 8, 8, 8, 8),
 };
 static const int g_ncorr_variants = (int)(sizeof(g_corr_variants) / sizeof(g_corr_variants[0]));
 

@@ -297,6 +297,37 @@ def test_harmsum_forms_identical(config1_data, which):
     plan.close()
 
 
+def test_fractional_rlo_every_sum_form(config1_data):
+    """-rlo with a fractional value (kept as given, accel_utils.c:1923): every
+    rstep starts at a half-integer r, so the subharmonic r indices come from the
+    per-rstep tables instead of integer arithmetic.  Hits match the oracle and
+    are identical for every harmonic-sum form."""
+    fft, kw = _c1(config1_data, numharm=8, rlo=7000.5, rhi=12000.25)
+    kw.pop("flo", None)
+    plan = _plan(kw, fft)
+    obs, ctx32 = _oracle(kw, O.FP32)
+    assert obs.rlo != int(obs.rlo)
+    co, ho, _ = ctx32.search(fft)
+    ref = None
+    for order, rpt in ((1, 1), (0, 2), (1, 4)):
+        plan.set_sum_form(order, rpt)
+        cg, _ = plan.search()
+        hg, pg = plan.hits()
+        near, worst, common = _compare_hits(plan, hg, pg, ho, list(obs.powcut))
+        print("rlo %.2f, form (%d, %d): %d gpu hits, %d oracle hits, %d near, err %.3g"
+              % (obs.rlo, order, rpt, len(hg), len(ho), near, worst))
+        assert common > 0 and worst < TOL_POW
+        _assert_same_cands(cg, co, near)
+        idx = np.lexsort(hg.T[::-1])
+        got = (cg, hg[idx], pg[idx])
+        if ref is None:
+            ref = got
+        else:
+            assert got[0] == ref[0]
+            assert np.array_equal(got[1], ref[1]) and np.array_equal(got[2], ref[2])
+    plan.close()
+
+
 def _variant_shapes(config1_data):
     from prestozl_amd import synth
     fft1, N1, dt1, _ = config1_data
