@@ -225,8 +225,10 @@ void free_plan(Plan *p)
     (void)hipFree(p->d_spows);
     (void)hipFree(p->d_recs);
     (void)hipFree(p->d_sort_temp);
-    for (int s = 0; s < Plan::kSlots; s++)
-        if (p->h_recs[s]) (void)hipHostFree(p->h_recs[s]);
+    for (auto &hb : p->hpool) {
+        if (hb.ptr) (void)hipHostFree(hb.ptr);
+        if (hb.ev) (void)hipEventDestroy(hb.ev);
+    }
     for (int k = 0; k < Plan::kSets; k++)
         if (p->gexec[k]) (void)hipGraphExecDestroy(p->gexec[k]);
     (void)hipFree(p->d_r0);
@@ -466,17 +468,20 @@ int sync_all(Plan &p)
 }
 
 // Candidate insertion runs on a thread of its own so that the launching
-// thread keeps the GPU queue full.  Jobs are whole slots, applied in batch
-// order (the scan order of the reference's insertion).
+// thread keeps the GPU queue full.  Jobs are the ordered hits of whole
+// batches, applied in batch order (the scan order of the reference's
+// insertion); each sits in a pinned buffer of the plan's pool together with
+// the event of the copy that fills it, which the insertion thread waits for.
 struct SearchState {
     Plan *p = nullptr;
     CandList cl;
     double ms_host = 0;  // insertion thread busy time
     long long nhits = 0;
+    int err = 0;         // first failure seen by the insertion thread
+    std::string err_msg;
     std::mutex mu;
     std::condition_variable cv;
-    std::deque<std::pair<int, long long>> jobs;
-    bool busy[Plan::kSlots] = {};
+    std::deque<std::pair<int, long long>> jobs;  // (pool buffer, records)
     bool stop = false;
     std::thread th;
 
@@ -491,36 +496,99 @@ struct SearchState {
         cv.notify_all();
         if (th.joinable()) th.join();
     }
-    void submit(int slot, long long n)
+    // A free pool buffer for n records: the smallest idle one large enough,
+    // else a new one while the pool stays within its limit, else wait for the
+    // insertion thread to release one.  Buffers are not freed during a search
+    // (freeing pinned memory can wait for the device), except, with nothing in
+    // flight, idle ones too small to reuse.
+    int acquire(long long n)
+    {
+        const long long want = std::max<long long>(2 * n, 1 << 20);
+        const long long add = (long long)sizeof(PzlHostHit) * want;
+        std::unique_lock<std::mutex> lk(mu);
+        for (;;) {
+            if (err) return -1;
+            int fit = -1, nbusy = 0;
+            for (int k = 0; k < (int)p->hpool.size(); k++) {
+                const auto &hb = p->hpool[k];
+                if (hb.busy) {
+                    nbusy++;
+                } else if (hb.cap >= n && (fit < 0 || hb.cap < p->hpool[fit].cap)) {
+                    fit = k;
+                }
+            }
+            if (fit >= 0) {
+                p->hpool[fit].busy = true;
+                return fit;
+            }
+            if (p->hpool_bytes + add > p->hpool_limit && nbusy > 0) {
+                cv.wait(lk);
+                continue;
+            }
+            if (p->hpool_bytes + add > p->hpool_limit) {
+                for (auto &hb : p->hpool) {
+                    if (hb.ptr) (void)hipHostFree(hb.ptr);
+                    hb.ptr = nullptr;
+                    hb.cap = 0;
+                }
+                p->hpool_bytes = 0;
+            }
+            int k = -1;
+            for (int q = 0; q < (int)p->hpool.size(); q++)
+                if (!p->hpool[q].busy && !p->hpool[q].ptr) k = q;
+            if (k < 0) {
+                p->hpool.emplace_back();
+                k = (int)p->hpool.size() - 1;
+                if (hipEventCreateWithFlags(&p->hpool[k].ev, hipEventDisableTiming) != hipSuccess) {
+                    p->hpool.pop_back();
+                    return -1;
+                }
+            }
+            auto &hb = p->hpool[k];
+            if (hipHostMalloc((void **)&hb.ptr, sizeof(PzlHostHit) * want, hipHostMallocDefault) != hipSuccess) {
+                hb.ptr = nullptr;
+                return -1;
+            }
+            hb.cap = want;
+            p->hpool_bytes += add;
+            hb.busy = true;
+            return k;
+        }
+    }
+    void submit(int buf, long long n)
     {
         {
             std::lock_guard<std::mutex> lk(mu);
-            busy[slot] = true;
-            jobs.emplace_back(slot, n);
+            jobs.emplace_back(buf, n);
         }
         cv.notify_all();
-    }
-    // until the insertion thread no longer reads this slot's host records
-    void wait_slot(int slot)
-    {
-        std::unique_lock<std::mutex> lk(mu);
-        cv.wait(lk, [&] { return !busy[slot]; });
     }
     void run()
     {
         for (;;) {
             std::pair<int, long long> j;
+            Plan::HostBuf hb;
             {
                 std::unique_lock<std::mutex> lk(mu);
                 cv.wait(lk, [&] { return stop || !jobs.empty(); });
                 if (jobs.empty()) return;
                 j = jobs.front();
                 jobs.pop_front();
+                hb = p->hpool[j.first];  // the pool only grows while jobs are pending
             }
-            insert_hits(p->h_recs[j.first], j.second);
+            const hipError_t e = hipEventSynchronize(hb.ev);
+            if (e == hipSuccess) {
+                insert_hits(hb.ptr, j.second);
+            } else {
+                std::lock_guard<std::mutex> lk(mu);
+                if (!err) {
+                    err = PZL_ERR_HIP;
+                    err_msg = hipGetErrorString(e);
+                }
+            }
             {
                 std::lock_guard<std::mutex> lk(mu);
-                busy[j.first] = false;
+                p->hpool[j.first].busy = false;
             }
             cv.notify_all();
         }
@@ -555,28 +623,31 @@ struct SearchState {
     }
 };
 
-// Order the hits of one slot on the GPU, bring them over and queue them for
-// the insertion thread (the caller has waited for the slot's count).
+// Order the hits of one slot on the GPU, release the slot, bring the hits over
+// and queue them for the insertion thread (the caller has waited for the
+// slot's count).  Nothing here waits for the GPU: the copy's event travels
+// with the job.
+int reset_slot(Plan &p, int slot, hipStream_t st);
+
 int apply_slot(Plan &p, int slot, long long n, SearchState &ss)
 {
-    if (n <= 0) return 0;
+    if (n <= 0) return reset_slot(p, slot, p.st2);
     PZL_HIPCHK(pzl_sort_hits(p.d_sort_temp, &p.sort_temp_bytes, p.d_hkeys[slot], p.d_skeys, p.d_hpows[slot],
                              p.d_spows, n, p.key_bits, p.st2));
+    // the sort has read the slot: a later batch's harmonic sum may refill it
+    // while the records are prepared and copied
+    int rc = reset_slot(p, slot, p.st2);
+    if (rc) return rc;
     PZL_HIPCHK(pzl_launch_hitprep(p.d_skeys, p.d_spows, n, p.d_startr, p.g.corr_uselen, p.fnumz, p.fnumw,
                                   p.g.numharmstages, p.numindep_d, p.d_recs, p.st2));
-    ss.wait_slot(slot);
-    if (n > p.h_recs_cap[slot]) {
-        if (p.h_recs[slot]) PZL_HIPCHK(hipHostFree(p.h_recs[slot]));
-        p.h_recs[slot] = nullptr;
-        p.h_recs_cap[slot] = 0;
-        const long long cap = std::min<long long>(std::max<long long>(2 * n, 1 << 20), p.hitcap);
-        PZL_HIPCHK(hipHostMalloc((void **)&p.h_recs[slot], sizeof(PzlHostHit) * cap, hipHostMallocDefault));
-        p.h_recs_cap[slot] = cap;
-    }
-    PZL_HIPCHK(hipMemcpyAsync(p.h_recs[slot], p.d_recs, sizeof(PzlHostHit) * n, hipMemcpyDeviceToHost, p.st2));
-    PZL_HIPCHK(hipStreamSynchronize(p.st2));
+    const int buf = ss.acquire(n);
+    if (buf < 0)
+        return fail(ss.err ? ss.err : PZL_ERR_HIP, "no pinned buffer for %lld hit records%s%s", n,
+                    ss.err_msg.empty() ? "" : ": ", ss.err_msg.c_str());
+    PZL_HIPCHK(hipMemcpyAsync(p.hpool[buf].ptr, p.d_recs, sizeof(PzlHostHit) * n, hipMemcpyDeviceToHost, p.st2));
+    PZL_HIPCHK(hipEventRecord(p.hpool[buf].ev, p.st2));
     ss.nhits += n;
-    ss.submit(slot, n);
+    ss.submit(buf, n);
     return 0;
 }
 
@@ -594,11 +665,7 @@ int finalize(Plan &p, int slot, long long r0, int nb, SearchState &ss, Timers *t
 {
     PZL_HIPCHK(hipEventSynchronize(p.ev_count[slot]));
     const unsigned long long n = p.h_counts[2 * slot];
-    if ((long long)n <= p.hitcap) {
-        int rc = apply_slot(p, slot, (long long)n, ss);
-        if (rc) return rc;
-        return reset_slot(p, slot, p.st2);
-    }
+    if ((long long)n <= p.hitcap) return apply_slot(p, slot, (long long)n, ss);  // resets the slot
     // overflow: let every batch in flight finish, then search this batch
     // again one rstep at a time
     int rc = sync_all(p);
@@ -618,7 +685,7 @@ int finalize(Plan &p, int slot, long long r0, int nb, SearchState &ss, Timers *t
         rc = apply_slot(p, slot, (long long)n1, ss);
         if (rc) return rc;
     }
-    return reset_slot(p, slot, p.st2);
+    return 0;
 }
 
 double plane_alg_bytes(const Plan &p)
@@ -992,6 +1059,7 @@ int pzl_search(void *plan, long long rstep_lo, long long rstep_hi, PzlCand *cand
         if (rc) return rc;
     }
     ss.finish();
+    if (ss.err) return fail(ss.err, "hit records did not arrive: %s", ss.err_msg.c_str());
     p->last_cands = ss.cl.items();
     const auto &items = p->last_cands;
     if (ncands) *ncands = (long long)items.size();

@@ -92,8 +92,18 @@ struct Plan {
     PzlHostHit *d_recs = nullptr;
     void *d_sort_temp = nullptr;
     size_t sort_temp_bytes = 0;
-    PzlHostHit *h_recs[kSlots] = {};  // pinned, one per slot (read by the insertion thread)
-    long long h_recs_cap[kSlots] = {};
+    // pinned hit-record buffers handed to the insertion thread: each carries
+    // the event of the copy that fills it; the pool grows up to hpool_limit
+    // bytes, so that bursts of hits never hold up the launching thread
+    struct HostBuf {
+        PzlHostHit *ptr = nullptr;
+        long long cap = 0;
+        hipEvent_t ev = nullptr;
+        bool busy = false;
+    };
+    std::vector<HostBuf> hpool;
+    long long hpool_bytes = 0;
+    long long hpool_limit = 2LL << 30;
     hipEvent_t ev_count[kSlots] = {};
     hipEvent_t ev_free[kSlots] = {};
     float *d_sums = nullptr;

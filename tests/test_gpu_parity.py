@@ -328,6 +328,36 @@ def test_fractional_rlo_every_sum_form(config1_data):
     plan.close()
 
 
+def test_hit_overflow_rerun_identical(config1_data):
+    """A batch whose hits overflow the hit slot is searched again one rstep at
+    a time (the slot always holds one rstep's cells x stages): candidates and
+    hits equal those of a plan whose slots hold whole batches.  Strong bins in
+    one band of the spectrum (every third bin, well above the noise median)
+    make nearly every cell there a hit."""
+    fft, kw = _c1(config1_data)
+    fft = fft.copy()
+    lo_bin, hi_bin = 12000, 19000
+    fft[lo_bin:hi_bin:3] *= 40.0
+    lo, hi = 8, 24  # two batches of 8 rsteps over the band
+    p1 = _plan(kw, fft)
+    c1, _ = p1.search(lo, hi)
+    h1, w1 = p1.hits()
+    kw2 = dict(kw, maxhits=1)  # slot = one rstep's cells x stages
+    p2 = _plan(kw2, fft)
+    c2, _ = p2.search(lo, hi)
+    h2, w2 = p2.hits()
+    g = p1.geom
+    per_rstep = g.corr_uselen * p1.harms[0].numz * max(p1.harms[0].numw, 1) * g.numharmstages
+    per_batch = np.bincount((h1[:, 0] - lo) // kw["batchsize"])
+    print("hits %d, per batch %s, slot of the small plan %d" % (len(h1), per_batch.tolist(), per_rstep))
+    assert per_batch.max() > per_rstep  # the small plan's slot did overflow
+    assert c1 == c2
+    i1, i2 = np.lexsort(h1.T[::-1]), np.lexsort(h2.T[::-1])
+    assert np.array_equal(h1[i1], h2[i2]) and np.array_equal(w1[i1], w2[i2])
+    p1.close()
+    p2.close()
+
+
 def _variant_shapes(config1_data):
     from prestozl_amd import synth
     fft1, N1, dt1, _ = config1_data
