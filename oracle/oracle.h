@@ -50,6 +50,38 @@ void oracle_gen_w_response(double roffset, int numbetween, double z, double w, i
 /* Fresnel integrals S(x), C(x) with the pi/2 convention (fresnl.c:176). */
 void oracle_fresnel(double x, double *s, double *c);
 
+/* ---------------- candidate optimisation (optimize.c) -------------------- */
+/* rderivs and fourierprops, presto.h:161-189 (same layout). */
+typedef struct {
+    double pow, phs, dpow, dphs, d2pow, d2phs, locpow;
+} OracleRDerivs;
+typedef struct {
+    double r;
+    float rerr;
+    double z;
+    float zerr;
+    double w;
+    float werr;
+    float pow, powerr, sig, rawpow, phs, phserr, cen, cenerr, pur, purerr, locpow;
+} OracleFourierProps;
+int oracle_z_resp_halfwidth_acc(double z, int highacc);            /* responses.c:27-47 */
+int oracle_w_resp_halfwidth_acc(double z, double w, int highacc);  /* responses.c:49-84 */
+/* data: interleaved float complex bins; ans: re, im */
+void oracle_rz_interp(const float *data, int numdata, double r, double z, int kern_half_width, float *ans);
+void oracle_rzw_interp(const float *data, int numdata, double r, double z, double w, int kern_half_width,
+                       float *ans);                                 /* rzwinterp.c:145-229 */
+double oracle_get_localpower3d(const float *data, int numdata, double r, double z, double w); /* :75-136 */
+void oracle_get_derivs3d(const float *data, int numdata, double r, double z, double w, double localpower,
+                         OracleRDerivs *result);                    /* characteristics.c:138-191 */
+int oracle_corr_rzw_vol(const float *data, int numdata, int numbetween, int startbin, double zlo, double zhi,
+                        int numz, double wlo, double whi, int numw, int fftlen, int *nextbin,
+                        float *out);                                /* rzwinterp.c:3-140 */
+void oracle_max_rzw_arr_harmonics(const float *data, int numdata, int num_harmonics, double rin, double zin,
+                                  double win, double *rout, double *zout, double *wout, OracleRDerivs *derivs,
+                                  double *powers);                 /* maximize_rzw.c:85-191 */
+int oracle_calc_props(const OracleRDerivs *d, double r, double z, double w,
+                      OracleFourierProps *res);                     /* characteristics.c:193-265 */
+
 /* ---------------- statistics -------------------------------------------- */
 double oracle_power_for_sigma(double sigma, int numsum, double numtrials); /* characteristics.c:670 */
 double oracle_candidate_sigma(double power, int numsum, double numtrials); /* characteristics.c:647 */

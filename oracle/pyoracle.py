@@ -49,6 +49,17 @@ class Hit(C.Structure):
                 ("ri", C.c_int), ("power", C.c_float), ("sigma", C.c_float)]
 
 
+class RDerivs(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("pow", "phs", "dpow", "dphs", "d2pow", "d2phs", "locpow")]
+
+
+class FourierProps(C.Structure):
+    _fields_ = [("r", C.c_double), ("rerr", C.c_float), ("z", C.c_double), ("zerr", C.c_float),
+                ("w", C.c_double), ("werr", C.c_float)] + \
+               [(k, C.c_float) for k in ("pow", "powerr", "sig", "rawpow", "phs", "phserr", "cen", "cenerr",
+                                         "pur", "purerr", "locpow")]
+
+
 _lib = None
 
 
@@ -103,8 +114,86 @@ def lib():
         L.oracle_insert_cands.argtypes = [P(C.c_double), P(C.c_float), P(C.c_float), P(C.c_int),
                                           P(C.c_double), P(C.c_double), C.c_longlong, P(Cand), C.c_longlong]
         L.oracle_insert_cands.restype = C.c_longlong
+        L.oracle_z_resp_halfwidth_acc.argtypes = [C.c_double, C.c_int]
+        L.oracle_w_resp_halfwidth_acc.argtypes = [C.c_double, C.c_double, C.c_int]
+        L.oracle_rz_interp.argtypes = [P(C.c_float), C.c_int, C.c_double, C.c_double, C.c_int, P(C.c_float)]
+        L.oracle_rzw_interp.argtypes = [P(C.c_float), C.c_int, C.c_double, C.c_double, C.c_double, C.c_int,
+                                        P(C.c_float)]
+        L.oracle_get_localpower3d.argtypes = [P(C.c_float), C.c_int, C.c_double, C.c_double, C.c_double]
+        L.oracle_get_localpower3d.restype = C.c_double
+        L.oracle_get_derivs3d.argtypes = [P(C.c_float), C.c_int, C.c_double, C.c_double, C.c_double,
+                                          C.c_double, P(RDerivs)]
+        L.oracle_corr_rzw_vol.argtypes = [P(C.c_float), C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                          C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, P(C.c_int),
+                                          P(C.c_float)]
+        L.oracle_corr_rzw_vol.restype = C.c_int
+        L.oracle_max_rzw_arr_harmonics.argtypes = [P(C.c_float), C.c_int, C.c_int, C.c_double, C.c_double,
+                                                   C.c_double, P(C.c_double), P(C.c_double), P(C.c_double),
+                                                   P(RDerivs), P(C.c_double)]
+        L.oracle_calc_props.argtypes = [P(RDerivs), C.c_double, C.c_double, C.c_double, P(FourierProps)]
+        L.oracle_calc_props.restype = C.c_int
         _lib = L
     return _lib
+
+
+def _c64(data):
+    a = np.ascontiguousarray(data)
+    if a.dtype == np.complex64:
+        a = a.view(np.float32)
+    assert a.dtype == np.float32
+    return a
+
+
+def w_resp_halfwidth_acc(z, w, highacc=False):
+    return lib().oracle_w_resp_halfwidth_acc(z, w, int(highacc))
+
+
+def rzw_interp(data, r, z, w, khw):
+    """Complex Fourier interpolation at (r, z, w) (rzwinterp.c:145)."""
+    a = _c64(data)
+    out = np.zeros(2, dtype=np.float32)
+    lib().oracle_rzw_interp(fptr(a), len(a) // 2, r, z, w, khw, fptr(out))
+    return complex(out[0], out[1])
+
+
+def localpower3d(data, r, z, w):
+    a = _c64(data)
+    return lib().oracle_get_localpower3d(fptr(a), len(a) // 2, r, z, w)
+
+
+def derivs3d(data, r, z, w, locpow):
+    a = _c64(data)
+    d = RDerivs()
+    lib().oracle_get_derivs3d(fptr(a), len(a) // 2, r, z, w, locpow, C.byref(d))
+    return d
+
+
+def corr_rzw_vol(data, numbetween, startbin, zlo, zhi, numz, wlo, whi, numw, fftlen):
+    """(volume [numw][numz][numgoodbins] complex64, nextbin) (rzwinterp.c:3)."""
+    a = _c64(data)
+    out = np.zeros(2 * numw * numz * fftlen, dtype=np.float32)
+    nb = C.c_int(0)
+    n = lib().oracle_corr_rzw_vol(fptr(a), len(a) // 2, numbetween, startbin, zlo, zhi, numz, wlo, whi, numw,
+                                  fftlen, C.byref(nb), fptr(out))
+    return out[: 2 * numw * numz * n].view(np.complex64).reshape(numw, numz, n), nb.value
+
+
+def max_rzw_arr_harmonics(data, numharm, r, z, w):
+    """(r, z, w, [RDerivs per harmonic], powers) (maximize_rzw.c:85)."""
+    a = _c64(data)
+    ro, zo, wo = C.c_double(), C.c_double(), C.c_double()
+    ders = (RDerivs * numharm)()
+    pows = (C.c_double * numharm)()
+    lib().oracle_max_rzw_arr_harmonics(fptr(a), len(a) // 2, numharm, r, z, w, C.byref(ro), C.byref(zo),
+                                       C.byref(wo), ders, pows)
+    return ro.value, zo.value, wo.value, list(ders), list(pows)
+
+
+def calc_props(derivs, r, z, w):
+    p = FourierProps()
+    if lib().oracle_calc_props(C.byref(derivs), r, z, w, C.byref(p)) != 0:
+        raise ValueError("calc_props: power or local power out of bounds")
+    return p
 
 
 def fptr(a):
