@@ -181,6 +181,37 @@ int pzl_insert_cands(const double *rr, const float *sigma, const float *power, c
                      const double *z, const double *w, long long n, PzlCand *out, long long maxout,
                      long long *nout);
 
+/* ---------------- candidate optimisation (accelsearch_CPU2) --------------
+ * optimize_accelcand with harmonic polishing (accel_utils.c:573-612):
+ * max_rzw_arr_harmonics (maximize_rzw.c:85-191) -- the maximum of the
+ * 8x-oversampled F/Fdot/Fdotdot volume around the candidate, summed over its
+ * harmonics with each harmonic's local power -- then, at the maximum, the
+ * local power and the power/phase derivatives of every harmonic
+ * (get_localpower3d / get_derivs3d, characteristics.c:75-191).  All
+ * candidates are done together on the GPU.  For searches with w > 0 (the
+ * reference's w = 0 searches use a simplex optimiser instead,
+ * maximize_rz.c:98-220, which these entry points do not cover). */
+#define PZL_MAX_NUMHARM 32
+typedef struct PzlRDerivs { /* rderivs, presto.h:161-169 */
+    double pow, phs, dpow, dphs, d2pow, d2phs, locpow;
+} PzlRDerivs;
+typedef struct PzlOptCand {
+    /* in: the search-grid candidate; r counts bins of the spectrum given
+     * (file bin minus the spectrum's first bin) */
+    double r, z, w;
+    int numharm;
+    /* out: the volume maximum (fundamental), and per harmonic ii (number
+     * ii + 1) its derivatives at (r, z, w) * (ii + 1) and its power */
+    double ropt, zopt, wopt;
+    PzlRDerivs derivs[PZL_MAX_NUMHARM];
+    double pows[PZL_MAX_NUMHARM];
+} PzlOptCand;
+/* With the plan's resident spectrum (pzl_set_spectrum). */
+int pzl_optimize(void *plan, PzlOptCand *cands, long long n);
+/* With a host spectrum of numbins complex bins (interleaved re, im). */
+int pzl_optimize_spectrum(int device, const float *host_fft, long long numbins, PzlOptCand *cands,
+                          long long n);
+
 /* ---------------- request loop (continuous batching on one GPU) ----------
  * Replaces the reference's multi-process pipeline
  * (bin/accelsearch_pipeline_cu.py: pool_size processes of CPU1/GPU/CPU2

@@ -304,8 +304,12 @@ void oracle_max_rzw_arr_harmonics(const float *data, int numdata, int num_harmon
             for (jj = 0; jj < numz; jj++)
                 for (kk = 0; kk < numr; kk++) {
                     const long ri = (long)round(((hhlobin + kk * dr) * hfrac - lobin) * numbetween);
-                    const float *a = vol + 2 * (((size_t)ii * numz + jj) * ngood + ri);
-                    powsum[((size_t)ii * numz + jj) * numr + kk] += power_f(a) / locpow;
+                    /* vol is one contiguous [w][z][r] block: a negative index
+                     * reads the end of the preceding row; before the block the
+                     * reference reads outside its allocation, taken here as 0 */
+                    const long flat = ((long)ii * numz + jj) * ngood + ri;
+                    const float pw0 = flat >= 0 ? power_f(vol + 2 * flat) : 0.0f;
+                    powsum[((size_t)ii * numz + jj) * numr + kk] += pw0 / locpow;
                 }
         free(vol);
     }

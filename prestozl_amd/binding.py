@@ -63,6 +63,19 @@ class Stats(C.Structure):
         return d
 
 
+MAX_NUMHARM = 32
+
+
+class RDerivs(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("pow", "phs", "dpow", "dphs", "d2pow", "d2phs", "locpow")]
+
+
+class OptCand(C.Structure):
+    _fields_ = [("r", C.c_double), ("z", C.c_double), ("w", C.c_double), ("numharm", C.c_int),
+                ("ropt", C.c_double), ("zopt", C.c_double), ("wopt", C.c_double),
+                ("derivs", RDerivs * MAX_NUMHARM), ("pows", C.c_double * MAX_NUMHARM)]
+
+
 class RequestStats(C.Structure):
     _fields_ = [("ms_queue", C.c_double), ("ms_run", C.c_double), ("ms_total", C.c_double),
                 ("t_submit_ms", C.c_double), ("t_admit_ms", C.c_double), ("t_end_ms", C.c_double),
@@ -131,6 +144,8 @@ def lib():
                                          C.c_longlong, P(C.c_int), P(C.c_double)]
         L.pzl_plan_set_variant.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.pzl_plan_sum_form.argtypes = [C.c_void_p, P(C.c_int), P(C.c_int)]
+        L.pzl_optimize.argtypes = [C.c_void_p, P(OptCand), C.c_longlong]
+        L.pzl_optimize_spectrum.argtypes = [C.c_int, P(C.c_float), C.c_longlong, P(OptCand), C.c_longlong]
         L.pzl_plan_set_sum_form.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.pzl_corr_num_variants.argtypes = [C.c_int]
         L.pzl_corr_variant_name.argtypes = [C.c_int, C.c_int]
@@ -241,6 +256,13 @@ class Plan:
     def set_sum_form(self, order, rpt):
         check(lib().pzl_plan_set_sum_form(self.h, int(order), int(rpt)))
 
+    def optimize(self, cands):
+        """optimize_accelcand of (r, z, w, numharm) candidates with the plan's
+        resident spectrum (r in bins of it)."""
+        arr = _opt_array(cands)
+        check(lib().pzl_optimize(self.h, arr, len(cands)))
+        return _opt_result(arr, len(cands))
+
     def fftlens(self):
         """The FFT lengths of the plan's harmonics (one kernel variant each)."""
         return sorted({h.fftlen for h in self.harms})
@@ -323,6 +345,34 @@ def tune_batchsize(cfg, fft, candidates=(4, 8, 16), band=64, device=0):
     check(lib().pzl_tune_batchsize(C.byref(cfg), int(device), _fptr(a), len(a) // 2, cand, len(candidates),
                                    int(band), C.byref(best), ms))
     return best.value, {int(c): ms[i] for i, c in enumerate(candidates)}
+
+
+def _opt_array(cands):
+    arr = (OptCand * max(len(cands), 1))()
+    for i, (r, z, w, nh) in enumerate(cands):
+        arr[i].r, arr[i].z, arr[i].w, arr[i].numharm = float(r), float(z), float(w), int(nh)
+    return arr
+
+
+def _opt_result(arr, n):
+    out = []
+    for i in range(n):
+        a = arr[i]
+        out.append(dict(r=a.ropt, z=a.zopt, w=a.wopt,
+                        derivs=[{k: getattr(a.derivs[h], k) for k, _ in RDerivs._fields_} for h in range(a.numharm)],
+                        pows=[a.pows[h] for h in range(a.numharm)]))
+    return out
+
+
+def optimize_spectrum(fft, cands, device=0):
+    """optimize_accelcand (harmonic polishing) of each (r, z, w, numharm), r
+    in bins of 'fft': [{r, z, w, derivs: [...], pows: [...]}, ...]."""
+    a = np.ascontiguousarray(fft)
+    if a.dtype == np.complex64:
+        a = a.view(np.float32)
+    arr = _opt_array(cands)
+    check(lib().pzl_optimize_spectrum(int(device), _fptr(a), len(a) // 2, arr, len(cands)))
+    return _opt_result(arr, len(cands))
 
 
 class Server:

@@ -22,6 +22,7 @@
 #include "pzl_fft.h"
 #include "pzl_kernels.h"
 #include "pzl_math.h"
+#include "pzl_response.h"
 #include "pzl_stats.h"
 
 // ---------------------------------------------------------------------------
@@ -63,99 +64,7 @@ __global__ __launch_bounds__(BANK_NT) void k_bank_build(const PzlHarm *__restric
     const int half = numkern / 2;
     const int fftlen = H.fftlen;
     double2 resp[BANK_MAXV];
-
-    if (fabs(w) >= 1e-4) {
-        // gen_w_response (responses.c:333-418): float samples of a cubic-phase
-        // chirp, zero-padded by numbetween; only numkern bins of its real FFT
-        // are kept, so they are computed directly as a DFT in double.
-        const int npts = pzl_next2_to_n(6 * khw + 200 + numkern / PZL_NUMBETWEEN);
-        const double dt = 1.0 / (double)npts, amp = 2.0 * dt;
-        const int fbar = npts / 4;
-        const double f = (double)fbar - 0.5 * z + w / 12.0, fd = (z - 0.5 * w) / 2.0, fdd = w / 6.0;
-        float *samp = (float *)sbuf;
-        for (int t = tid; t < npts; t += BANK_NT) {
-            const double tt = t * dt;
-            samp[t] = (float)(amp * cos(6.283185307179586477 * (tt * (tt * (tt * fdd + fd) + f))));
-        }
-        __syncthreads();
-        const int nfft = npts * PZL_NUMBETWEEN;  // real FFT length
-        const int k0 = fbar * PZL_NUMBETWEEN - half;
-#pragma unroll
-        for (int i = 0; i < BANK_MAXV; i++) {
-            const int m = tid + i * BANK_NT;
-            double ar = 0.0, ai = 0.0;
-            if (m < numkern) {
-                const long long k = k0 + m;
-                // X[k] = sum_t samp[t] exp(-2 pi i t k / nfft); the rotation is
-                // re-anchored exactly every 64 samples.
-                double sr, cr;
-                sincospi(-2.0 * (double)k / (double)nfft, &sr, &cr);
-                for (int t0 = 0; t0 < npts; t0 += 64) {
-                    double s0, c0;
-                    const long long ph = ((long long)t0 * k) % nfft;
-                    sincospi(-2.0 * (double)ph / (double)nfft, &s0, &c0);
-                    const int tend = t0 + 64 < npts ? t0 + 64 : npts;
-                    for (int t = t0; t < tend; t++) {
-                        const double d = (double)samp[t];
-                        ar += d * c0;
-                        ai += d * s0;
-                        const double nc = c0 * cr - s0 * sr;
-                        s0 = c0 * sr + s0 * cr;
-                        c0 = nc;
-                    }
-                }
-            }
-            resp[i] = double2{ar, ai};
-        }
-        __syncthreads();
-    } else if (fabs(z) >= 1e-4) {
-        // gen_z_response (responses.c:228-316), roffset = 0
-        const double PI = 3.14159265358979323846;
-        const double absz = fabs(z);
-        const int signz = (z < 0.0) ? -1 : 1;
-        const double zd = signz * 1.41421356237309504880 / sqrt(absz);
-        const double cons = zd / 2.0, pibyz = PI / z;
-        const double startr = -0.5 * z + numkern / (double)(2 * PZL_NUMBETWEEN);
-        const double delta = -1.0 / PZL_NUMBETWEEN;
-#pragma unroll
-        for (int i = 0; i < BANK_MAXV; i++) {
-            const int m = tid + i * BANK_NT;
-            double re = 0.0, im = 0.0;
-            if (m < numkern) {
-                const double r = startr + m * delta;
-                const double yy = r * zd, zz = yy + z * zd, xx = pibyz * r * r;
-                double fsy, fcy, fsz, fcz;
-                pzl_fresnel(yy, &fsy, &fcy);
-                pzl_fresnel(zz, &fsz, &fcz);
-                const double c = cos(xx), s = sin(xx);
-                const double tr = signz * (fcz - fcy), ti = fsy - fsz;
-                re = (tr * c - ti * s) * cons;
-                im = -(tr * s + ti * c) * cons;
-            }
-            resp[i] = double2{re, im};
-        }
-    } else {
-        // gen_r_response (responses.c:159-225), roffset = 0: sinc * e^{i r}
-#pragma unroll
-        for (int i = 0; i < BANK_MAXV; i++) {
-            const int m = tid + i * BANK_NT;
-            double re = 0.0, im = 0.0;
-            if (m < numkern) {
-                const double x = numkern / (double)(2 * PZL_NUMBETWEEN) - m * 0.5;  // r / pi
-                if (m == half) {
-                    re = 1.0;
-                    im = 0.0;
-                } else {
-                    double s, c;
-                    sincospi(x, &s, &c);
-                    const double sinc = s / (3.14159265358979323846 * x);
-                    re = c * sinc;
-                    im = s * sinc;
-                }
-            }
-            resp[i] = double2{re, im};
-        }
-    }
+    pzl_gen_response<BANK_NT, BANK_MAXV>(0.0, PZL_NUMBETWEEN, z, w, numkern, tid, (float *)sbuf, resp);
     // place_complex_kernel (corr_prep.c:92-114) then forward FFT
     for (int k = tid; k < fftlen; k += BANK_NT) sbuf[k] = double2{0.0, 0.0};
     __syncthreads();

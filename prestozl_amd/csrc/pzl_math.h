@@ -43,6 +43,28 @@ PZL_HD int pzl_fftlen_from_kernwidth(int kw)
     return pzl_next2_to_n(kw * 5);
 }
 
+// responses.c:27-84; highacc: HIGHACC (three times the Fourier-interpolation
+// bins) instead of LOWACC
+PZL_HD int pzl_z_resp_halfwidth_acc(double z, bool highacc)
+{
+    return (int)(0.5 * 1.1 * fabs(z)) + (highacc ? 3 * PZL_NUMFINTBINS : PZL_NUMFINTBINS);
+}
+
+PZL_HD int pzl_w_resp_halfwidth_acc(double z, double w, bool highacc)
+{
+    if (fabs(w) < 1.0e-7) return pzl_z_resp_halfwidth_acc(z, highacc);
+    double r0 = 0.5 * (w / 6.0 - z);
+    double r1 = 0.5 * (w / 6.0 + z);
+    double maxdev = fabs(r0) > fabs(r1) ? fabs(r0) : fabs(r1);
+    double u_ext = 0.5 - z / w;
+    if (u_ext > 0.0 && u_ext < 1.0) {
+        double z0 = z - w / 2.0;
+        double r_ext = 0.5 * w * u_ext * u_ext + z0 * u_ext + r0;
+        maxdev = fabs(r_ext) > maxdev ? fabs(r_ext) : maxdev;
+    }
+    return (int)(1.1 * maxdev) + (highacc ? 3 * PZL_NUMFINTBINS : PZL_NUMFINTBINS);
+}
+
 PZL_HD int pzl_z_resp_halfwidth(double z) { return (int)(0.5 * 1.1 * fabs(z)) + PZL_NUMFINTBINS; }
 
 PZL_HD int pzl_w_resp_halfwidth(double z, double w)

@@ -1,0 +1,93 @@
+"""Candidate optimisation on the GPU (pzl_optimize / pzl_optimize_spectrum)
+against the oracle's restatement of optimize_accelcand with harmonic
+polishing (max_rzw_arr_harmonics, maximize_rzw.c:85-191; get_localpower3d /
+get_derivs3d, characteristics.c:75-191).
+
+Tolerances.  The GPU builds its responses in double precision where the
+reference (and the oracle) use float FFTs, so interpolated amplitudes differ
+at ~1e-6 relative:
+  * the volume maximum (r, z, w): identical grid point;
+  * power and local power: 1e-4 relative; phase: 1e-4 rad;
+  * the finite differences of get_derivs3d (h = 0.003) amplify that: dpow
+    within 2e-3 of the power, d2pow within 2% of |d2pow| + 0.05 pow, dphs
+    within 2e-3 rad/bin, d2phs within 0.5 rad/bin^2.
+"""
+import numpy as np
+import pytest
+
+import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+N = 1 << 16
+
+
+def _chirp(harmonics, noise, seed=5):
+    t = np.arange(N, dtype=np.float64) / N
+    x = np.zeros(N)
+    for a, f0, z0, w in harmonics:
+        x += a * np.cos(2 * np.pi * (f0 * t + 0.5 * z0 * t * t + w * t ** 3 / 6.0))
+    x += np.random.Generator(np.random.PCG64(seed)).standard_normal(N) * noise
+    return np.fft.rfft(x)[: N // 2].astype(np.complex64)
+
+
+def _compare(g, o, label):
+    ro, zo, wo, ders, pows = o
+    assert (g["r"], g["z"], g["w"]) == pytest.approx((ro, zo, wo), abs=1e-9), label
+    for h, (dg, do) in enumerate(zip(g["derivs"], ders)):
+        p = do.pow
+        assert dg["pow"] == pytest.approx(p, rel=1e-4), (label, h)
+        assert dg["locpow"] == pytest.approx(do.locpow, rel=1e-4), (label, h)
+        assert abs(dg["phs"] - do.phs) < 1e-4, (label, h)
+        assert abs(dg["dpow"] - do.dpow) <= 2e-3 * p, (label, h)
+        assert abs(dg["d2pow"] - do.d2pow) <= 0.02 * abs(do.d2pow) + 0.05 * p, (label, h)
+        assert abs(dg["dphs"] - do.dphs) <= 2e-3, (label, h)
+        assert abs(dg["d2phs"] - do.d2phs) <= 0.5, (label, h)
+        assert g["pows"][h] == dg["pow"]
+
+
+@pytest.mark.parametrize("numharm", [1, 2, 4, 8])
+def test_optimize_matches_oracle_synthetic(numharm):
+    from prestozl_amd import binding as B
+    f0, z0, w = 1510.37, 3.1, 8.0
+    comps = [(1.0 / (1 + 0.3 * k), f0 * (k + 1), z0 * (k + 1), w * (k + 1)) for k in range(numharm)]
+    data = _chirp(comps, noise=5.0)
+    r, z, ww = f0 + 0.5 * z0 + w / 6.0, z0 + 0.5 * w, w
+    starts = [(np.round(r * numharm * 2) / 2 / numharm + dr, np.round(z * numharm / 2) * 2 / numharm,
+               np.round(ww * numharm / 20) * 20 / numharm, numharm) for dr in (0.0, 0.5 / numharm)]
+    got = B.optimize_spectrum(data, starts)
+    for s, g in zip(starts, got):
+        o = O.max_rzw_arr_harmonics(data, numharm, s[0], s[1], s[2])
+        print("numharm %d start %s -> gpu (%.4f %.3f %.2f) oracle (%.4f %.3f %.2f) pow %.4g / %.4g"
+              % (numharm, s[:3], g["r"], g["z"], g["w"], o[0], o[1], o[2], g["pows"][0], o[4][0]))
+        _compare(g, o, s)
+
+
+def test_optimize_search_candidates_config2_band():
+    """Candidates of a zmax 200 / wmax 300 / 16-harmonic search band around an
+    injection, optimised with the plan's resident spectrum."""
+    from prestozl_amd import binding as B
+    from prestozl_amd import synth
+    fft, N2, dt, opts, sigs = synth.config2(N=1 << 21)
+    kw = dict(N=N2, dt=dt, numbins=len(fft), zmax=200, wmax=300, numharm=16, sigma=3.0, wmaxP=True, batchsize=4)
+    plan = B.Plan(B.make_config(**kw))
+    plan.set_spectrum(fft)
+    cands, _ = plan.search(60, 63)
+    assert cands
+    top = sorted(cands, key=lambda c: -c[1])[:3]
+    sel = [(c[3], c[4], c[5], c[2]) for c in top]  # (r, z, w, numharm); the spectrum starts at bin 0
+    got = plan.optimize(sel)
+    plan.close()
+    for s, g in zip(sel, got):
+        o = O.max_rzw_arr_harmonics(fft, s[3], s[0], s[1], s[2])
+        print("cand %s -> gpu (%.4f %.3f %.2f) oracle (%.4f %.3f %.2f)" % (s, g["r"], g["z"], g["w"], *o[:3]))
+        _compare(g, o, s)
+
+
+def test_optimize_rejects_bad_candidates():
+    from prestozl_amd import binding as B
+    data = _chirp([], noise=1.0)
+    with pytest.raises(B.PzlError):
+        B.optimize_spectrum(data, [(100.0, 0.0, 10.0, 0)])
+    with pytest.raises(B.PzlError):
+        B.optimize_spectrum(data, [(100.0, 0.0, 10.0, 40)])
