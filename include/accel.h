@@ -59,6 +59,27 @@ typedef struct RDERIVS {
 #ifndef INFODATA_DEFINED
 #define INFODATA_DEFINED
 #define MAXNUMONOFF 40
+/* presto.h:171-189 (the records of the binary .cand file) */
+typedef struct FOURIERPROPS {
+    double r;
+    float rerr;
+    double z;
+    float zerr;
+    double w;
+    float werr;
+    float pow;
+    float powerr;
+    float sig;
+    float rawpow;
+    float phs;
+    float phserr;
+    float cen;
+    float cenerr;
+    float pur;
+    float purerr;
+    float locpow;
+} fourierprops;
+
 typedef struct INFODATA {
     double ra_s, dec_s, N, dt, fov, mjd_f, dm, freq, freqband, chan_wid, wavelen, waveband, energy, energyband;
     double onoff[MAXNUMONOFF * 2];
@@ -264,6 +285,17 @@ void add_ffdotpows_ptrs(ffdotpows *fundamental, ffdotpows *subharmonic, int numh
 /* accel_utils.c:1507-1555 */
 GSList *search_ffdotpows(ffdotpows *ffdot, int numharm, accelobs *obs, GSList *cands);
 /* accelsearch_lib.c:360-471, :473-901, :903-980 */
+/* accelsearch_CPU2's candidate stage (accel_utils.c:573-1047) */
+void optimize_accelcand(accelcand *cand, accelobs *obs);
+void output_fundamentals(fourierprops *props, GSList *list, accelobs *obs, infodata *idata);
+void output_harmonics(GSList *list, accelobs *obs, infodata *idata);
+void calc_props(rderivs data, double r, double z, double w, fourierprops *result); /* characteristics.c:193 */
+/* "value(error)" strings with a one- or two-digit error (output.c:6, :114);
+ * len 0: left-justified, else right-justified in len characters (output
+ * must hold at least max(len, 64) characters) */
+int nice_output_1(char *output, double val, double err, int len);
+int nice_output_2(char *output, double val, double err, int len);
+
 void accelsearch_CPU1(int argc, char *argv[], subharminfo ***subharminfs_ptr, accelobs *obs_ptr,
                       infodata *idata_ptr, Cmdline **cmd_ptr);
 int accelsearch_GPU(accelobs obs, subharminfo **subharminfs, GSList **cands_ptr, Cmdline *cmd);

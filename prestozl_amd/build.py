@@ -19,7 +19,7 @@ ARCH = os.environ.get("PZL_ARCH", "gfx950")
 
 SOURCES = ["pzl_kernels.hip", "pzl_optimize.hip", "pzl_engine.cpp", "pzl_optimize.cpp", "pzl_geometry.cpp",
            "pzl_stats.cpp", "pzl_server.cpp",
-           "accel_api.cpp"]
+           "accel_api.cpp", "accel_output.cpp"]
 GLIB_INC = ["/opt/conda/include/glib-2.0", "/opt/conda/lib/glib-2.0/include"]
 # Headers from the conda tree; the library is the system's libglib-2.0.so.0
 # (found by the loader without an rpath, so no other library of the conda

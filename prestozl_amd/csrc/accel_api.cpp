@@ -8,7 +8,8 @@
 //   search_ffdotpows            accel_utils.c:1507-1555 (GPU threshold, host list)
 //   sort_accelcands, eliminate_harmonics, free/print_accelcand
 //                               accel_utils.c:327-380, :482-571, :1049-1056
-//   accelsearch_CPU1/GPU/CPU2   accelsearch_lib.c:360-980
+//   accelsearch_CPU1/GPU/CPU2   accelsearch_lib.c:360-980 (CPU2's candidate
+//                               stage and outputs: accel_output.cpp)
 // Error behaviour follows the reference: bad inputs print a message and exit;
 // accelsearch_GPU returns 1 when the hit buffer overflows.
 #include <fcntl.h>
@@ -33,6 +34,8 @@
 #include "pzl_kernels.h"
 #include "pzl_math.h"
 #include "pzl_stats.h"
+
+extern "C" int pzl_optimize_accelcands(GSList *list, accelobs *obs);  // accel_output.cpp
 
 namespace {
 
@@ -897,26 +900,50 @@ int accelsearch_GPU(accelobs obs, subharminfo **subharminfs, GSList **cands_ptr,
 
 void accelsearch_CPU2(GSList **cands, accelobs *obs, infodata *idata, Cmdline *cmd)
 {
-    (void)idata;
     int numcands = (int)g_slist_length(*cands);
     printf("numcands after while:%d\n", numcands);
     if (numcands) {
         *cands = sort_accelcands(*cands);
         if (cmd->numharm > 1 && !cmd->noharmremoveP) *cands = eliminate_harmonics(*cands, &numcands);
-        std::string fn = std::string(obs->accelnm) + ".pzlcands";
-        FILE *f = fopen(fn.c_str(), "w");
-        if (f) {
-            fprintf(f, "# sigma  power  numharm  r  z  w  (search-grid values, not optimised)\n");
-            for (GSList *p = *cands; p; p = p->next) {
+        if (pzl_optimize_accelcands(*cands, obs) == 0) {
+            // the fundamentals' properties, with the search-grid errors
+            // (accelsearch_lib.c:932-948), then the text and binary outputs
+            std::vector<fourierprops> props((size_t)numcands);
+            int ii = 0;
+            for (GSList *p = *cands; p; p = p->next, ii++) {
                 const accelcand *c = (const accelcand *)p->data;
-                fprintf(f, "%.6f %.4f %d %.4f %.4f %.4f\n", c->sigma, c->power, c->numharm, c->r, c->z, c->w);
+                calc_props(c->derivs[0], c->r, c->z, c->w, &props[ii]);
+                props[ii].rerr = (float)(PZL_DR) / c->numharm;
+                props[ii].zerr = (float)(PZL_DZ) / c->numharm;
+                props[ii].werr = (float)(PZL_DW) / c->numharm;
             }
+            output_fundamentals(props.data(), *cands, obs, idata);
+            output_harmonics(*cands, obs, idata);
+            FILE *f = fopen(obs->candnm, "wb");
+            if (!f || fwrite(props.data(), sizeof(fourierprops), (size_t)numcands, f) != (size_t)numcands)
+                die(1, "\nCannot write '%s'\n\n", obs->candnm);
             fclose(f);
+            printf("\n\n");
+        } else {
+            // w = 0 searches and -noharmpolish: not optimised here; the
+            // search-grid list is written instead
+            std::string fn = std::string(obs->accelnm) + ".pzlcands";
+            FILE *f = fopen(fn.c_str(), "w");
+            if (f) {
+                fprintf(f, "# sigma  power  numharm  r  z  w  (search-grid values, not optimised)\n");
+                for (GSList *p = *cands; p; p = p->next) {
+                    const accelcand *c = (const accelcand *)p->data;
+                    fprintf(f, "%.6f %.4f %d %.4f %.4f %.4f\n", c->sigma, c->power, c->numharm, c->r, c->z, c->w);
+                }
+                fclose(f);
+            }
+            printf("Search-grid candidates (not optimised) are in '%s'.\n\n", fn.c_str());
         }
-        printf("Search-grid candidates are in '%s'.\n\n", fn.c_str());
     } else {
         printf("No candidates above sigma = %.2f were found.\n\n", obs->sigma);
     }
+    printf("Final candidates in binary format are in '%s'.\n", obs->candnm);
+    printf("Final Candidates in a text format are in '%s'.\n\n", obs->accelnm);
     free_accelobs(obs);
     g_slist_foreach(*cands, free_accelcand, nullptr);
     g_slist_free(*cands);

@@ -294,3 +294,63 @@ def test_driver_end_to_end(tmp_path, config1_data):
            [(c[2], round(c[3], 4), round(c[4], 4), round(c[5], 4)) for c in exp]
     for g, e in zip(got, exp):
         assert g[1] == pytest.approx(e[1], abs=1e-4)
+
+
+FOURIERPROPS = np.dtype([("r", "<f8"), ("rerr", "<f4"), ("z", "<f8"), ("zerr", "<f4"), ("w", "<f8"), ("werr", "<f4")] +
+                        [(k, "<f4") for k in ("pow", "powerr", "sig", "rawpow", "phs", "phserr", "cen", "cenerr",
+                                              "pur", "purerr", "locpow")], align=True)  # presto.h:171-189
+
+
+def _accel_tables(path):
+    """Rows of the fundamentals and harmonics tables of an ACCEL text file."""
+    lines = open(path).read().split("\n")
+    tables, cur, in_table = [], [], False
+    for ln in lines:
+        if ln.startswith("----"):
+            in_table, cur = True, []
+            continue
+        if in_table:
+            if not ln.strip():
+                tables.append(cur)
+                in_table = False
+            else:
+                cur.append(ln)
+    return tables
+
+
+@pytest.mark.gpu
+def test_driver_jerk_search_outputs(tmp_path):
+    """CPU2 of a w > 0 search: each candidate optimised on the GPU (harmonic
+    polishing), its fundamental's Fourier properties in the binary .cand file
+    (fourierprops records) and the ACCEL text tables.  The .cand records equal
+    the oracle's calc_props of its optimisation of the same candidates (the
+    tolerances of tests/test_gpu_optimize.py)."""
+    from prestozl_amd import synth
+    N, dt = 1 << 20, 1e-4
+    sigs = [(0.06, 123.4, 8.0, 30.0), (0.04, 2 * 123.4, 16.0, 60.0)]
+    x = synth.timeseries(N, dt, sigs, seed=21)
+    fft = synth.realfft_packed(x)
+    root = D.setup_files(tmp_path, fft, N, dt, name="jerk")
+    r = subprocess.run([DRIVER, "-zmax", "20", "-wmax", "40", "-numharm", "2", "-sigma", "3.0", root + ".fft"],
+                       capture_output=True, text=True, timeout=300, cwd=str(tmp_path))
+    print(r.stdout[-1500:])
+    assert r.returncode == 0, r.stdout + r.stderr
+    base = root + "_ACCEL_20_JERK_40"
+    props = np.fromfile(base + ".cand", dtype=FOURIERPROPS)
+    assert FOURIERPROPS.itemsize == 88 and len(props) > 0
+    fund, harm = _accel_tables(base)[:2]
+    assert len(fund) == len(props)
+    numharm = [int(row.split()[4]) for row in fund]
+    assert len(harm) == sum(numharm)
+    assert "Freq Deriv" in open(base).read() and "FFT 'w'" in open(base).read()
+    for p, nh in zip(props, numharm):
+        ro, zo, wo, ders, pows = O.max_rzw_arr_harmonics(fft, nh, p["r"], p["z"], p["w"])
+        e = O.calc_props(ders[0], p["r"], p["z"], p["w"])
+        print("cand r %.3f z %.2f w %.2f nh %d: pow %.4f / %.4f  pur %.4f / %.4f" %
+              (p["r"], p["z"], p["w"], nh, p["pow"], e.pow, p["pur"], e.pur))
+        assert (p["rerr"], p["zerr"], p["werr"]) == (np.float32(0.5 / nh), np.float32(2.0 / nh), np.float32(20.0 / nh))
+        for k in ("pow", "rawpow", "sig", "locpow"):
+            assert p[k] == pytest.approx(getattr(e, k), rel=1e-4), k
+        assert abs(p["phs"] - e.phs) < 1e-4
+        assert p["pur"] == pytest.approx(e.pur, rel=3e-2)
+        assert abs(p["cen"] - e.cen) < 1e-3

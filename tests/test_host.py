@@ -120,3 +120,26 @@ def test_candidate_list_out_of_order_overwrite():
         sig = np.round(rng.uniform(1, 6, size=m), 0)
         prod, orac = _insert_both(rr, sig, sig, [1] * m, [0.0] * m, [0.0] * m)
         assert prod == orac
+
+
+@pytest.mark.parametrize("fn,val,err,exp", [
+    # one-digit error (output.c:6): rounded at the error's decade
+    (1, 1.2345, 0.012, " 1.23(1)"),
+    (1, 1234.5, 12.0, " 1.23(1)x10^3"),
+    (1, -0.5, 0.03, "-0.50(3)"),
+    (1, 3.14159, 0.096, " 3.1(1)"),        # error 0.096 rounds to 0.1: one decade up
+    # two-digit error (output.c:114)
+    (2, 1.25, 0.012, " 1.250(12)"),
+    (2, 1234.5, 12.0, " 1235(12)"),
+    (2, 0.5, 0.5, " 0.50(50)"),
+])
+def test_nice_output_known_answers(fn, val, err, exp):
+    """value(error) strings as in scientific journals (hand-worked)."""
+    L = B.lib()
+    f = getattr(L, "nice_output_%d" % fn)
+    f.argtypes = [C.c_char_p, C.c_double, C.c_double, C.c_int]
+    buf = C.create_string_buffer(80)
+    assert f(buf, val, err, 0) == len(exp)
+    assert buf.value.decode() == exp
+    f(buf, val, err, 16)  # right-justified
+    assert buf.value.decode() == exp.rjust(16)
