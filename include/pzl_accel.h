@@ -185,12 +185,12 @@ int pzl_insert_cands(const double *rr, const float *sigma, const float *power, c
  * optimize_accelcand with harmonic polishing (accel_utils.c:573-612):
  * max_rzw_arr_harmonics (maximize_rzw.c:85-191) -- the maximum of the
  * 8x-oversampled F/Fdot/Fdotdot volume around the candidate, summed over its
- * harmonics with each harmonic's local power -- then, at the maximum, the
- * local power and the power/phase derivatives of every harmonic
+ * harmonics with each harmonic's local power -- or, for w = 0 searches,
+ * max_rz_arr_harmonics' downhill simplex -- then, at the maximum, the local
+ * power and the power/phase derivatives of every harmonic
  * (get_localpower3d / get_derivs3d, characteristics.c:75-191).  All
- * candidates are done together on the GPU.  For searches with w > 0 (the
- * reference's w = 0 searches use a simplex optimiser instead,
- * maximize_rz.c:98-220, which these entry points do not cover). */
+ * candidates are done together on the GPU (the simplex steps of all
+ * candidates evaluated in one batch per round). */
 #define PZL_MAX_NUMHARM 32
 typedef struct PzlRDerivs { /* rderivs, presto.h:161-169 */
     double pow, phs, dpow, dphs, d2pow, d2phs, locpow;
@@ -200,7 +200,11 @@ typedef struct PzlOptCand {
      * (file bin minus the spectrum's first bin) */
     double r, z, w;
     int numharm;
-    /* out: the volume maximum (fundamental), and per harmonic ii (number
+    /* 1 for a w = 0 search: the downhill simplex in (r, z) of
+     * max_rz_arr_harmonics (maximize_rz.c:141-220, amoeba.c) instead of the
+     * rzw volume; w is ignored */
+    int fdot_only;
+    /* out: the maximum (fundamental), and per harmonic ii (number
      * ii + 1) its derivatives at (r, z, w) * (ii + 1) and its power */
     double ropt, zopt, wopt;
     PzlRDerivs derivs[PZL_MAX_NUMHARM];

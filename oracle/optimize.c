@@ -367,3 +367,150 @@ int oracle_calc_props(const OracleRDerivs *d, double r, double z, double w, Orac
     res->locpow = (float)d->locpow;
     return 0;
 }
+
+/* ---------------- w = 0 searches: simplex in (r, z / 4) ------------------ */
+#define ZSCALE 4.0
+
+typedef struct {
+    const float *data;
+    int numdata, nh, khw;
+    const float *locpows;
+} RzFunc;
+
+/* power_call_rz_harmonics (maximize_rz.c:99-113): minus the locpow-normalised
+ * harmonic power sum at (r, z / ZSCALE) */
+static double rz_func(const RzFunc *F, const double x[2])
+{
+    double total = 0.0;
+    float ans[2];
+    for (int ii = 0; ii < F->nh; ii++) {
+        const int n = ii + 1;
+        oracle_rz_interp(F->data, F->numdata, x[0] * n, x[1] * ZSCALE * n, F->khw, ans);
+        total += power_d(ans) / F->locpows[ii];
+    }
+    return -total;
+}
+
+/* amotry (amoeba.c:90-106) */
+static double amotry(const RzFunc *F, double p[3][2], double *y, double *psum, int ihi, double fac)
+{
+    const double fac1 = 0.5 * (1.0 - fac), fac2 = fac1 - fac;
+    double ptry[2];
+    ptry[0] = psum[0] * fac1 - p[ihi][0] * fac2;
+    ptry[1] = psum[1] * fac1 - p[ihi][1] * fac2;
+    const double ytry = rz_func(F, ptry);
+    if (ytry < y[ihi]) {
+        y[ihi] = ytry;
+        psum[0] += ptry[0] - p[ihi][0];
+        p[ihi][0] = ptry[0];
+        psum[1] += ptry[1] - p[ihi][1];
+        p[ihi][1] = ptry[1];
+    }
+    return ytry;
+}
+
+/* amoeba (amoeba.c:21-88): the 2-D downhill simplex, at most 5000 evaluations */
+static void amoeba(const RzFunc *F, double p[3][2], double *y, double ftol)
+{
+    int ii, ihi, ilo, inhi, nfunk = 0;
+    double rtol, ysave, ytry, psum[2], t;
+    psum[0] = p[0][0] + p[1][0] + p[2][0];
+    psum[1] = p[0][1] + p[1][1] + p[2][1];
+    for (;;) {
+        ilo = 0;
+        if (y[0] > y[1]) {
+            inhi = 1;
+            ihi = 0;
+        } else {
+            inhi = 0;
+            ihi = 1;
+        }
+        for (ii = 0; ii <= 2; ii++) {
+            if (y[ii] <= y[ilo]) ilo = ii;
+            if (y[ii] > y[ihi]) {
+                inhi = ihi;
+                ihi = ii;
+            } else if (y[ii] > y[inhi] && ii != ihi) {
+                inhi = ii;
+            }
+        }
+        rtol = 2.0 * fabs(y[ihi] - y[ilo]) / (fabs(y[ihi]) + fabs(y[ilo]) + 1.0e-15);
+        if (rtol < ftol) {
+            t = y[0];
+            y[0] = y[ilo];
+            y[ilo] = t;
+            for (ii = 0; ii < 2; ii++) {
+                t = p[0][ii];
+                p[0][ii] = p[ilo][ii];
+                p[ilo][ii] = t;
+            }
+            break;
+        }
+        if (nfunk >= 5000) return;
+        nfunk += 2;
+        ytry = amotry(F, p, y, psum, ihi, -1.0);
+        if (ytry <= y[ilo]) {
+            ytry = amotry(F, p, y, psum, ihi, 2.0);
+        } else if (ytry >= y[inhi]) {
+            ysave = y[ihi];
+            ytry = amotry(F, p, y, psum, ihi, 0.5);
+            if (ytry >= ysave) {
+                for (ii = 0; ii <= 2; ii++) {
+                    if (ii != ilo) {
+                        p[ii][0] = psum[0] = 0.5 * (p[ii][0] + p[ilo][0]);
+                        p[ii][1] = psum[1] = 0.5 * (p[ii][1] + p[ilo][1]);
+                        y[ii] = rz_func(F, psum);
+                    }
+                }
+                nfunk += 2;
+                psum[0] = p[0][0] + p[1][0] + p[2][0];
+                psum[1] = p[0][1] + p[1][1] + p[2][1];
+            }
+        } else {
+            --nfunk;
+        }
+    }
+}
+
+/* max_rz_arr_harmonics (maximize_rz.c:141-220) */
+void oracle_max_rz_arr_harmonics(const float *data, int numdata, int num_harmonics, double rin, double zin,
+                                 double *rout, double *zout, OracleRDerivs *derivs, double *powers)
+{
+    double y[3], x[3][2];
+    const double step = 0.4;
+    float *locpow = (float *)malloc(sizeof(float) * num_harmonics);
+    int ii;
+    for (ii = 0; ii < num_harmonics; ii++) {
+        const int n = ii + 1;
+        locpow[ii] = (float)oracle_get_localpower3d(data, numdata, rin * n, zin * n, 0.0);
+    }
+    RzFunc F = {data, numdata, num_harmonics, oracle_z_resp_halfwidth_acc(fabs(zin * num_harmonics) + 4.0, 0),
+                locpow};
+    x[0][0] = rin - step;
+    x[0][1] = zin / ZSCALE - step;
+    x[1][0] = rin - step;
+    x[1][1] = zin / ZSCALE + step;
+    x[2][0] = rin + step;
+    x[2][1] = zin / ZSCALE;
+    for (ii = 0; ii < 3; ii++) y[ii] = rz_func(&F, x[ii]);
+    amoeba(&F, x, y, 1.0e-7);
+    F.khw = oracle_z_resp_halfwidth_acc(fabs(x[0][1] * num_harmonics) + 4.0, 1);
+    x[1][0] = x[0][0] + 0.01;
+    x[1][1] = x[0][1];
+    x[2][0] = x[0][0];
+    x[2][1] = x[0][1] + 0.01;
+    for (ii = 0; ii < 3; ii++) y[ii] = rz_func(&F, x[ii]);
+    amoeba(&F, x, y, 1.0e-10);
+    *rout = x[0][0];
+    *zout = x[0][1] * ZSCALE;
+    for (ii = 0; ii < num_harmonics; ii++) {
+        const int n = ii + 1;
+        float ans[2];
+        locpow[ii] = (float)oracle_get_localpower3d(data, numdata, *rout * n, *zout * n, 0.0);
+        /* power_call_rz (maximize_rz.c:10-18) at (rout n, zout n): the plain power */
+        oracle_rz_interp(data, numdata, *rout * n, *zout / ZSCALE * n * ZSCALE, F.khw, ans);
+        powers[ii] = power_d(ans);
+        oracle_get_derivs3d(data, numdata, *rout * n, *zout * n, 0.0, locpow[ii], &derivs[ii]);
+    }
+    free(locpow);
+}

@@ -79,6 +79,9 @@ int oracle_corr_rzw_vol(const float *data, int numdata, int numbetween, int star
 void oracle_max_rzw_arr_harmonics(const float *data, int numdata, int num_harmonics, double rin, double zin,
                                   double win, double *rout, double *zout, double *wout, OracleRDerivs *derivs,
                                   double *powers);                 /* maximize_rzw.c:85-191 */
+void oracle_max_rz_arr_harmonics(const float *data, int numdata, int num_harmonics, double rin, double zin,
+                                 double *rout, double *zout, OracleRDerivs *derivs,
+                                 double *powers);                  /* maximize_rz.c:141-220, amoeba.c */
 int oracle_calc_props(const OracleRDerivs *d, double r, double z, double w,
                       OracleFourierProps *res);                     /* characteristics.c:193-265 */
 

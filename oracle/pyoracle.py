@@ -131,6 +131,8 @@ def lib():
                                                    C.c_double, P(C.c_double), P(C.c_double), P(C.c_double),
                                                    P(RDerivs), P(C.c_double)]
         L.oracle_calc_props.argtypes = [P(RDerivs), C.c_double, C.c_double, C.c_double, P(FourierProps)]
+        L.oracle_max_rz_arr_harmonics.argtypes = [P(C.c_float), C.c_int, C.c_int, C.c_double, C.c_double,
+                                                  P(C.c_double), P(C.c_double), P(RDerivs), P(C.c_double)]
         L.oracle_calc_props.restype = C.c_int
         _lib = L
     return _lib
@@ -187,6 +189,16 @@ def max_rzw_arr_harmonics(data, numharm, r, z, w):
     lib().oracle_max_rzw_arr_harmonics(fptr(a), len(a) // 2, numharm, r, z, w, C.byref(ro), C.byref(zo),
                                        C.byref(wo), ders, pows)
     return ro.value, zo.value, wo.value, list(ders), list(pows)
+
+
+def max_rz_arr_harmonics(data, numharm, r, z):
+    """(r, z, [RDerivs per harmonic], powers) of a w = 0 search (maximize_rz.c:141)."""
+    a = _c64(data)
+    ro, zo = C.c_double(), C.c_double()
+    ders = (RDerivs * numharm)()
+    pows = (C.c_double * numharm)()
+    lib().oracle_max_rz_arr_harmonics(fptr(a), len(a) // 2, numharm, r, z, C.byref(ro), C.byref(zo), ders, pows)
+    return ro.value, zo.value, list(ders), list(pows)
 
 
 def calc_props(derivs, r, z, w):

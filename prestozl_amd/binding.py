@@ -72,7 +72,7 @@ class RDerivs(C.Structure):
 
 class OptCand(C.Structure):
     _fields_ = [("r", C.c_double), ("z", C.c_double), ("w", C.c_double), ("numharm", C.c_int),
-                ("ropt", C.c_double), ("zopt", C.c_double), ("wopt", C.c_double),
+                ("fdot_only", C.c_int), ("ropt", C.c_double), ("zopt", C.c_double), ("wopt", C.c_double),
                 ("derivs", RDerivs * MAX_NUMHARM), ("pows", C.c_double * MAX_NUMHARM)]
 
 
@@ -256,10 +256,10 @@ class Plan:
     def set_sum_form(self, order, rpt):
         check(lib().pzl_plan_set_sum_form(self.h, int(order), int(rpt)))
 
-    def optimize(self, cands):
+    def optimize(self, cands, fdot_only=False):
         """optimize_accelcand of (r, z, w, numharm) candidates with the plan's
         resident spectrum (r in bins of it)."""
-        arr = _opt_array(cands)
+        arr = _opt_array(cands, fdot_only)
         check(lib().pzl_optimize(self.h, arr, len(cands)))
         return _opt_result(arr, len(cands))
 
@@ -347,10 +347,11 @@ def tune_batchsize(cfg, fft, candidates=(4, 8, 16), band=64, device=0):
     return best.value, {int(c): ms[i] for i, c in enumerate(candidates)}
 
 
-def _opt_array(cands):
+def _opt_array(cands, fdot_only=False):
     arr = (OptCand * max(len(cands), 1))()
     for i, (r, z, w, nh) in enumerate(cands):
         arr[i].r, arr[i].z, arr[i].w, arr[i].numharm = float(r), float(z), float(w), int(nh)
+        arr[i].fdot_only = int(bool(fdot_only))
     return arr
 
 
@@ -364,13 +365,14 @@ def _opt_result(arr, n):
     return out
 
 
-def optimize_spectrum(fft, cands, device=0):
+def optimize_spectrum(fft, cands, device=0, fdot_only=False):
     """optimize_accelcand (harmonic polishing) of each (r, z, w, numharm), r
-    in bins of 'fft': [{r, z, w, derivs: [...], pows: [...]}, ...]."""
+    in bins of 'fft': [{r, z, w, derivs: [...], pows: [...]}, ...];
+    fdot_only: the candidates of a w = 0 search (simplex in r, z)."""
     a = np.ascontiguousarray(fft)
     if a.dtype == np.complex64:
         a = a.view(np.float32)
-    arr = _opt_array(cands)
+    arr = _opt_array(cands, fdot_only)
     check(lib().pzl_optimize_spectrum(int(device), _fptr(a), len(a) // 2, arr, len(cands)))
     return _opt_result(arr, len(cands))
 

@@ -925,8 +925,8 @@ void accelsearch_CPU2(GSList **cands, accelobs *obs, infodata *idata, Cmdline *c
             fclose(f);
             printf("\n\n");
         } else {
-            // w = 0 searches and -noharmpolish: not optimised here; the
-            // search-grid list is written instead
+            // -noharmpolish: not optimised here; the search-grid list is
+            // written instead
             std::string fn = std::string(obs->accelnm) + ".pzlcands";
             FILE *f = fopen(fn.c_str(), "w");
             if (f) {

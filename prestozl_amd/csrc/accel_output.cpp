@@ -264,11 +264,12 @@ void calc_props(rderivs data, double r, double z, double w, fourierprops *result
 }
 
 // All candidates of a list at once (the loop of accelsearch_lib.c:921-930 as
-// one GPU batch).  Returns 0, or -1 when this path cannot optimise them (w = 0
-// searches, -noharmpolish): the caller then keeps the search-grid values.
+// one GPU batch): the rzw volume for w > 0 searches, the (r, z) simplex for
+// w = 0 ones.  Returns 0, or -1 when this path cannot optimise them
+// (-noharmpolish): the caller then keeps the search-grid values.
 int pzl_optimize_accelcands(GSList *list, accelobs *obs)
 {
-    if (!obs->use_harmonic_polishing || !(obs->mmap_file || obs->dat_input) || obs->numw == 0) return -1;
+    if (!obs->use_harmonic_polishing || !(obs->mmap_file || obs->dat_input)) return -1;
     std::vector<PzlOptCand> oc;
     std::vector<accelcand *> cs;
     for (GSList *p = list; p; p = p->next) {
@@ -279,6 +280,7 @@ int pzl_optimize_accelcands(GSList *list, accelobs *obs)
         o.z = c->z;
         o.w = c->w;
         o.numharm = c->numharm;
+        o.fdot_only = obs->numw == 0;
         oc.push_back(o);
         cs.push_back(c);
     }
@@ -311,7 +313,7 @@ void optimize_accelcand(accelcand *cand, accelobs *obs)
     one.data = cand;
     one.next = nullptr;
     if (pzl_optimize_accelcands(&one, obs) != 0) {
-        printf("\noptimize_accelcand: only harmonic polishing of w > 0 searches is done on the GPU\n\n");
+        printf("\noptimize_accelcand: only harmonic polishing is done on the GPU (not -noharmpolish)\n\n");
         exit(1);
     }
 }

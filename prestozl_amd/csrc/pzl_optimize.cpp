@@ -327,6 +327,276 @@ int optimize_chunk(OptBuffers &B, const float2 *d_data, int numdata, PzlOptCand 
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// w = 0 searches: max_rz_arr_harmonics (maximize_rz.c:141-220) with the
+// downhill simplex of amoeba.c:21-106, one resumable state machine per
+// candidate; every round evaluates the pending points of all candidates in
+// one batch of interpolations.
+// ---------------------------------------------------------------------------
+
+constexpr double kZScale = 4.0;  // maximize_rz.c:2
+
+struct Simplex {
+    double p[3][2], y[3], psum[2];
+    double ptry[2], ysave = 0.0, ftol = 0.0;
+    int nfunk = 0, ihi = 0, ilo = 0, inhi = 0;
+    enum { kInit1, kRun1, kInit2, kRun2, kDone } phase = kInit1;
+    enum { kTop, kReflect, kExpand, kContract, kShrink } step = kTop;
+    int npend = 0;
+    double pend[3][2];
+    int pend_idx[3];
+    int khw = 0, nh = 1;
+    std::vector<float> locpow;
+
+    void request(const double *pt, int idx)
+    {
+        pend[npend][0] = pt[0];
+        pend[npend][1] = pt[1];
+        pend_idx[npend++] = idx;
+    }
+    // amotry's trial point (amoeba.c:95-98)
+    void try_point(double fac)
+    {
+        const double fac1 = 0.5 * (1.0 - fac), fac2 = fac1 - fac;
+        ptry[0] = psum[0] * fac1 - p[ihi][0] * fac2;
+        ptry[1] = psum[1] * fac1 - p[ihi][1] * fac2;
+        request(ptry, -1);
+    }
+    // amotry's update (amoeba.c:99-105)
+    void accept(double ytry)
+    {
+        if (ytry < y[ihi]) {
+            y[ihi] = ytry;
+            psum[0] += ptry[0] - p[ihi][0];
+            p[ihi][0] = ptry[0];
+            psum[1] += ptry[1] - p[ihi][1];
+            p[ihi][1] = ptry[1];
+        }
+    }
+    void start_amoeba(double tol)
+    {
+        ftol = tol;
+        nfunk = 0;
+        psum[0] = p[0][0] + p[1][0] + p[2][0];
+        psum[1] = p[0][1] + p[1][1] + p[2][1];
+        top();
+    }
+    // the head of amoeba's loop (amoeba.c:30-62)
+    void top()
+    {
+        ilo = 0;
+        if (y[0] > y[1]) {
+            inhi = 1;
+            ihi = 0;
+        } else {
+            inhi = 0;
+            ihi = 1;
+        }
+        for (int ii = 0; ii <= 2; ii++) {
+            if (y[ii] <= y[ilo]) ilo = ii;
+            if (y[ii] > y[ihi]) {
+                inhi = ihi;
+                ihi = ii;
+            } else if (y[ii] > y[inhi] && ii != ihi) {
+                inhi = ii;
+            }
+        }
+        const double rtol = 2.0 * fabs(y[ihi] - y[ilo]) / (fabs(y[ihi]) + fabs(y[ilo]) + 1.0e-15);
+        if (rtol < ftol) {
+            std::swap(y[0], y[ilo]);
+            std::swap(p[0][0], p[ilo][0]);
+            std::swap(p[0][1], p[ilo][1]);
+            end_amoeba();
+            return;
+        }
+        if (nfunk >= 5000) {
+            end_amoeba();
+            return;
+        }
+        nfunk += 2;
+        step = kReflect;
+        try_point(-1.0);
+    }
+    void end_amoeba()
+    {
+        if (phase == kRun1) {
+            // restart at the minimum at HIGHACC (maximize_rz.c:183-199)
+            khw = pzl_z_resp_halfwidth_acc(fabs(p[0][1] * nh) + 4.0, true);
+            p[1][0] = p[0][0] + 0.01;
+            p[1][1] = p[0][1];
+            p[2][0] = p[0][0];
+            p[2][1] = p[0][1] + 0.01;
+            phase = kInit2;
+            for (int ii = 0; ii < 3; ii++) request(p[ii], ii);
+        } else {
+            phase = kDone;
+        }
+    }
+    // the answers to the pending requests, in request order
+    void deliver(const double *vals)
+    {
+        const int n = npend;
+        npend = 0;
+        if (phase == kInit1 || phase == kInit2) {
+            for (int k = 0; k < n; k++) y[pend_idx[k]] = vals[k];
+            phase = phase == kInit1 ? kRun1 : kRun2;
+            start_amoeba(phase == kRun1 ? 1.0e-7 : 1.0e-10);
+            return;
+        }
+        switch (step) {
+        case kReflect: {
+            const double ytry = vals[0];
+            accept(ytry);
+            if (ytry <= y[ilo]) {
+                step = kExpand;
+                try_point(2.0);
+            } else if (ytry >= y[inhi]) {
+                ysave = y[ihi];
+                step = kContract;
+                try_point(0.5);
+            } else {
+                --nfunk;
+                top();
+            }
+            break;
+        }
+        case kExpand:
+            accept(vals[0]);
+            top();
+            break;
+        case kContract: {
+            const double ytry = vals[0];
+            accept(ytry);
+            if (ytry >= ysave) {
+                // shrink towards the best point (amoeba.c:72-83)
+                step = kShrink;
+                for (int ii = 0; ii <= 2; ii++)
+                    if (ii != ilo) {
+                        p[ii][0] = 0.5 * (p[ii][0] + p[ilo][0]);
+                        p[ii][1] = 0.5 * (p[ii][1] + p[ilo][1]);
+                        request(p[ii], ii);
+                    }
+            } else {
+                top();
+            }
+            break;
+        }
+        case kShrink:
+            for (int k = 0; k < n; k++) y[pend_idx[k]] = vals[k];
+            nfunk += 2;
+            psum[0] = p[0][0] + p[1][0] + p[2][0];
+            psum[1] = p[0][1] + p[1][1] + p[2][1];
+            top();
+            break;
+        default:
+            break;
+        }
+    }
+};
+
+// power_call_rz_harmonics (maximize_rz.c:99-113) as interpolation jobs
+void rz_point_jobs(const Simplex &S, const double *pt, std::vector<PzlInterpJob> &jobs)
+{
+    for (int ii = 0; ii < S.nh; ii++) {
+        const int n = ii + 1;
+        jobs.push_back({pt[0] * n, pt[1] * kZScale * n, 0.0, S.khw, 0});
+    }
+}
+
+int optimize_rz(OptBuffers &B, const float2 *d_data, int numdata, PzlOptCand *cands, long long nc, hipStream_t st)
+{
+    std::vector<Simplex> S((size_t)nc);
+    std::vector<PzlInterpJob> jobs;
+    std::vector<float2> ans;
+    std::vector<int> l0, ln;
+    // local powers at the start (maximize_rz.c:157-161)
+    for (long long c = 0; c < nc; c++)
+        for (int ii = 0; ii < cands[c].numharm; ii++) {
+            const int n = ii + 1;
+            l0.push_back((int)jobs.size());
+            ln.push_back(localpower_jobs(cands[c].r * n, cands[c].z * n, 0.0, numdata, jobs));
+        }
+    int rc = check_interp_jobs(jobs, 0);
+    if (rc) return rc;
+    rc = run_interp(B, d_data, numdata, jobs, ans, st);
+    if (rc) return rc;
+    size_t k = 0;
+    for (long long c = 0; c < nc; c++) {
+        Simplex &X = S[c];
+        const PzlOptCand &C = cands[c];
+        X.nh = C.numharm;
+        for (int ii = 0; ii < C.numharm; ii++, k++) X.locpow.push_back((float)localpower_from(ans.data() + l0[k], ln[k]));
+        X.khw = pzl_z_resp_halfwidth_acc(fabs(C.z * C.numharm) + 4.0, false);
+        const double step = 0.4;  // maximize_rz.c:165-172
+        const double init[3][2] = {{C.r - step, C.z / kZScale - step},
+                                   {C.r - step, C.z / kZScale + step},
+                                   {C.r + step, C.z / kZScale}};
+        for (int ii = 0; ii < 3; ii++) {
+            X.p[ii][0] = init[ii][0];
+            X.p[ii][1] = init[ii][1];
+            X.request(X.p[ii], ii);
+        }
+    }
+    // rounds: every pending point of every candidate in one batch
+    for (;;) {
+        jobs.clear();
+        std::vector<std::pair<long long, int>> owner;  // (candidate, pending slot) of each point
+        for (long long c = 0; c < nc; c++)
+            for (int q = 0; q < S[c].npend; q++) {
+                owner.push_back({c, q});
+                rz_point_jobs(S[c], S[c].pend[q], jobs);
+            }
+        if (owner.empty()) break;
+        rc = check_interp_jobs(jobs, 0);
+        if (rc) return rc;
+        rc = run_interp(B, d_data, numdata, jobs, ans, st);
+        if (rc) return rc;
+        std::vector<double> vals(3 * (size_t)nc, 0.0);
+        size_t j = 0;
+        for (auto &o : owner) {
+            const Simplex &X = S[o.first];
+            double total = 0.0;
+            for (int ii = 0; ii < X.nh; ii++, j++) total += power_d(ans[j]) / X.locpow[ii];
+            vals[3 * o.first + o.second] = -total;
+        }
+        for (long long c = 0; c < nc; c++)
+            if (S[c].npend) S[c].deliver(vals.data() + 3 * c);
+    }
+    // the peak (maximize_rz.c:207-219): local power, power and derivatives per harmonic
+    std::vector<PzlInterpJob> dj;
+    std::vector<int> dl0, dln, dp0, dd0;
+    for (long long c = 0; c < nc; c++) {
+        PzlOptCand &C = cands[c];
+        const Simplex &X = S[c];
+        C.ropt = X.p[0][0];
+        C.zopt = X.p[0][1] * kZScale;
+        C.wopt = 0.0;
+        for (int ii = 0; ii < C.numharm; ii++) {
+            const int n = ii + 1;
+            dl0.push_back((int)dj.size());
+            dln.push_back(localpower_jobs(C.ropt * n, C.zopt * n, 0.0, numdata, dj));
+            dp0.push_back((int)dj.size());
+            dj.push_back({C.ropt * n, C.zopt / kZScale * n * kZScale, 0.0, X.khw, 0});
+            dd0.push_back((int)dj.size());
+            derivs_jobs(C.ropt * n, C.zopt * n, 0.0, dj);
+        }
+    }
+    rc = check_interp_jobs(dj, 0);
+    if (rc) return rc;
+    rc = run_interp(B, d_data, numdata, dj, ans, st);
+    if (rc) return rc;
+    k = 0;
+    for (long long c = 0; c < nc; c++) {
+        PzlOptCand &C = cands[c];
+        for (int ii = 0; ii < C.numharm; ii++, k++) {
+            const float lp = (float)localpower_from(ans.data() + dl0[k], dln[k]);
+            C.pows[ii] = power_d(ans[dp0[k]]);
+            C.derivs[ii] = derivs_from(ans.data() + dd0[k], lp);
+        }
+    }
+    return 0;
+}
+
 int optimize_on(const float2 *d_data, long long numbins, PzlOptCand *cands, long long n, hipStream_t st)
 {
     if (numbins <= 0 || numbins > INT_MAX) return fail(PZL_ERR_ARG, "spectrum of %lld bins", numbins);
@@ -337,14 +607,36 @@ int optimize_on(const float2 *d_data, long long numbins, PzlOptCand *cands, long
             return fail(PZL_ERR_ARG, "candidate %lld: r %.3f, numharm %d", c, C.r, C.numharm);
     }
     OptBuffers B;
+    // w = 0 candidates (the simplex) and the rest (the rzw volume), each as one batch
+    std::vector<PzlOptCand> rz;
+    std::vector<long long> rz_at;
+    for (long long c = 0; c < n; c++)
+        if (cands[c].fdot_only) {
+            rz.push_back(cands[c]);
+            rz_at.push_back(c);
+        }
+    if (!rz.empty()) {
+        const int rc = optimize_rz(B, d_data, (int)numbins, rz.data(), (long long)rz.size(), st);
+        if (rc) return rc;
+        for (size_t k = 0; k < rz.size(); k++) cands[rz_at[k]] = rz[k];
+    }
+    std::vector<PzlOptCand> vw;
+    std::vector<long long> vw_at;
+    for (long long c = 0; c < n; c++)
+        if (!cands[c].fdot_only) {
+            vw.push_back(cands[c]);
+            vw_at.push_back(c);
+        }
+    const long long nv = (long long)vw.size();
     long long c0 = 0;
-    while (c0 < n) {
+    while (c0 < nv) {
         long long c1 = c0, jobs = 0;
-        while (c1 < n && (c1 == c0 || jobs + cands[c1].numharm <= kChunkJobs)) jobs += cands[c1++].numharm;
-        const int rc = optimize_chunk(B, d_data, (int)numbins, cands, c0, c1, st);
+        while (c1 < nv && (c1 == c0 || jobs + vw[c1].numharm <= kChunkJobs)) jobs += vw[c1++].numharm;
+        const int rc = optimize_chunk(B, d_data, (int)numbins, vw.data(), c0, c1, st);
         if (rc) return rc;
         c0 = c1;
     }
+    for (long long k = 0; k < nv; k++) cands[vw_at[k]] = vw[k];
     return 0;
 }
 

@@ -261,41 +261,6 @@ def test_presto_loop_through_plane_api(tmp_path, config1_data):
     L.free_accelobs(C.byref(obs))
 
 
-def read_pzlcands(path):
-    out = []
-    with open(path) as f:
-        for line in f:
-            if line.startswith("#"):
-                continue
-            s, p, nh, r, z, w = line.split()
-            out.append((float(p), float(s), int(nh), float(r), float(z), float(w)))
-    return out
-
-
-@pytest.mark.gpu
-def test_driver_end_to_end(tmp_path, config1_data):
-    """The accelsearch driver (CPU1 -> GPU -> CPU2) on a config-1 file: its
-    sorted, harmonic-filtered list equals the oracle's."""
-    fft, N, dt, opts = config1_data
-    root = D.setup_files(tmp_path, fft, N, dt)
-    r = subprocess.run([DRIVER, "-zmax", "20", "-wmax", "0", "-numharm", "4", "-sigma", "2.0", root + ".fft"],
-                       capture_output=True, text=True, timeout=300, cwd=str(tmp_path))
-    print(r.stdout[-2000:])
-    assert r.returncode == 0, r.stdout + r.stderr
-    got = read_pzlcands(root + "_ACCEL_20.pzlcands")  # wmax 0: numw = 0, no _JERK_ (accel_utils.c:1873-1893)
-    ref_obs = O.create_accelobs(O.make_params(N, dt, len(fft), 20, 0, 4, 2.0, wmaxP=True))
-    ctx = O.Ctx(ref_obs, O.FP32, nthreads=8)
-    exp, _hits, _ = ctx.search(fft)
-    ctx.close()
-    exp = O.eliminate_harmonics(O.sort_cands(exp))
-    print("driver: %d candidates, oracle %d" % (len(got), len(exp)))
-    assert len(got) == len(exp)
-    assert [(c[2], round(c[3], 4), round(c[4], 4), round(c[5], 4)) for c in got] == \
-           [(c[2], round(c[3], 4), round(c[4], 4), round(c[5], 4)) for c in exp]
-    for g, e in zip(got, exp):
-        assert g[1] == pytest.approx(e[1], abs=1e-4)
-
-
 FOURIERPROPS = np.dtype([("r", "<f8"), ("rerr", "<f4"), ("z", "<f8"), ("zerr", "<f4"), ("w", "<f8"), ("werr", "<f4")] +
                         [(k, "<f4") for k in ("pow", "powerr", "sig", "rawpow", "phs", "phserr", "cen", "cenerr",
                                               "pur", "purerr", "locpow")], align=True)  # presto.h:171-189
@@ -316,6 +281,42 @@ def _accel_tables(path):
             else:
                 cur.append(ln)
     return tables
+
+
+@pytest.mark.gpu
+def test_driver_end_to_end(tmp_path, config1_data):
+    """The accelsearch driver (CPU1 -> GPU -> CPU2) on a config-1 file (w = 0):
+    its sorted, harmonic-filtered list equals the oracle's, and every
+    candidate's .cand record holds the fundamental's properties after the
+    (r, z) simplex optimisation (maximize_rz.c:141-220), as the oracle's."""
+    fft, N, dt, opts = config1_data
+    root = D.setup_files(tmp_path, fft, N, dt)
+    r = subprocess.run([DRIVER, "-zmax", "20", "-wmax", "0", "-numharm", "4", "-sigma", "2.0", root + ".fft"],
+                       capture_output=True, text=True, timeout=300, cwd=str(tmp_path))
+    print(r.stdout[-2000:])
+    assert r.returncode == 0, r.stdout + r.stderr
+    base = root + "_ACCEL_20"  # wmax 0: numw = 0, no _JERK_ (accel_utils.c:1873-1893)
+    props = np.fromfile(base + ".cand", dtype=FOURIERPROPS)
+    fund, harm = _accel_tables(base)[:2]
+    numharm = [int(row.split()[4]) for row in fund]
+    assert "FFT 'w'" not in open(base).read()  # no jerk columns
+    ref_obs = O.create_accelobs(O.make_params(N, dt, len(fft), 20, 0, 4, 2.0, wmaxP=True))
+    ctx = O.Ctx(ref_obs, O.FP32, nthreads=8)
+    exp, _hits, _ = ctx.search(fft)
+    ctx.close()
+    exp = O.eliminate_harmonics(O.sort_cands(exp))
+    print("driver: %d candidates, oracle %d" % (len(props), len(exp)))
+    assert len(props) == len(exp) == len(fund) and len(harm) == sum(numharm)
+    assert [(h, round(p["r"], 4), round(p["z"], 4), round(p["w"], 4)) for h, p in zip(numharm, props)] == \
+           [(c[2], round(c[3], 4), round(c[4], 4), round(c[5], 4)) for c in exp]
+    for p, e in zip(props, exp):
+        ro, zo, ders, pows = O.max_rz_arr_harmonics(fft, e[2], e[3], e[4])
+        q = O.calc_props(ders[0], e[3], e[4], e[5])
+        # the two simplexes stop at optima that agree to the flatness of the
+        # summed peak (test_gpu_optimize.py); a noise-dominated fundamental's
+        # power moves by up to ~1e-3 over that distance
+        assert p["pow"] == pytest.approx(q.pow, rel=5e-3) and p["sig"] == pytest.approx(q.sig, rel=5e-3)
+        assert (p["rerr"], p["zerr"]) == (np.float32(0.5 / e[2]), np.float32(2.0 / e[2]))
 
 
 @pytest.mark.gpu

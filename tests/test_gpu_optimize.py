@@ -91,3 +91,38 @@ def test_optimize_rejects_bad_candidates():
         B.optimize_spectrum(data, [(100.0, 0.0, 10.0, 0)])
     with pytest.raises(B.PzlError):
         B.optimize_spectrum(data, [(100.0, 0.0, 10.0, 40)])
+
+
+@pytest.mark.parametrize("numharm", [1, 3])
+def test_optimize_fdot_only_matches_oracle(numharm):
+    """w = 0 searches: the downhill simplex in (r, z) (maximize_rz.c:141-220,
+    amoeba.c), every candidate's steps batched on the GPU.  Both sides stop at
+    a relative simplex spread of 1e-10 on function values that differ at
+    ~1e-6, so their optima agree to the flatness of the peak: r within 2e-3
+    bins, z within 2e-2; powers 1e-4; local powers 1e-2; phases 1e-2 rad; first derivatives
+    within 1e-2 of the power (and of a radian), second within 2% + 0.05 pow
+    (and 0.5 rad)."""
+    from prestozl_amd import binding as B
+    f0, z0 = 2210.3, 6.7
+    comps = [(1.0 / (1 + 0.5 * k), f0 * (k + 1), z0 * (k + 1), 0.0) for k in range(numharm)]
+    data = _chirp(comps, noise=0.3)
+    r, z = f0 + 0.5 * z0, z0
+    starts = [(np.round(r * numharm * 2) / 2 / numharm + dr, np.round(z * numharm / 2) * 2 / numharm, 0.0, numharm)
+              for dr in (0.0, 0.5 / numharm)]
+    got = B.optimize_spectrum(data, starts, fdot_only=True)
+    for s, g in zip(starts, got):
+        ro, zo, ders, pows = O.max_rz_arr_harmonics(data, numharm, s[0], s[1])
+        print("numharm %d start %s -> gpu (%.5f %.4f) oracle (%.5f %.4f)" % (numharm, s[:2], g["r"], g["z"], ro, zo))
+        assert abs(g["r"] - ro) < 2e-3 and abs(g["z"] - zo) < 2e-2 and g["w"] == 0.0
+        for h, (dg, do) in enumerate(zip(g["derivs"], ders)):
+            p = do.pow
+            assert dg["pow"] == pytest.approx(p, rel=1e-4), h
+            assert g["pows"][h] == pytest.approx(pows[h], rel=1e-4), h
+            # the 20 local-power points sit on this strong harmonic's own
+            # sidelobes, whose power changes by ~40 per bin near their nulls
+            assert dg["locpow"] == pytest.approx(do.locpow, rel=1e-2), h
+            assert abs(dg["phs"] - do.phs) < 1e-2, h
+            assert abs(dg["dpow"] - do.dpow) <= 1e-2 * p, h
+            assert abs(dg["d2pow"] - do.d2pow) <= 0.02 * abs(do.d2pow) + 0.05 * p, h
+            assert abs(dg["dphs"] - do.dphs) <= 1e-2, h
+            assert abs(dg["d2phs"] - do.d2phs) <= 0.5, h

@@ -127,3 +127,26 @@ def test_calc_props_refuses_zero_power():
     d = O.RDerivs(0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 1.0)
     with pytest.raises(ValueError):
         O.calc_props(d, 1.0, 0.0, 0.0)
+
+
+@pytest.mark.parametrize("numharm", [1, 3])
+def test_rz_simplex_finds_injection(numharm):
+    """w = 0 searches optimise with a downhill simplex (maximize_rz.c:141,
+    amoeba.c): continuous, so it lands on the injected (r, z) far inside the
+    search grid's steps; the power there is the coherent power (N A / 2)^2."""
+    f0, z0 = 2210.3, 6.7
+    comps = [(1.0 / (1 + 0.5 * k), f0 * (k + 1), z0 * (k + 1), 0.0) for k in range(numharm)]
+    data = chirp(comps, noise=0.3)
+    r, z = f0 + 0.5 * z0, z0
+    rs = np.round(r * numharm * 2) / 2 / numharm
+    zs = np.round(z * numharm / 2) * 2 / numharm
+    ro, zo, ders, pows = O.max_rz_arr_harmonics(data, numharm, rs, zs)
+    p = O.calc_props(ders[0], ro, zo, 0.0)
+    # within the statistical errors (Middleditch et al.; these are pessimistic
+    # here: the local power holds the strong signal's own sidelobes) and far
+    # inside the search grid's steps
+    assert abs(ro - r) < min(3 * p.rerr, 2e-3)
+    assert abs(zo - z) < min(3 * p.zerr, 2e-2)
+    assert pows[0] == pytest.approx((N / 2) ** 2, rel=2e-2)
+    assert ders[0].pow == pytest.approx(pows[0], rel=1e-2)
+    assert p.pur == pytest.approx(1.0, abs=0.01)
