@@ -48,6 +48,21 @@ def cpu_baseline(fft, kw, rsteps, nthreads, cells_per_rstep, start):
                       "spectrum (%.1f s), kernel-bank build excluded as on the GPU" % (start, start + rsteps, dt)}
 
 
+def cpu_optimize_sample(fft, cands, ncand=8):
+    """The oracle's optimize_accelcand (harmonic polishing, one core) on the
+    first 'ncand' candidates: seconds per candidate and mean harmonics."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+    sample = cands[:ncand]
+    t0 = time.perf_counter()
+    for r, z, w, nh in sample:
+        O.max_rzw_arr_harmonics(fft, nh, r, z, w)
+    dt = time.perf_counter() - t0
+    return {"ms_per_candidate": dt * 1e3 / max(len(sample), 1), "cores": 1, "kind": "port",
+            "sample": "oracle max_rzw_arr_harmonics + derivatives on the first %d candidates "
+                      "(mean %.1f harmonics)" % (len(sample), np.mean([c[3] for c in sample]) if sample else 0)}
+
+
 def copy_calibrate(mib):
     """A plain copy kernel of known size: under rocprofv3 --pmc its FETCH_SIZE
     and WRITE_SIZE per dispatch give the counters' scale (bytes moved / KiB read)."""
@@ -304,6 +319,19 @@ def main():
         "candidates": len(cands),
         "hits": stats_all[0]["nhits"],
     }
+    # accelsearch_CPU2's candidate optimisation (harmonic polishing) of every
+    # candidate of the search, one GPU batch
+    opt_cands = [(c[3], c[4], c[5], c[2]) for c in cands]  # (r, z, w, numharm)
+    if not args.rsteps and opt_cands:
+        plan.optimize(opt_cands[:4])  # warm
+        ts = time.perf_counter()
+        plan.optimize(opt_cands)
+        topt = time.perf_counter() - ts
+        out["cpu2_optimize"] = {"candidates": len(opt_cands), "ms": topt * 1e3,
+                                "candidates_per_s": len(opt_cands) / topt,
+                                "mean_numharm": float(np.mean([c[3] for c in opt_cands])),
+                                "note": "optimize_accelcand with harmonic polishing of every search candidate "
+                                        "(a superset of CPU2's harmonic-filtered list)"}
     # variants of the execution: graph replay of every full batch (on top of the
     # default overlap), and everything in one stream without overlap
     if args.no_legs:
@@ -359,6 +387,10 @@ def main():
         out["cpu_baseline"] = cpu_baseline(fft, kw, args.cpu_rsteps, nthreads,
                                            g.corr_uselen * g.numz * max(g.numw, 1), g.nrsteps // 2)
         out["cpu_baseline"]["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
+        if "cpu2_optimize" in out:
+            out["cpu_baseline"]["cpu2_optimize"] = cpu_optimize_sample(fft, opt_cands)
+            out["cpu2_optimize"]["gpu_over_cpu"] = (out["cpu_baseline"]["cpu2_optimize"]["ms_per_candidate"] *
+                                                   out["cpu2_optimize"]["candidates_per_s"] / 1e3)
     print(json.dumps(out))
 
 
