@@ -123,7 +123,8 @@ struct OptBuffers {
     float2 *ans = nullptr;
     size_t njobs_cap = 0;
     PzlVolJob *vjobs = nullptr;
-    double2 *dfft = nullptr;
+    double2 *tw = nullptr;
+    int *err = nullptr;
     float *powv = nullptr, *tail = nullptr, *locpow = nullptr;
     int *job0 = nullptr, *nharm = nullptr, *best = nullptr;
     size_t nv_cap = 0, nc_cap = 0;
@@ -132,7 +133,8 @@ struct OptBuffers {
         (void)hipFree(jobs);
         (void)hipFree(ans);
         (void)hipFree(vjobs);
-        (void)hipFree(dfft);
+        (void)hipFree(tw);
+        (void)hipFree(err);
         (void)hipFree(powv);
         (void)hipFree(tail);
         (void)hipFree(locpow);
@@ -153,22 +155,24 @@ struct OptBuffers {
         njobs_cap = n;
         return 0;
     }
-    int volume(size_t nv, size_t nc)
+    int volume(size_t nv, size_t nc, hipStream_t st)
     {
+        if (!tw) {
+            PZL_HIPCHK(hipMalloc(&err, sizeof(int)));
+            PZL_HIPCHK(hipMalloc(&tw, sizeof(double2) * PZL_OPT_TWN));
+            PZL_HIPCHK(pzl_launch_opt_twiddles(tw, st));
+        }
         if (nv > nv_cap) {
             (void)hipFree(vjobs);
-            (void)hipFree(dfft);
             (void)hipFree(powv);
             (void)hipFree(tail);
             (void)hipFree(locpow);
             vjobs = nullptr;
-            dfft = nullptr;
             powv = nullptr;
             tail = nullptr;
             locpow = nullptr;
             nv_cap = 0;
             PZL_HIPCHK(hipMalloc(&vjobs, sizeof(PzlVolJob) * nv));
-            PZL_HIPCHK(hipMalloc(&dfft, sizeof(double2) * PZL_OPT_MAX_FFTLEN * nv));
             PZL_HIPCHK(hipMalloc(&powv, sizeof(float) * kVolCells * nv));
             PZL_HIPCHK(hipMalloc(&tail, sizeof(float) * PZL_OPT_NUMZ * PZL_OPT_NUMW * PZL_OPT_TAIL * nv));
             PZL_HIPCHK(hipMalloc(&locpow, sizeof(float) * nv));
@@ -214,6 +218,7 @@ int optimize_chunk(OptBuffers &B, const float2 *d_data, int numdata, PzlOptCand 
     std::vector<long> hhlobin((size_t)nc);
     std::vector<PzlInterpJob> lj;
     std::vector<int> lj0, ljn;
+    int ldsn = 0;  // LDS of a volume row: its numkern and w-response chirp samples (double2)
     for (long long c = 0; c < nc; c++) {
         const PzlOptCand &C = cands[c0 + c];
         const int H = C.numharm;
@@ -259,23 +264,28 @@ int optimize_chunk(OptBuffers &B, const float2 *d_data, int numdata, PzlOptCand 
             if (J.fftlen > PZL_OPT_MAX_FFTLEN || numkern > J.fftlen || lobin > INT_MAX)
                 return fail(PZL_ERR_UNSUPPORTED, "candidate at r %.3f z %.2f w %.2f: harmonic %d needs an FFT of %d",
                             C.r, C.z, C.w, n, J.fftlen);
+            ldsn = std::max(ldsn, numkern);
             for (int i = 0; i < PZL_OPT_NUMW; i++)
-                for (int j = 0; j < PZL_OPT_NUMZ; j++)
-                    if (w_response_npts(zlo + j * J.dz, wlo + i * J.dw, numkern, PZL_OPT_NB) > 4 * J.fftlen)
-                        return fail(PZL_ERR_UNSUPPORTED, "response samples of harmonic %d exceed its FFT buffer", n);
+                for (int j = 0; j < PZL_OPT_NUMZ; j++) {
+                    const int npts = w_response_npts(zlo + j * J.dz, wlo + i * J.dw, numkern, PZL_OPT_NB);
+                    if (npts > PZL_OPT_RESP_FFT_MAX)
+                        return fail(PZL_ERR_UNSUPPORTED,
+                                    "candidate at r %.3f z %.2f w %.2f: harmonic %d needs a w response of %d samples",
+                                    C.r, C.z, C.w, n, npts);
+                    ldsn = std::max(ldsn, npts);
+                }
             vj.push_back(J);
         }
     }
     int rc = check_interp_jobs(lj, 0);
     if (rc) return rc;
     const size_t nv = vj.size();
-    rc = B.volume(nv, (size_t)nc);
+    rc = B.volume(nv, (size_t)nc, st);
     if (rc) return rc;
-    int max_fftlen = 0;
-    for (auto &J : vj) max_fftlen = std::max(max_fftlen, J.fftlen);
     // volumes (independent of the local powers, which only scale the sum)
     PZL_HIPCHK(hipMemcpyAsync(B.vjobs, vj.data(), sizeof(PzlVolJob) * nv, hipMemcpyHostToDevice, st));
-    PZL_HIPCHK(pzl_launch_opt_volume(d_data, numdata, B.vjobs, (int)nv, max_fftlen, B.dfft, B.powv, B.tail, st));
+    PZL_HIPCHK(hipMemsetAsync(B.err, 0, sizeof(int), st));
+    PZL_HIPCHK(pzl_launch_opt_volume(d_data, numdata, B.vjobs, (int)nv, ldsn, B.tw, B.powv, B.tail, B.err, st));
     std::vector<float2> ans;
     rc = run_interp(B, d_data, numdata, lj, ans, st);
     if (rc) return rc;
@@ -287,7 +297,10 @@ int optimize_chunk(OptBuffers &B, const float2 *d_data, int numdata, PzlOptCand 
     PZL_HIPCHK(hipMemcpyAsync(B.nharm, nharm.data(), sizeof(int) * nc, hipMemcpyHostToDevice, st));
     PZL_HIPCHK(pzl_launch_opt_sum(B.powv, B.tail, B.vjobs, B.locpow, B.job0, B.nharm, (int)nc, B.best, st));
     PZL_HIPCHK(hipMemcpyAsync(best.data(), B.best, sizeof(int) * nc, hipMemcpyDeviceToHost, st));
+    int verr = 0;
+    PZL_HIPCHK(hipMemcpyAsync(&verr, B.err, sizeof(int), hipMemcpyDeviceToHost, st));
     PZL_HIPCHK(hipStreamSynchronize(st));
+    if (verr) return fail(PZL_ERR_UNSUPPORTED, "a volume row's response exceeded the launch's LDS");
 
     // the maximum (maximize_rzw.c:177-184), then every harmonic's local power
     // and derivatives there (:186-193)

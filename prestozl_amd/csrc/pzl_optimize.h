@@ -12,6 +12,8 @@
 #define PZL_OPT_NUMW 17
 #define PZL_OPT_MAX_FFTLEN 4096
 #define PZL_OPT_TAIL 8       // last powers of each volume row kept for negative r indices
+#define PZL_OPT_RESP_FFT_MAX 4096  // chirp samples of a volume row's w response (and numkern)
+#define PZL_OPT_TWN 65536    // twiddle table length (multiple of 8 PZL_OPT_RESP_FFT_MAX)
 
 // One Fourier interpolation at (r, z, w) with kernel half-width khw.
 struct PzlInterpJob {
@@ -34,8 +36,13 @@ extern "C" {
 #endif
 hipError_t pzl_launch_opt_interp(const float2 *data, int numdata, const PzlInterpJob *jobs, int njobs,
                                  float2 *ans, hipStream_t st);
-hipError_t pzl_launch_opt_volume(const float2 *data, int numdata, const PzlVolJob *jobs, int njobs,
-                                 int max_fftlen, double2 *dfft, float *powv, float *tail, hipStream_t st);
+// tw[j] = exp(-2 pi i j / PZL_OPT_TWN), j < PZL_OPT_TWN
+hipError_t pzl_launch_opt_twiddles(double2 *tw, hipStream_t st);
+// ldsn (<= PZL_OPT_RESP_FFT_MAX): at least the largest numkern (16 khw) of the
+// jobs and the largest chirp length of their rows' w responses; a row that
+// needs more ORs 1 into *err
+hipError_t pzl_launch_opt_volume(const float2 *data, int numdata, const PzlVolJob *jobs, int njobs, int ldsn,
+                                 const double2 *tw, float *powv, float *tail, int *err, hipStream_t st);
 hipError_t pzl_launch_opt_sum(const float *powv, const float *tail, const PzlVolJob *jobs, const float *locpow,
                               const int *job0, const int *nharm, int ncand, int *best, hipStream_t st);
 #ifdef __cplusplus

@@ -8,12 +8,10 @@
 //   k_opt_interp    one point interpolation per workgroup: the response at
 //                   the point's fractional offset, then its correlation with
 //                   2 khw bins of the spectrum
-//   k_opt_datafft   per (candidate, harmonic): the spectrum segment of the
-//                   volume, spread by 8 and FFT'd
 //   k_opt_volrow    per (candidate, harmonic, w, z): the 8x-oversampled
-//                   response, placed and FFT'd, times the conjugate of the
-//                   segment's transform, inverse FFT, and the powers of the 24
-//                   fine r steps the harmonic sum reads
+//                   response (five FFTs of the chirp for a w response) and its
+//                   correlation with the spectrum at the 24 fine r steps the
+//                   harmonic sum reads, as their powers
 //   k_opt_sum       per candidate: the powers of its harmonics divided by
 //                   their local powers, summed in the reference's order, and
 //                   the first maximum of the volume
@@ -25,8 +23,8 @@
 
 #define OPT_NT 256
 #define OPT_INTERP_MAXV 4   // interpolation responses: numkern <= 1024
-#define OPT_VOL_MAXV 16     // volume responses: numkern <= 4096
 #define OPT_SAMP_MAX 8192   // chirp samples of a w response
+#define OPT_RESP_SAMP (PZL_OPT_RESP_FFT_MAX / OPT_NT)  // chirp samples per thread, residue FFTs
 
 template <typename T> __device__ T block_sum(T v, T *red)
 {
@@ -40,6 +38,89 @@ template <typename T> __device__ T block_sum(T v, T *red)
     const T r = red[0];
     __syncthreads();
     return r;
+}
+
+// Unnormalised FFT (SIGN -1 forward, +1 inverse) of n = 2^k <= PZL_OPT_RESP_FFT_MAX
+// points in LDS: Stockham passes of radix 4 (one of radix 2 last when k is
+// odd), each staging its butterflies in registers between two barriers, with
+// twiddles exp(SIGN 2 pi i j / (Ns R)) read from tw[j] = exp(-2 pi i j / PZL_OPT_TWN).
+// Every thread of the workgroup must call; ends with a barrier.
+template <int SIGN>
+__device__ void opt_fft_pow2(double2 *buf, int n, const double2 *__restrict__ tw, int tid)
+{
+    constexpr int B4 = PZL_OPT_RESP_FFT_MAX / 4 / OPT_NT, B2 = PZL_OPT_RESP_FFT_MAX / 2 / OPT_NT;
+    int Ns = 1;
+    while (Ns < n) {
+        if ((n / Ns) % 4 == 0) {
+            const int nbf = n >> 2, step = PZL_OPT_TWN / (Ns * 4);
+            double2 v[B4][4];
+#pragma unroll
+            for (int b = 0; b < B4; b++) {
+                const int j = tid + b * OPT_NT;
+                if (j < nbf) {
+                    const int jm = j & (Ns - 1);
+                    v[b][0] = buf[j];
+#pragma unroll
+                    for (int r = 1; r < 4; r++) {
+                        double2 t = tw[jm * r * step];
+                        if (SIGN > 0) t.y = -t.y;
+                        v[b][r] = cmul(buf[j + r * nbf], t);
+                    }
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int b = 0; b < B4; b++) {
+                const int j = tid + b * OPT_NT;
+                if (j < nbf) {
+                    dft4<SIGN>(v[b]);
+                    const int jm = j & (Ns - 1), o = ((j - jm) << 2) + jm;
+#pragma unroll
+                    for (int r = 0; r < 4; r++) buf[o + r * Ns] = v[b][r];
+                }
+            }
+            __syncthreads();
+            Ns <<= 2;
+        } else {
+            const int nbf = n >> 1, step = PZL_OPT_TWN / (Ns * 2);
+            double2 v[B2][2];
+#pragma unroll
+            for (int b = 0; b < B2; b++) {
+                const int j = tid + b * OPT_NT;
+                if (j < nbf) {
+                    const int jm = j & (Ns - 1);
+                    double2 t = tw[jm * step];
+                    if (SIGN > 0) t.y = -t.y;
+                    v[b][0] = buf[j];
+                    v[b][1] = cmul(buf[j + nbf], t);
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int b = 0; b < B2; b++) {
+                const int j = tid + b * OPT_NT;
+                if (j < nbf) {
+                    dft2<SIGN>(v[b][0], v[b][1]);
+                    const int jm = j & (Ns - 1), o = ((j - jm) << 1) + jm;
+                    buf[o] = v[b][0];
+                    buf[o + Ns] = v[b][1];
+                }
+            }
+            __syncthreads();
+            Ns <<= 1;
+        }
+    }
+}
+
+// tw[j] = exp(-2 pi i j / PZL_OPT_TWN)
+__global__ __launch_bounds__(OPT_NT) void k_opt_twiddles(double2 *__restrict__ tw)
+{
+    const int j = blockIdx.x * OPT_NT + threadIdx.x;
+    if (j < PZL_OPT_TWN) {
+        double s, c;
+        sincospi(-2.0 * (double)j / (double)PZL_OPT_TWN, &s, &c);
+        tw[j] = double2{c, s};
+    }
 }
 
 // rzw_interp (rzwinterp.c:145-229) and rz_interp (rzinterp.c:144-230): the
@@ -99,79 +180,151 @@ __global__ __launch_bounds__(OPT_NT) void k_opt_interp(const float2 *__restrict_
     if (tid == 0) ans[blockIdx.x] = float2{(float)ar, (float)ai};
 }
 
-// corr_complex with RAW data (corr_routines.c:140-190): numbins = fftlen / 8
-// bins from lobin - khw (zeros outside the data), spread by 8, forward FFT.
-__global__ __launch_bounds__(OPT_NT) void k_opt_datafft(const float2 *__restrict__ data, int numdata,
-                                                        const PzlVolJob *__restrict__ jobs,
-                                                        double2 *__restrict__ dfft, int stride)
+// One (candidate harmonic, w, z) row of corr_rzw_vol (rzwinterp.c:86-139)
+// at the r steps the harmonic sum reads.  The reference correlates the data
+// segment (fftlen / 8 bins from lobin - khw, spread by 8) with the numkern =
+// 16 khw point response gen_w_response(0, 8, z, w, numkern) through FFTs of
+// fftlen points (complex_corr_conv CORR, correlations.c:56-64) and keeps the
+// outputs 8 khw + e, 0 <= e < numgoodbins, where the circular wrap does not
+// reach; output e is the direct sum of 2 khw terms
+//   out(e) = sum_j data[lobin - khw + b0 + j] conj(resp[8 (b0 + j) - e]),
+//   b0 = ceil(e / 8), 0 <= j < 2 khw,
+// so the row evaluates only the outputs read: the 24 fine r steps of the sum
+// and the row's last PZL_OPT_TAIL (a negative step index reads the end of the
+// preceding row of the reference's contiguous volume).  Their float powers
+// follow maximize_rzw.c:144-155.
+//
+// The w response (|w| >= 1e-4) is the numkern bins from k0 = 8 fbar - 8 khw
+// (fbar = npts / 4) of the 8x zero-padded real FFT of npts float chirp
+// samples s[t] (responses.c:333-418).  With k = 8 q + p,
+//   X[k] = sum_t s[t] exp(-2 pi i t k / (8 npts)) = Y_p[q],
+//   Y_p = FFT_npts(s[t] exp(-2 pi i t p / (8 npts))),
+// and s being real, X[8 q + 8 - p] = conj(Y_p[npts - 1 - q]): five npts-point
+// FFTs give every residue.  Output e reads residue p = -e mod 8 at q = fbar -
+// khw + j (mod npts: X is periodic in k; where k0 < 0 the reference copies from
+// before its array).  The z and r responses are evaluated per value into LDS.
+// The LDS holds ldsn double2; a row needing more sets *err and writes nothing.
+__global__ __launch_bounds__(OPT_NT, 2) void k_opt_volrow(const float2 *__restrict__ data, int numdata,
+                                                       const PzlVolJob *__restrict__ jobs,
+                                                       const double2 *__restrict__ tw, int ldsn,
+                                                       float *__restrict__ powv, float *__restrict__ tail,
+                                                       int *__restrict__ err)
 {
     extern __shared__ double2 sbuf[];
-    const int tid = threadIdx.x;
-    const PzlVolJob J = jobs[blockIdx.x];
-    const int fftlen = J.fftlen, numbins = fftlen / PZL_OPT_NB, beginbin = J.lobin - J.khw;
-    for (int k = tid; k < fftlen; k += OPT_NT) sbuf[k] = double2{0.0, 0.0};
-    __syncthreads();
-    for (int b = tid; b < numbins; b += OPT_NT) {
-        const int idx = beginbin + b;
-        if (idx >= 0 && idx < numdata) {
-            const float2 x = data[idx];
-            sbuf[b * PZL_OPT_NB] = double2{(double)x.x, (double)x.y};
-        }
-    }
-    __syncthreads();
-    lds_fft_f64<-1>(sbuf, fftlen, tid, OPT_NT);
-    double2 *out = dfft + (long long)blockIdx.x * stride;
-    for (int k = tid; k < fftlen; k += OPT_NT) out[k] = sbuf[k];
-}
-
-// One (candidate harmonic, w, z) row of corr_rzw_vol (rzwinterp.c:86-139):
-// gen_w_response(0, 8, z, w, 16 khw), place_complex_kernel, forward FFT,
-// complex_corr_conv CORR (correlations.c:56-64: data x conj(kernel) / fftlen),
-// inverse FFT, chop 8 khw points; then the float powers (maximize_rzw.c:144-155)
-// of the rows' 24 fine r steps.
-__global__ __launch_bounds__(OPT_NT) void k_opt_volrow(const PzlVolJob *__restrict__ jobs,
-                                                       const double2 *__restrict__ dfft, int stride,
-                                                       float *__restrict__ powv, float *__restrict__ tail)
-{
-    extern __shared__ double2 sbuf[];
-    const int tid = threadIdx.x;
+    constexpr int NOUT = PZL_OPT_NUMR + PZL_OPT_TAIL;  // 32 outputs per row
+    constexpr int GL = 32, NG = OPT_NT / GL;           // lanes per output, outputs in flight
+    __shared__ int outs[NOUT];      // e of each output slot, -1 if not needed
+    __shared__ int lst[5][NOUT];    // output slots of each residue pass
+    __shared__ int lcnt[5];
+    const int tid = threadIdx.x, lane = tid % GL, grp = tid / GL;
     constexpr int NZW = PZL_OPT_NUMZ * PZL_OPT_NUMW;
     const int job = blockIdx.x / NZW, ij = blockIdx.x % NZW;
     const int wi = ij / PZL_OPT_NUMZ, zj = ij % PZL_OPT_NUMZ;
     const PzlVolJob &J = jobs[job];
-    const int fftlen = J.fftlen, khw = J.khw;
+    const int khw = J.khw, beginbin = J.lobin - khw, nterm = 2 * khw;
     const double z = J.zlo + zj * J.dz, w = J.wlo + wi * J.dw;
-    const int numkern = 2 * PZL_OPT_NB * khw, half = numkern / 2;
-    double2 resp[OPT_VOL_MAXV];
-    pzl_gen_response<OPT_NT, OPT_VOL_MAXV>(0.0, PZL_OPT_NB, z, w, numkern, tid, (float *)sbuf, resp);
-    for (int k = tid; k < fftlen; k += OPT_NT) sbuf[k] = double2{0.0, 0.0};
+    const int numkern = 2 * PZL_OPT_NB * khw;
+    const bool wresp = fabs(w) >= 1e-4;
+    if (tid < NOUT) {
+        int e;
+        if (tid < PZL_OPT_NUMR) e = J.rind[tid] >= 0 ? J.rind[tid] : -1;
+        else e = J.numgoodbins - PZL_OPT_TAIL + (tid - PZL_OPT_NUMR);
+        outs[tid] = e;
+    }
     __syncthreads();
+    if (tid == 0) {
+        for (int p = 0; p < 5; p++) lcnt[p] = 0;
+        for (int o = 0; o < NOUT; o++) {
+            const int e = outs[o];
+            if (e < 0) continue;
+            const int c = (-e) & (PZL_OPT_NB - 1);
+            const int p = wresp ? (c <= PZL_OPT_NB / 2 ? c : PZL_OPT_NB - c) : 0;
+            lst[p][lcnt[p]++] = o;
+        }
+    }
+    // out(e) from the response values in sbuf: pass p of the w response holds
+    // Y_p (mirror classes read npts - 1 - q), the z and r responses hold resp[m]
+    auto emit = [&](int p, int npts) {
+        const int cnt = lcnt[p];
+        for (int base = 0; base < cnt; base += NG) {
+            const int i = base + grp;
+            double ar = 0.0, ai = 0.0;
+            int o = -1;
+            if (i < cnt) {
+                o = lst[p][i];
+                const int e = outs[o], b0 = (e + PZL_OPT_NB - 1) / PZL_OPT_NB;
+                const bool mirror = wresp && ((-e) & (PZL_OPT_NB - 1)) != p;
+                const int q0 = npts / 4 - khw;
+                for (int j = lane; j < nterm; j += GL) {
+                    const int b = beginbin + b0 + j;
+                    const float2 d = (b >= 0 && b < numdata) ? data[b] : float2{0.f, 0.f};
+                    double yr, yi;  // conj of the response value
+                    if (wresp) {
+                        const int q = (q0 + j) & (npts - 1);
+                        const double2 y = sbuf[mirror ? npts - 1 - q : q];
+                        yr = y.x;
+                        yi = mirror ? y.y : -y.y;
+                    } else {
+                        const double2 y = sbuf[PZL_OPT_NB * (b0 + j) - e];
+                        yr = y.x;
+                        yi = -y.y;
+                    }
+                    ar += (double)d.x * yr - (double)d.y * yi;
+                    ai += (double)d.x * yi + (double)d.y * yr;
+                }
+            }
 #pragma unroll
-    for (int i = 0; i < OPT_VOL_MAXV; i++) {
-        const int m = tid + i * OPT_NT;
-        if (m < numkern) sbuf[(m >= half) ? (m - half) : (fftlen - half + m)] = resp[i];
-    }
-    __syncthreads();
-    lds_fft_f64<-1>(sbuf, fftlen, tid, OPT_NT);
-    const double2 *D = dfft + (long long)job * stride;
-    const double norm = 1.0 / fftlen;
-    for (int k = tid; k < fftlen; k += OPT_NT) {
-        const double2 d = D[k], kk = sbuf[k];
-        sbuf[k] = double2{(d.x * kk.x + d.y * kk.y) * norm, (d.y * kk.x - kk.y * d.x) * norm};
-    }
-    __syncthreads();
-    lds_fft_f64<1>(sbuf, fftlen, tid, OPT_NT);
-    // the r steps of the sum, and the row's last PZL_OPT_TAIL powers: the sum
-    // reads a negative step index in the preceding row of the contiguous volume
-    int e = -1;
-    if (tid < PZL_OPT_NUMR) e = J.rind[tid];
-    else if (tid < PZL_OPT_NUMR + PZL_OPT_TAIL) e = J.numgoodbins - PZL_OPT_TAIL + (tid - PZL_OPT_NUMR);
-    if (e >= 0) {
-        const double2 v = sbuf[PZL_OPT_NB * khw + e];
-        const float ar = (float)v.x, ai = (float)v.y;
-        const float a2 = ar * ar, b2 = ai * ai;
-        if (tid < PZL_OPT_NUMR) powv[((long long)job * NZW + ij) * PZL_OPT_NUMR + tid] = a2 + b2;
-        else tail[((long long)job * NZW + ij) * PZL_OPT_TAIL + (tid - PZL_OPT_NUMR)] = a2 + b2;
+            for (int off = GL / 2; off > 0; off >>= 1) {
+                ar += __shfl_xor(ar, off, GL);
+                ai += __shfl_xor(ai, off, GL);
+            }
+            if (o >= 0 && lane == 0) {
+                const float fr = (float)ar, fi = (float)ai;
+                const float pw = fr * fr + fi * fi;
+                if (o < PZL_OPT_NUMR) powv[((long long)job * NZW + ij) * PZL_OPT_NUMR + o] = pw;
+                else tail[((long long)job * NZW + ij) * PZL_OPT_TAIL + (o - PZL_OPT_NUMR)] = pw;
+            }
+        }
+    };
+    if (wresp) {
+        const int npts = pzl_next2_to_n(6 * pzl_w_resp_halfwidth(z, w) + 200 + numkern / PZL_OPT_NB);
+        if (npts > ldsn) {  // the launch sized the LDS for shorter chirps
+            if (tid == 0) atomicOr(err, 1);
+            return;
+        }
+        const double dt = 1.0 / (double)npts, amp = 2.0 * dt;
+        const int fbar = npts / 4;
+        const double f = (double)fbar + 0.0 - 0.5 * z + w / 12.0, fd = (z - 0.5 * w) / 2.0, fdd = w / 6.0;
+        float smp[OPT_RESP_SAMP];
+#pragma unroll
+        for (int i = 0; i < OPT_RESP_SAMP; i++) {
+            const int t = tid + i * OPT_NT;
+            const double tt = t * dt;
+            smp[i] = (t < npts) ? (float)(amp * cos(6.283185307179586477 * (tt * (tt * (tt * fdd + fd) + f)))) : 0.0f;
+        }
+        const int tstep = PZL_OPT_TWN / (PZL_OPT_NB * npts);
+        for (int p = 0; p <= PZL_OPT_NB / 2; p++) {
+            __syncthreads();  // the previous pass has read sbuf
+#pragma unroll
+            for (int i = 0; i < OPT_RESP_SAMP; i++) {
+                const int t = tid + i * OPT_NT;
+                if (t < npts) {
+                    const double2 e = tw[t * p * tstep];
+                    sbuf[t] = double2{(double)smp[i] * e.x, (double)smp[i] * e.y};
+                }
+            }
+            __syncthreads();
+            opt_fft_pow2<-1>(sbuf, npts, tw, tid);
+            emit(p, npts);
+        }
+    } else {
+        if (numkern > ldsn) {
+            if (tid == 0) atomicOr(err, 1);
+            return;
+        }
+        for (int m = tid; m < numkern; m += OPT_NT) sbuf[m] = pzl_zr_response_at(0.0, PZL_OPT_NB, z, numkern, m);
+        __syncthreads();
+        emit(0, 0);
     }
 }
 
@@ -237,17 +390,20 @@ extern "C" hipError_t pzl_launch_opt_interp(const float2 *data, int numdata, con
     return hipGetLastError();
 }
 
+extern "C" hipError_t pzl_launch_opt_twiddles(double2 *tw, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_opt_twiddles, dim3(PZL_OPT_TWN / OPT_NT), dim3(OPT_NT), 0, st, tw);
+    return hipGetLastError();
+}
+
 extern "C" hipError_t pzl_launch_opt_volume(const float2 *data, int numdata, const PzlVolJob *jobs, int njobs,
-                                            int max_fftlen, double2 *dfft, float *powv, float *tail,
+                                            int ldsn, const double2 *tw, float *powv, float *tail, int *err,
                                             hipStream_t st)
 {
     if (njobs <= 0) return hipSuccess;
-    const size_t smem = sizeof(double2) * (size_t)max_fftlen;
-    hipLaunchKernelGGL(k_opt_datafft, dim3(njobs), dim3(OPT_NT), smem, st, data, numdata, jobs, dfft, max_fftlen);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_opt_volrow, dim3(njobs * PZL_OPT_NUMZ * PZL_OPT_NUMW), dim3(OPT_NT), smem, st, jobs,
-                       dfft, max_fftlen, powv, tail);
+    if (ldsn <= 0 || ldsn > PZL_OPT_RESP_FFT_MAX) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_opt_volrow, dim3(njobs * PZL_OPT_NUMZ * PZL_OPT_NUMW), dim3(OPT_NT),
+                       sizeof(double2) * (size_t)ldsn, st, data, numdata, jobs, tw, ldsn, powv, tail, err);
     return hipGetLastError();
 }
 

@@ -16,11 +16,56 @@
 
 #include "pzl_math.h"
 
+// Value m (< numkern) of gen_z_response (responses.c:228-316) for
+// |z| >= 1e-4, else of gen_r_response (responses.c:159-225).
+__device__ inline double2 pzl_zr_response_at(double roffset, int numbetween, double z, int numkern, int m)
+{
+    const double PI = 3.14159265358979323846;
+    if (fabs(z) >= 1e-4) {
+        const double absz = fabs(z);
+        const int signz = (z < 0.0) ? -1 : 1;
+        const double zd = signz * 1.41421356237309504880 / sqrt(absz);
+        const double cons = zd / 2.0, pibyz = PI / z;
+        double startr = roffset - 0.5 * z;
+        double ip;
+        const double startroffset = (startr < 0) ? 1.0 + modf(startr, &ip) : modf(startr, &ip);
+        startr += numkern / (double)(2 * numbetween);
+        const double delta = -1.0 / numbetween;
+        const double r = startr + m * delta;
+        const double yy = r * zd, zz = yy + z * zd, xx = pibyz * r * r;
+        double fsy, fcy, fsz, fcz;
+        pzl_fresnel(yy, &fsy, &fcy);
+        pzl_fresnel(zz, &fsz, &fcz);
+        const double c = cos(xx), s = sin(xx);
+        const double tr = signz * (fcz - fcy), ti = fsy - fsz;
+        double re = (tr * c - ti * s) * cons;
+        double im = -(tr * s + ti * c) * cons;
+        if (m == numkern / 2 && startroffset < 1E-3 && absz < 1E-3) {
+            const double z2 = z * z, x2 = startroffset * startroffset;
+            re = 1.0 - 0.16449340668482264365 * z2 + startroffset * 1.6449340668482264365 * z +
+                 x2 * (-6.579736267392905746 + 0.9277056288952613070 * z2);
+            im = -0.5235987755982988731 * z + startroffset * (PI - 0.5167712780049970029 * z2) +
+                 x2 * (3.1006276680299820175 * z);
+        }
+        return double2{re, im};
+    }
+    // sinc(r) e^{i r}, r = pi (numkern / (2 nb) + roffset - m / nb)
+    const double x = numkern / (double)(2 * numbetween) + roffset - m / (double)numbetween;  // r / pi
+    if (m == numkern / 2 && roffset < 1E-3) {
+        const double r2 = roffset * roffset;
+        return double2{1.0 - 6.579736267392905746 * r2, roffset * (PI - 10.335425560099940058 * r2)};
+    }
+    if (x == 0.0) return double2{1.0, 0.0};
+    double s, c;
+    sincospi(x, &s, &c);
+    const double sinc = s / (PI * x);
+    return double2{c * sinc, s * sinc};
+}
+
 template <int NT, int MAXV>
 __device__ void pzl_gen_response(double roffset, int numbetween, double z, double w, int numkern, int tid,
                                  float *samp, double2 (&resp)[MAXV])
 {
-    const double PI = 3.14159265358979323846;
     if (fabs(w) >= 1e-4) {
         // gen_w_response (responses.c:333-418)
         const int npts = pzl_next2_to_n(6 * pzl_w_resp_halfwidth(z, w) + 200 + numkern / numbetween);
@@ -62,65 +107,11 @@ __device__ void pzl_gen_response(double roffset, int numbetween, double z, doubl
             resp[i] = double2{ar, ai};
         }
         __syncthreads();
-    } else if (fabs(z) >= 1e-4) {
-        // gen_z_response (responses.c:228-316)
-        const double absz = fabs(z);
-        const int signz = (z < 0.0) ? -1 : 1;
-        const double zd = signz * 1.41421356237309504880 / sqrt(absz);
-        const double cons = zd / 2.0, pibyz = PI / z;
-        double startr = roffset - 0.5 * z;
-        double ip;
-        const double startroffset = (startr < 0) ? 1.0 + modf(startr, &ip) : modf(startr, &ip);
-        startr += numkern / (double)(2 * numbetween);
-        const double delta = -1.0 / numbetween;
-#pragma unroll
-        for (int i = 0; i < MAXV; i++) {
-            const int m = tid + i * NT;
-            double re = 0.0, im = 0.0;
-            if (m < numkern) {
-                const double r = startr + m * delta;
-                const double yy = r * zd, zz = yy + z * zd, xx = pibyz * r * r;
-                double fsy, fcy, fsz, fcz;
-                pzl_fresnel(yy, &fsy, &fcy);
-                pzl_fresnel(zz, &fsz, &fcz);
-                const double c = cos(xx), s = sin(xx);
-                const double tr = signz * (fcz - fcy), ti = fsy - fsz;
-                re = (tr * c - ti * s) * cons;
-                im = -(tr * s + ti * c) * cons;
-                if (m == numkern / 2 && startroffset < 1E-3 && absz < 1E-3) {
-                    const double z2 = z * z, x2 = startroffset * startroffset;
-                    re = 1.0 - 0.16449340668482264365 * z2 + startroffset * 1.6449340668482264365 * z +
-                         x2 * (-6.579736267392905746 + 0.9277056288952613070 * z2);
-                    im = -0.5235987755982988731 * z + startroffset * (PI - 0.5167712780049970029 * z2) +
-                         x2 * (3.1006276680299820175 * z);
-                }
-            }
-            resp[i] = double2{re, im};
-        }
     } else {
-        // gen_r_response (responses.c:159-225): sinc(r) e^{i r}, r = pi (numkern / (2 nb) + roffset - m / nb)
 #pragma unroll
         for (int i = 0; i < MAXV; i++) {
             const int m = tid + i * NT;
-            double re = 0.0, im = 0.0;
-            if (m < numkern) {
-                const double x = numkern / (double)(2 * numbetween) + roffset - m / (double)numbetween;  // r / pi
-                if (m == numkern / 2 && roffset < 1E-3) {
-                    const double r2 = roffset * roffset;
-                    re = 1.0 - 6.579736267392905746 * r2;
-                    im = roffset * (PI - 10.335425560099940058 * r2);
-                } else if (x == 0.0) {
-                    re = 1.0;
-                    im = 0.0;
-                } else {
-                    double s, c;
-                    sincospi(x, &s, &c);
-                    const double sinc = s / (PI * x);
-                    re = c * sinc;
-                    im = s * sinc;
-                }
-            }
-            resp[i] = double2{re, im};
+            resp[i] = (m < numkern) ? pzl_zr_response_at(roffset, numbetween, z, numkern, m) : double2{0.0, 0.0};
         }
     }
 }
