@@ -93,6 +93,9 @@ typedef struct PzlStats {
 
 const char *pzl_last_error(void);
 int pzl_device_count(int *count);
+/* Free and total device memory of 'device' (the cudaMemGetInfo measurement
+ * that accelsearch_maxsize.c:564-566 leaves commented out). */
+int pzl_device_memory(int device, long long *free_bytes, long long *total_bytes);
 
 /* Kernel banks are cached per (device, harmonic shapes) and shared by plans;
  * unreferenced banks are evicted oldest-first above the byte budget
@@ -108,8 +111,15 @@ int pzl_plan_create(const PzlConfig *cfg, int device, void **plan);
 void pzl_plan_destroy(void *plan);
 int pzl_plan_geometry(void *plan, PzlGeometry *g);
 int pzl_plan_harm(void *plan, int idx, PzlHarmInfo *h);
-/* Search shape only, without touching the GPU. */
+/* Search shape only, without touching the GPU.  g->device_bytes is the HBM
+ * footprint model that admission uses (replaces accelsearch_maxsize,
+ * accelsearch_maxsize.c:551-619): a plan's exact allocations item by item,
+ * with the sort scratch bounded before the plan exists; pzl_plan_geometry
+ * reports it with the plan's actual sort scratch. */
 int pzl_geometry_from_config(const PzlConfig *cfg, PzlGeometry *g);
+/* Device memory the plan holds: what it allocated (its spectrum once set
+ * from the host included) plus its kernel bank.  accelsearch_maxsize.c:551-619 */
+int pzl_plan_device_bytes(void *plan, long long *held);
 
 /* The spectrum: numbins complex64 values, realfft packing (read_fft.c:4-47).
  * _device takes a caller-owned device pointer that must stay valid. */
@@ -238,7 +248,11 @@ typedef struct PzlServerStats {
     int inflight, peak_inflight;
 } PzlServerStats;
 
-/* hbm_budget <= 0: 80% of the device's free memory. */
+/* A request is admitted by its footprint: the geometry's device_bytes plus
+ * PZL_RUNTIME_BYTES_PER_SEARCH for the HIP runtime's own memory behind a
+ * plan's streams (measured at most 27 MiB on MI355X).
+ * hbm_budget <= 0: 80% of the device's free memory. */
+#define PZL_RUNTIME_BYTES_PER_SEARCH (32LL << 20)
 int pzl_server_create(int device, long long hbm_budget, int max_inflight, void **server);
 /* Queue a search; host_fft (numbins complex64) must stay valid until the
  * request completes.  Fails if the request alone exceeds the budget. */

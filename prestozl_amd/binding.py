@@ -113,11 +113,13 @@ def lib():
         P = C.POINTER
         L.pzl_last_error.restype = C.c_char_p
         L.pzl_device_count.argtypes = [P(C.c_int)]
+        L.pzl_device_memory.argtypes = [C.c_int, P(C.c_longlong), P(C.c_longlong)]
         L.pzl_geometry_from_config.argtypes = [P(Config), P(Geometry)]
         L.pzl_plan_create.argtypes = [P(Config), C.c_int, P(C.c_void_p)]
         L.pzl_plan_destroy.argtypes = [C.c_void_p]
         L.pzl_plan_destroy.restype = None
         L.pzl_plan_geometry.argtypes = [C.c_void_p, P(Geometry)]
+        L.pzl_plan_device_bytes.argtypes = [C.c_void_p, P(C.c_longlong)]
         L.pzl_plan_harm.argtypes = [C.c_void_p, C.c_int, P(HarmInfo)]
         L.pzl_set_spectrum.argtypes = [C.c_void_p, P(C.c_float), C.c_longlong]
         L.pzl_set_spectrum_device.argtypes = [C.c_void_p, C.c_void_p, C.c_longlong]
@@ -177,6 +179,21 @@ def make_config(N, dt, numbins, zmax, wmax, numharm, sigma, wmaxP=True, flo=1.0,
     return c
 
 
+RUNTIME_BYTES_PER_SEARCH = 32 << 20  # PZL_RUNTIME_BYTES_PER_SEARCH (pzl_accel.h)
+
+
+def device_memory(device=0):
+    """(free, total) bytes of the device's memory."""
+    f, t = C.c_longlong(), C.c_longlong()
+    check(lib().pzl_device_memory(device, C.byref(f), C.byref(t)))
+    return f.value, t.value
+
+
+def bank_cache_clear():
+    """Free every kernel bank no plan holds."""
+    check(lib().pzl_bank_cache_clear())
+
+
 def geometry(cfg):
     g = Geometry()
     check(lib().pzl_geometry_from_config(C.byref(cfg), C.byref(g)))
@@ -221,6 +238,18 @@ class Plan:
 
     def __exit__(self, *a):
         self.close()
+
+    def device_bytes(self):
+        """Device memory the plan holds (its allocations and its kernel bank)."""
+        v = C.c_longlong()
+        check(lib().pzl_plan_device_bytes(self.h, C.byref(v)))
+        return v.value
+
+    def geometry(self):
+        """The plan's geometry; device_bytes carries its actual sort scratch."""
+        g = Geometry()
+        check(lib().pzl_plan_geometry(self.h, C.byref(g)))
+        return g
 
     def set_spectrum(self, fft):
         """fft: complex64 array (numbins) or float32 interleaved."""

@@ -156,17 +156,62 @@ PlanSizes plan_sizes(const PzlConfig &cfg, const PzlGeometry &g, const std::vect
     return z;
 }
 
-// HBM held by a plan: bank, spectrum, batch buffers, hit slots and ordering
-// buffers (sort scratch taken as its reported size, or estimated at 12 bytes
-// per hit when not yet known).
-long long plan_device_bytes(const PzlGeometry &g, const PlanSizes &z, int nh, long long sort_temp)
+// The buffers a plan carves from its one device arena, in carving order
+// (pzl_plan_create assigns them in this order): harmonic tables, two sets of
+// batch buffers (segments, r-index tables, harmonic r maps, planes), hit
+// slots and ordering buffers.  The sort scratch is its reported size, or,
+// before a plan exists, bounded by 12 bytes per hit + 16 MiB.
+constexpr long long kArenaAlign = 256;
+
+std::vector<long long> plan_arena_items(const PzlGeometry &g, const PlanSizes &z, const std::vector<PzlHarm> &harms,
+                                        long long sort_temp)
 {
+    const long long nh = (long long)harms.size();
+    const PzlHarm &F = harms[0];
     if (sort_temp < 0) sort_temp = 12 * z.hitcap + (16LL << 20);
-    return g.bank_bytes + 8 * g.numbins +
-           Plan::kSets * (g.plane_bytes_per_rstep * z.B + (long long)sizeof(float) * PZL_PLANE_SLACK +
-                          (long long)sizeof(float2) * z.seg_stride * z.B +
-                          (long long)sizeof(uint16_t) * z.B * nh * g.corr_uselen) +
-           (long long)(z.nslots * 12 + 12 + (long long)sizeof(PzlHostHit)) * z.hitcap + sort_temp;
+    std::vector<long long> v{(long long)sizeof(PzlHarm) * nh, (long long)sizeof(int) * nh,
+                             (long long)sizeof(double) * (g.nrsteps + 1), (long long)sizeof(uint16_t) * nh * F.numz,
+                             (long long)sizeof(int) * nh * F.numw};
+    for (int k = 0; k < Plan::kSets; k++) {
+        v.push_back((long long)sizeof(float2) * z.seg_stride * z.B);
+        v.push_back((long long)sizeof(uint16_t) * z.B * nh * g.corr_uselen);
+        v.push_back((long long)sizeof(PzlHrTab) * z.B * nh);
+        v.push_back((long long)sizeof(float) * (g.plane_bytes_per_rstep / 4 * z.B + PZL_PLANE_SLACK));
+    }
+    for (int s = 0; s < z.nslots; s++) {
+        v.push_back((long long)sizeof(long long) * z.hitcap);
+        v.push_back((long long)sizeof(float) * z.hitcap);
+    }
+    v.push_back((long long)sizeof(unsigned long long) * 2 * Plan::kSlots);
+    v.push_back((long long)sizeof(long long) * Plan::kSets);
+    v.push_back((long long)sizeof(long long) * z.hitcap);
+    v.push_back((long long)sizeof(float) * z.hitcap);
+    v.push_back((long long)sizeof(PzlHostHit) * z.hitcap);
+    v.push_back(std::max<long long>(sort_temp, 16));
+    return v;
+}
+
+long long arena_bytes(const std::vector<long long> &items)
+{
+    long long t = 0;
+    for (long long b : items) t += (b + kArenaAlign - 1) / kArenaAlign * kArenaAlign;
+    return t;
+}
+
+// HBM held by a plan whose spectrum was set from the host: kernel bank,
+// spectrum and arena.
+long long plan_device_bytes(const PzlGeometry &g, const PlanSizes &z, const std::vector<PzlHarm> &harms,
+                            long long sort_temp)
+{
+    return g.bank_bytes + (long long)sizeof(float2) * g.numbins + arena_bytes(plan_arena_items(g, z, harms, sort_temp));
+}
+
+// hipMalloc of a plan-owned buffer, counted in p->dev_bytes
+template <typename T> hipError_t plan_malloc(Plan *p, T **ptr, size_t bytes)
+{
+    hipError_t e = hipMalloc((void **)ptr, bytes);
+    if (e == hipSuccess) p->dev_bytes += (long long)bytes;
+    return e;
 }
 
 // Correlation variants chosen by pzl_autotune, per (device, FFT length,
@@ -196,42 +241,26 @@ void free_plan(Plan *p)
     if (p->st3) (void)hipStreamSynchronize(p->st3);
     if (p->st4) (void)hipStreamSynchronize(p->st4);
     if (p->st2) (void)hipStreamSynchronize(p->st2);
-    for (auto &c : p->classes) (void)hipFree(c.d_harms);
-    (void)hipFree(p->d_harms);
     release_bank(p);
-    (void)hipFree(p->d_startr);
-    (void)hipFree(p->d_zinds);
-    (void)hipFree(p->d_winds);
+    (void)hipFree(p->d_arena);
     (void)hipFree(p->d_fft_own);
     for (int k = 0; k < Plan::kSets; k++) {
-        (void)hipFree(p->d_segs[k]);
-        (void)hipFree(p->d_rinds[k]);
-        (void)hipFree(p->d_hr[k]);
-        (void)hipFree(p->d_planes[k]);
         if (p->ev_corr[k]) (void)hipEventDestroy(p->ev_corr[k]);
         if (p->ev_sum[k]) (void)hipEventDestroy(p->ev_sum[k]);
         if (p->ev_seg[k]) (void)hipEventDestroy(p->ev_seg[k]);
         if (p->ev_side[k]) (void)hipEventDestroy(p->ev_side[k]);
     }
     for (int s = 0; s < Plan::kSlots; s++) {
-        (void)hipFree(p->d_hkeys[s]);
-        (void)hipFree(p->d_hpows[s]);
         if (p->ev_count[s]) (void)hipEventDestroy(p->ev_count[s]);
         if (p->ev_free[s]) (void)hipEventDestroy(p->ev_free[s]);
     }
-    (void)hipFree(p->d_counts);
     if (p->h_counts) (void)hipHostFree(p->h_counts);
-    (void)hipFree(p->d_skeys);
-    (void)hipFree(p->d_spows);
-    (void)hipFree(p->d_recs);
-    (void)hipFree(p->d_sort_temp);
     for (auto &hb : p->hpool) {
         if (hb.ptr) (void)hipHostFree(hb.ptr);
         if (hb.ev) (void)hipEventDestroy(hb.ev);
     }
     for (int k = 0; k < Plan::kSets; k++)
         if (p->gexec[k]) (void)hipGraphExecDestroy(p->gexec[k]);
-    (void)hipFree(p->d_r0);
     if (p->h_r0) (void)hipHostFree(p->h_r0);
     (void)hipFree(p->d_sums);
     if (p->st) (void)hipStreamDestroy(p->st);
@@ -732,13 +761,24 @@ int pzl_device_count(int *count)
     return 0;
 }
 
+int pzl_device_memory(int device, long long *free_bytes, long long *total_bytes)
+{
+    if (!free_bytes || !total_bytes) return fail(PZL_ERR_ARG, "null argument");
+    size_t f = 0, t = 0;
+    PZL_HIPCHK(hipSetDevice(device));
+    PZL_HIPCHK(hipMemGetInfo(&f, &t));
+    *free_bytes = (long long)f;
+    *total_bytes = (long long)t;
+    return 0;
+}
+
 int pzl_geometry_from_config(const PzlConfig *cfg, PzlGeometry *g)
 {
     if (!cfg || !g) return fail(PZL_ERR_ARG, "null argument");
     std::vector<PzlHarm> harms;
     int rc = compute_geometry(*cfg, *g, &harms, nullptr);
     if (rc) return rc;
-    g->device_bytes = plan_device_bytes(*g, plan_sizes(*cfg, *g, harms), (int)harms.size(), -1);
+    g->device_bytes = plan_device_bytes(*g, plan_sizes(*cfg, *g, harms), harms, -1);
     return 0;
 }
 
@@ -865,32 +905,58 @@ int pzl_plan_create(const PzlConfig *cfg, int device, void **plan_out)
         PCHK(hipEventCreateWithFlags(&p->ev_count[s], hipEventDisableTiming));
         PCHK(hipEventCreateWithFlags(&p->ev_free[s], hipEventDisableTiming));
     }
-    PCHK(hipMalloc(&p->d_harms, sizeof(PzlHarm) * nh));
-    PCHK(hipMemcpy(p->d_harms, p->harms.data(), sizeof(PzlHarm) * nh, hipMemcpyHostToDevice));
-    for (auto &c : p->classes) {
-        PCHK(hipMalloc(&c.d_harms, sizeof(int) * c.harms.size()));
-        PCHK(hipMemcpy(c.d_harms, c.harms.data(), sizeof(int) * c.harms.size(), hipMemcpyHostToDevice));
-    }
-    // one spare entry after the rsteps: planes at an arbitrary startr
-    const size_t nstart = p->startr.size() + 1;
-    PCHK(hipMalloc(&p->d_startr, sizeof(double) * nstart));
-    if (!p->startr.empty())
-        PCHK(hipMemcpy(p->d_startr, p->startr.data(), sizeof(double) * p->startr.size(), hipMemcpyHostToDevice));
-    PCHK(hipMalloc(&p->d_zinds, sizeof(uint16_t) * p->zinds.size()));
-    PCHK(hipMemcpy(p->d_zinds, p->zinds.data(), sizeof(uint16_t) * p->zinds.size(), hipMemcpyHostToDevice));
-    PCHK(hipMalloc(&p->d_winds, sizeof(int) * p->winds.size()));
-    PCHK(hipMemcpy(p->d_winds, p->winds.data(), sizeof(int) * p->winds.size(), hipMemcpyHostToDevice));
-    p->seg_stride = 0;
-    for (auto &H : p->harms) p->seg_stride += H.fftlen / 2;
+    p->seg_stride = sizes.seg_stride;
     p->plane_stride = p->g.plane_bytes_per_rstep / 4;
     if (p->g.plane_bytes_per_rstep >= (1LL << 31))
         return cleanup(fail(PZL_ERR_UNSUPPORTED, "the planes of one rstep need %lld bytes (at most 2 GiB)",
                             p->g.plane_bytes_per_rstep));
+    p->sort_temp_bytes = 0;
+    PCHK(pzl_sort_hits(nullptr, &p->sort_temp_bytes, nullptr, nullptr, nullptr, nullptr, p->hitcap, p->key_bits,
+                       p->st2));
+    // every buffer in one allocation, in the order plan_arena_items lists them
+    const std::vector<long long> items = plan_arena_items(p->g, sizes, p->harms, (long long)p->sort_temp_bytes);
+    PCHK(plan_malloc(p, &p->d_arena, (size_t)arena_bytes(items)));
+    size_t item = 0;
+    char *cur = p->d_arena;
+    auto carve = [&]() {
+        char *r = cur;
+        cur += (items[item] + kArenaAlign - 1) / kArenaAlign * kArenaAlign;
+        item++;
+        return r;
+    };
+    p->d_harms = (PzlHarm *)carve();
+    int *cls_idx = (int *)carve();
+    p->d_startr = (double *)carve();
+    p->d_zinds = (uint16_t *)carve();
+    p->d_winds = (int *)carve();
     for (int k = 0; k < Plan::kSets; k++) {
-        PCHK(hipMalloc(&p->d_segs[k], sizeof(float2) * p->seg_stride * p->B));
-        PCHK(hipMalloc(&p->d_rinds[k], sizeof(uint16_t) * (size_t)p->B * nh * p->g.corr_uselen));
-        PCHK(hipMalloc(&p->d_hr[k], sizeof(PzlHrTab) * (size_t)p->B * nh));
-        PCHK(hipMalloc(&p->d_planes[k], sizeof(float) * ((size_t)p->plane_stride * p->B + PZL_PLANE_SLACK)));
+        p->d_segs[k] = (float2 *)carve();
+        p->d_rinds[k] = (uint16_t *)carve();
+        p->d_hr[k] = (PzlHrTab *)carve();
+        p->d_planes[k] = (float *)carve();
+    }
+    for (int s = 0; s < p->nslots; s++) {
+        p->d_hkeys[s] = (long long *)carve();
+        p->d_hpows[s] = (float *)carve();
+    }
+    p->d_counts = (unsigned long long *)carve();
+    p->d_r0 = (long long *)carve();
+    p->d_skeys = (long long *)carve();
+    p->d_spows = (float *)carve();
+    p->d_recs = (PzlHostHit *)carve();
+    p->d_sort_temp = carve();
+    if (item != items.size()) return cleanup(fail(PZL_ERR_ARG, "plan arena layout mismatch"));
+    PCHK(hipMemcpy(p->d_harms, p->harms.data(), sizeof(PzlHarm) * nh, hipMemcpyHostToDevice));
+    for (auto &c : p->classes) {
+        c.d_harms = cls_idx;
+        PCHK(hipMemcpy(c.d_harms, c.harms.data(), sizeof(int) * c.harms.size(), hipMemcpyHostToDevice));
+        cls_idx += c.harms.size();
+    }
+    if (!p->startr.empty())
+        PCHK(hipMemcpy(p->d_startr, p->startr.data(), sizeof(double) * p->startr.size(), hipMemcpyHostToDevice));
+    PCHK(hipMemcpy(p->d_zinds, p->zinds.data(), sizeof(uint16_t) * p->zinds.size(), hipMemcpyHostToDevice));
+    PCHK(hipMemcpy(p->d_winds, p->winds.data(), sizeof(int) * p->winds.size(), hipMemcpyHostToDevice));
+    for (int k = 0; k < Plan::kSets; k++) {
         PCHK(hipEventCreateWithFlags(&p->ev_corr[k], hipEventDisableTiming));
         PCHK(hipEventCreateWithFlags(&p->ev_sum[k], hipEventDisableTiming));
         PCHK(hipEventCreateWithFlags(&p->ev_seg[k], hipEventDisableTiming));
@@ -901,22 +967,9 @@ int pzl_plan_create(const PzlConfig *cfg, int device, void **plan_out)
         PCHK(hipEventRecord(p->ev_corr[k], p->st));
         PCHK(hipEventRecord(p->ev_sum[k], p->st));
     }
-    for (int s = 0; s < p->nslots; s++) {
-        PCHK(hipMalloc(&p->d_hkeys[s], sizeof(long long) * p->hitcap));
-        PCHK(hipMalloc(&p->d_hpows[s], sizeof(float) * p->hitcap));
-    }
-    PCHK(hipMalloc(&p->d_counts, 2 * Plan::kSlots * sizeof(unsigned long long)));
-    PCHK(hipMalloc(&p->d_r0, Plan::kSets * sizeof(long long)));
     PCHK(hipHostMalloc((void **)&p->h_r0, Plan::kR0Ring * sizeof(long long), hipHostMallocDefault));
     PCHK(hipMemset(p->d_counts, 0, 2 * Plan::kSlots * sizeof(unsigned long long)));
     PCHK(hipHostMalloc((void **)&p->h_counts, 2 * Plan::kSlots * sizeof(unsigned long long), hipHostMallocDefault));
-    PCHK(hipMalloc(&p->d_skeys, sizeof(long long) * p->hitcap));
-    PCHK(hipMalloc(&p->d_spows, sizeof(float) * p->hitcap));
-    PCHK(hipMalloc(&p->d_recs, sizeof(PzlHostHit) * p->hitcap));
-    p->sort_temp_bytes = 0;
-    PCHK(pzl_sort_hits(nullptr, &p->sort_temp_bytes, p->d_hkeys[0], p->d_skeys, p->d_hpows[0], p->d_spows,
-                       p->hitcap, p->key_bits, p->st2));
-    PCHK(hipMalloc(&p->d_sort_temp, std::max<size_t>(p->sort_temp_bytes, 16)));
     // The kernel bank: built once per shape and kept resident.
     {
         bool built = false;
@@ -924,13 +977,21 @@ int pzl_plan_create(const PzlConfig *cfg, int device, void **plan_out)
         if (brc) return cleanup(brc);
         p->bank_built = built;
     }
-    p->g.device_bytes = plan_device_bytes(p->g, sizes, nh, (long long)p->sort_temp_bytes);
+    p->g.device_bytes = plan_device_bytes(p->g, sizes, p->harms, (long long)p->sort_temp_bytes);
 #undef PCHK
     *plan_out = p;
     return 0;
 }
 
 void pzl_plan_destroy(void *plan) { free_plan((Plan *)plan); }
+
+int pzl_plan_device_bytes(void *plan, long long *held)
+{
+    Plan *p = (Plan *)plan;
+    if (!p || !held) return fail(PZL_ERR_ARG, "null argument");
+    *held = p->dev_bytes + (p->d_bank ? p->g.bank_bytes : 0);
+    return 0;
+}
 
 int pzl_plan_geometry(void *plan, PzlGeometry *g)
 {
@@ -965,8 +1026,9 @@ int pzl_set_spectrum(void *plan, const float *host_fft, long long numbins)
     if (p->d_fft_own && p->fft_bins != numbins) {
         PZL_HIPCHK(hipFree(p->d_fft_own));
         p->d_fft_own = nullptr;
+        p->dev_bytes -= (long long)sizeof(float2) * p->fft_bins;
     }
-    if (!p->d_fft_own) PZL_HIPCHK(hipMalloc(&p->d_fft_own, sizeof(float2) * numbins));
+    if (!p->d_fft_own) PZL_HIPCHK(plan_malloc(p, &p->d_fft_own, sizeof(float2) * numbins));
     PZL_HIPCHK(hipMemcpy(p->d_fft_own, host_fft, sizeof(float2) * numbins, hipMemcpyHostToDevice));
     p->d_fft = p->d_fft_own;
     p->fft_bins = numbins;
@@ -1443,7 +1505,7 @@ int pzl_compute_sums(void *plan, long long rstep, float *out, long long maxfloat
     const long long need = (long long)p->fnumw * p->fnumz * p->g.corr_uselen;
     if (need > maxfloats) return fail(PZL_ERR_ARG, "output too small");
     PZL_HIPCHK(hipSetDevice(p->device));
-    if (!p->d_sums) PZL_HIPCHK(hipMalloc(&p->d_sums, sizeof(float) * need));
+    if (!p->d_sums) PZL_HIPCHK(plan_malloc(p, &p->d_sums, sizeof(float) * need));
     PZL_HIPCHK(hipMemsetAsync(p->d_counts, 0, 4 * sizeof(unsigned long long), p->st));
     int rc = launch_batch(*p, rstep, 1, 0, 0, p->d_sums, nullptr);
     if (rc) return rc;

@@ -233,47 +233,74 @@ template <int R, int SIGN, typename C> PZL_FFT_HD void dftR(C *v)
     else static_assert(R == 2, "radix not provided");
 }
 
-// Block-cooperative double-precision Stockham FFT of n points held in LDS
-// (buf, n <= 4096), any n = 2^a 3^b 5^c.  Each pass reads every butterfly's
-// inputs into registers, synchronises, then writes in place.  Twiddles come
-// from sincospi in double.  Used off the hot path (kernel bank, segments).
-template <int SIGN>
-__device__ void lds_fft_f64(double2 *buf, int n, int tid, int nt)
+// One Stockham pass of radix R over n points in LDS: each of NT threads reads
+// its butterflies' inputs (twiddled by exp(SIGN 2 pi i jm r / (Ns R)), from
+// sincospi in double) into registers, the block synchronises, then writes them
+// in place.  R and the butterflies per thread are compile-time, so the
+// register arrays need no scratch.
+template <int SIGN, int R, int NT>
+__device__ void lds_fft_pass_f64(double2 *buf, int n, int Ns, int tid)
 {
-    int Ns = 1;
-    constexpr int MAXB = 8;  // butterflies per thread per pass
-    while (Ns < n) {
-        const int rem = n / Ns;
-        const int R = (rem % 4 == 0) ? 4 : (rem % 2 == 0) ? 2 : (rem % 3 == 0) ? 3 : 5;
-        const int nbf = n / R;
-        double2 v[MAXB][5];
-        int cnt = 0;
-        for (int j = tid; j < nbf && cnt < MAXB; j += nt, cnt++) {
+    constexpr int MAXB = (4096 / R + NT - 1) / NT;  // butterflies per thread, n <= 4096
+    const int nbf = n / R;
+    double2 v[MAXB][R];
+#pragma unroll
+    for (int b = 0; b < MAXB; b++) {
+        const int j = tid + b * NT;
+        if (j < nbf) {
             const int jm = j % Ns;
+#pragma unroll
             for (int r = 0; r < R; r++) {
                 double2 x = buf[j + r * nbf];
                 if (Ns > 1 && r > 0) {
-                    double s, c;
-                    // exp(SIGN 2 pi i jm r / (Ns R))
-                    sincospi((double)SIGN * 2.0 * (double)(jm * r) / (double)(Ns * R), &s, &c);
-                    x = cmul(x, double2{c, s});
+                    double sn, cs;
+                    sincospi((double)SIGN * 2.0 * (double)(jm * r) / (double)(Ns * R), &sn, &cs);
+                    x = cmul(x, double2{cs, sn});
                 }
-                v[cnt][r] = x;
+                v[b][r] = x;
             }
         }
-        __syncthreads();
-        cnt = 0;
-        for (int j = tid; j < nbf && cnt < MAXB; j += nt, cnt++) {
-            double2 *w = v[cnt];
-            if (R == 4) dft4<SIGN>(w);
-            else if (R == 2) dft2<SIGN>(w[0], w[1]);
-            else if (R == 3) dft3<SIGN>(w);
-            else dft5<SIGN>(w);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < MAXB; b++) {
+        const int j = tid + b * NT;
+        if (j < nbf) {
+            if constexpr (R == 2) dft2<SIGN>(v[b][0], v[b][1]);
+            else if constexpr (R == 3) dft3<SIGN>(v[b]);
+            else if constexpr (R == 4) dft4<SIGN>(v[b]);
+            else dft5<SIGN>(v[b]);
             const int jm = j % Ns;
             const int idxD = (j / Ns) * Ns * R + jm;
-            for (int r = 0; r < R; r++) buf[idxD + r * Ns] = w[r];
+#pragma unroll
+            for (int r = 0; r < R; r++) buf[idxD + r * Ns] = v[b][r];
         }
-        __syncthreads();
-        Ns *= R;
+    }
+    __syncthreads();
+}
+
+// Block-cooperative double-precision Stockham FFT of n points held in LDS
+// (buf, n <= 4096), any n = 2^a 3^b 5^c, by NT threads: passes of radix 4
+// while 4 divides what is left, then 2, 3, 5.  Used by the kernel-bank build
+// and the segment transforms.
+template <int SIGN, int NT>
+__device__ void lds_fft_f64(double2 *buf, int n, int tid)
+{
+    int Ns = 1;
+    while (Ns < n) {
+        const int rem = n / Ns;
+        if (rem % 4 == 0) {
+            lds_fft_pass_f64<SIGN, 4, NT>(buf, n, Ns, tid);
+            Ns *= 4;
+        } else if (rem % 2 == 0) {
+            lds_fft_pass_f64<SIGN, 2, NT>(buf, n, Ns, tid);
+            Ns *= 2;
+        } else if (rem % 3 == 0) {
+            lds_fft_pass_f64<SIGN, 3, NT>(buf, n, Ns, tid);
+            Ns *= 3;
+        } else {
+            lds_fft_pass_f64<SIGN, 5, NT>(buf, n, Ns, tid);
+            Ns *= 5;
+        }
     }
 }

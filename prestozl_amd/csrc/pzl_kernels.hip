@@ -77,7 +77,7 @@ __global__ __launch_bounds__(BANK_NT) void k_bank_build(const PzlHarm *__restric
         }
     }
     __syncthreads();
-    lds_fft_f64<-1>(sbuf, fftlen, tid, BANK_NT);
+    lds_fft_f64<-1, BANK_NT>(sbuf, fftlen, tid);
     // For power-of-two lengths the bank holds K / fftlen: the correlation's
     // 1 / fftlen^2 power normalisation then comes out of the product, exactly
     // (a power-of-two scale commutes with every rounding on the way).
@@ -156,7 +156,7 @@ __global__ __launch_bounds__(SEG_NT) void k_segments(PzlBatchArgs a)
     }
     __syncthreads();
     // FFT of the fftlen-point spread array == numdata-point FFT, repeated.
-    lds_fft_f64<-1>(sbuf, numdata, tid, SEG_NT);
+    lds_fft_f64<-1, SEG_NT>(sbuf, numdata, tid);
     float2 *seg = a.segs + (long long)b * a.seg_stride + H.seg_off;
     for (int k = tid; k < numdata; k += SEG_NT) seg[k] = float2{(float)sbuf[k].x, (float)sbuf[k].y};
     if (h > 0) {

@@ -64,6 +64,8 @@ struct Plan {
     int *d_winds = nullptr;
     const float2 *d_fft = nullptr;
     float2 *d_fft_own = nullptr;
+    long long dev_bytes = 0;  // device memory this plan has allocated and holds (the bank aside)
+    char *d_arena = nullptr;  // one allocation holding every buffer below (spectrum and d_sums aside)
     long long fft_bins = 0;
     // per batch, two sets: the correlation of one batch fills one set while
     // the harmonic sum of the previous batch reads the other

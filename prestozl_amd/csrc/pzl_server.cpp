@@ -244,8 +244,10 @@ int pzl_server_submit(void *srv, const PzlConfig *cfg, const float *host_fft, lo
     r->cfg = *cfg;
     r->fft = host_fft;
     r->numbins = numbins;
-    // plan buffers + resident spectrum + its kernel bank (shared when cached)
-    r->footprint = g.device_bytes + g.bank_bytes;
+    // plan buffers, resident spectrum and kernel bank (counted per request,
+    // although requests of one shape share a cached bank), and the runtime's
+    // memory behind the plan's streams
+    r->footprint = g.device_bytes + PZL_RUNTIME_BYTES_PER_SEARCH;
     r->t_submit = Clock::now();
     {
         std::lock_guard<std::mutex> lk(s->mu);

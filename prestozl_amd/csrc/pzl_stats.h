@@ -70,14 +70,19 @@ PZL_HD double pzl_q_integer_a(double a, double x)
     if (a >= 20.0 && (a > x || x >= 31.0)) {
         const double r = exp(a * log(x) - x) / tgamma(a);
         if (r == 0.0) return 0.0;
+        // the first terms kept, then added smallest first; both loops fully
+        // unrolled so that wk stays in registers
         double wk[20], amn = a - 1.0, t = amn / x, sum;
-        int n;
+        int n = 21;  // the term count where the first loop stopped
         wk[0] = t;
-        for (n = 2; n <= 20; ++n) {
-            amn -= 1.0;
-            t *= amn / x;
-            if (fabs(t) <= 1e-3) break;
-            wk[n - 1] = t;
+#pragma unroll
+        for (int k = 2; k <= 20; ++k) {
+            if (n == 21) {
+                amn -= 1.0;
+                t *= amn / x;
+                if (fabs(t) <= 1e-3) n = k;
+                else wk[k - 1] = t;
+            }
         }
         sum = t;
         while (fabs(t) > 5e-15) {
@@ -85,7 +90,9 @@ PZL_HD double pzl_q_integer_a(double a, double x)
             t *= amn / x;
             sum += t;
         }
-        for (int m = n - 1; m >= 1; --m) sum += wk[m - 1];
+#pragma unroll
+        for (int m = 20; m >= 1; --m)
+            if (m <= n - 1) sum += wk[m - 1];
         return r / x * (1.0 + sum);
     }
     const int ia = (int)a;

@@ -65,7 +65,7 @@ def test_admission_by_footprint(config1_data):
     foot = []
     for cfg, fft in reqs:
         g = B.geometry(cfg)
-        foot.append(g.device_bytes + g.bank_bytes)
+        foot.append(g.device_bytes + B.RUNTIME_BYTES_PER_SEARCH)
     budget = int(max(foot) * 1.5)  # room for one large request and at most a small one beside it
     srv = B.Server(budget=budget, max_inflight=4)
     ids = [srv.submit(cfg, fft) for cfg, fft in reqs]
