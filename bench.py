@@ -154,6 +154,70 @@ def dm_trials_bench(n, max_inflight):
             "strongest_trial": best}
 
 
+def realfft_bench(reps=20, cpu=True):
+    """realfft (fastffts.c:163-235), the .dat -> .fft producer, on the GPU:
+    the forward transform of config 2's series (N = 2^24 samples, dt = 1e-4 s,
+    noise + 3 jerked sinusoids) and of config 5's DM-trial length (2^18),
+    device-resident (one synchronous call per transform), and through the
+    host-memory entry point (PCIe both ways).  Algorithmic bytes per transform:
+    n floats read + n floats written (8 n)."""
+    from prestozl_amd import binding as B
+    from prestozl_amd import synth
+    hip = ctypes.CDLL("libamdhip64.so")
+    res = {}
+    for label, n in (("config2_N2^24", 1 << 24), ("config5_N2^18", 1 << 18)):
+        sigs = [(0.05, 123.4, 40.0, -80.0), (0.03, 311.0, -120.0, 200.0), (0.08, 57.7, 10.0, 0.0)]
+        x = synth.timeseries(n, 1e-4, sigs, seed=21)
+        d = ctypes.c_void_p()
+        assert hip.hipMalloc(ctypes.byref(d), ctypes.c_size_t(4 * n)) == 0
+        assert hip.hipMemcpy(d, x.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(4 * n), 1) == 0
+        for _ in range(2):
+            B.realfft_device(d.value, n, -1)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            B.realfft_device(d.value, n, -1)
+        ms = (time.perf_counter() - t0) * 1e3 / reps
+        hip.hipFree(d)
+        t0 = time.perf_counter()
+        got = B.realfft(x, -1)
+        ms_host = (time.perf_counter() - t0) * 1e3
+        gbs = 8.0 * n / (ms * 1e-3) / 1e9
+        r = {"n": n, "ms_per_transform": ms, "reps": reps, "alg_bytes": 8 * n,
+             "achieved_GBs": gbs, "frac_hbm": gbs / HBM_PEAK_GBS, "ms_host_api": ms_host}
+        if cpu:
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import pyoracle as O
+            t0 = time.perf_counter()
+            ref = O.realfft(x, -1, O.FP32)
+            r["cpu_oracle_fp32_ms"] = (time.perf_counter() - t0) * 1e3
+            r["cpu_cores"] = 1
+            d64 = got.astype(np.float64) - ref
+            r["rms_rel_vs_oracle_fp32"] = float(np.sqrt(np.mean(d64 * d64) / np.mean(ref.astype(np.float64) ** 2)))
+        res[label] = r
+    # config 5: the DM trials of one dedispersion run, transformed together
+    nfiles, n = 564, 1 << 18
+    x = np.stack([synth.timeseries(n, 6.4e-4, [(0.1, 37.2, 40.0, -80.0)], seed=300 + i) for i in range(8)])
+    d = ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(d), ctypes.c_size_t(4 * n * nfiles)) == 0
+    for i in range(0, nfiles, 8):
+        k = min(8, nfiles - i)
+        assert hip.hipMemcpy(ctypes.c_void_p(d.value + 4 * n * i), x.ctypes.data_as(ctypes.c_void_p),
+                             ctypes.c_size_t(4 * n * k), 1) == 0
+    B.realfft_device(d.value, n, -1, nseries=nfiles)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        B.realfft_device(d.value, n, -1, nseries=nfiles)
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    hip.hipFree(d)
+    gbs = 8.0 * n * nfiles / (ms * 1e-3) / 1e9
+    res["config5_batch_564xN2^18"] = {"n": n, "series": nfiles, "ms_per_batch": ms, "files_per_s": nfiles / ms * 1e3,
+                                      "achieved_GBs": gbs, "frac_hbm": gbs / HBM_PEAK_GBS}
+    res["note"] = ("device-resident: one synchronous pzl_realfft_device call per transform (column FFTs, row FFTs "
+                   "written transposed, NR packing: three passes over the series); ms_host_api includes both "
+                   "PCIe copies; cpu_oracle_fp32_ms is the oracle's float realfft on one core")
+    return res
+
+
 def config3_bench(steps):
     """The batched form at zmax 200, wmax 500, numharm 16 (4096-point
     fundamental): N = 2^23 samples at dt = 1e-4 s with five injected jerked
@@ -226,6 +290,8 @@ def main():
                     help="profiling runs only: search this many rsteps from the middle of the band")
     ap.add_argument("--serve-only", action="store_true",
                     help="run only the served-stream leg (--serve requests at --serve-rate) and print its line")
+    ap.add_argument("--realfft-only", action="store_true",
+                    help="run only the realfft (.dat -> .fft) leg and print its line")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
                     help="run a plain device copy of MIB MiB five times and exit (HBM counter calibration)")
     args = ap.parse_args()
@@ -242,6 +308,9 @@ def main():
 
     if args.serve_only:
         print(json.dumps(serve_bench(args.serve, args.serve_rate, args.serve_inflight)), flush=True)
+        return
+    if args.realfft_only:
+        print(json.dumps(realfft_bench(cpu=not args.no_cpu)), flush=True)
         return
     fft, N, dt, opts, sigs = synth.config2(N=1 << args.log2n)
     kw = dict(N=N, dt=dt, numbins=len(fft), zmax=opts["zmax"], wmax=opts["wmax"], numharm=opts["numharm"],
@@ -382,6 +451,8 @@ def main():
         out["config3"] = config3_bench(1)
     if args.dm_trials and not args.rsteps:
         out["dm_trials"] = dm_trials_bench(args.dm_trials, args.serve_inflight)
+    if not args.no_legs and not args.rsteps:
+        out["realfft"] = realfft_bench(cpu=not args.no_cpu)
     if not args.no_cpu:
         nthreads = args.cpu_threads or min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(fft, kw, args.cpu_rsteps, nthreads,

@@ -267,6 +267,21 @@ int pzl_server_stats(void *server, PzlServerStats *stats);
 /* Finish every queued and running request, then free the pooled plans. */
 void pzl_server_destroy(void *server);
 
+/* ---------------- realfft: the .dat -> .fft producer ----------------------
+ * realfft(idata, n, isign) (fastffts.c:163-235) on the GPU, in place on n
+ * floats: isign -1 is the forward transform, n/2 bins in the Numerical
+ * Recipes packing (bin 0 = (DC, Nyquist)), unnormalised; +1 the inverse, every
+ * value times 2/n.  n even, n/2 a product of 2, 3, 5, 7 and 11 up to 2^26
+ * (the lengths choose_good_N picks, misc_utils.c:327-366; the reference,
+ * through FFTW, takes any even n).  Calls on one device run one at
+ * a time and return when the transform is done. */
+int pzl_realfft(float *idata, long long n, int isign);      /* host memory */
+int pzl_realfft_device(int device, void *d_data, long long n, int isign);
+/* nseries (<= 65535) series of n floats each, back to back, in one set of
+ * launches: the DM trials of a dedispersion run (realfft per .dat file,
+ * realfft.c:66-251). */
+int pzl_realfft_device_batch(int device, void *d_data, long long n, int nseries, int isign);
+
 #ifdef __cplusplus
 }
 #endif

@@ -11,7 +11,7 @@
  * rounded to the working precision, as FFTW does.
  *
  * realfft() follows fastffts.c:163-235 (the Numerical Recipes real-FFT
- * unpacking with the corrected exponent sign), forward direction only.
+ * unpacking with the corrected exponent sign), forward and inverse.
  */
 #include <math.h>
 #include <pthread.h>
@@ -238,3 +238,51 @@ DEFINE_CFFT(d, double, twd, rootd)
 
 DEFINE_REALFFT(f, float)
 DEFINE_REALFFT(d, double)
+
+/* fastffts.c:163-235, isign = +1 branch: the NR packing undone (with the
+ * sign flips of bin n/4 before and after the unpacking loop, :186-188 and
+ * :224-226), the unnormalised inverse FFT of n/2 points, then every value
+ * times 2/n. */
+#define DEFINE_REALFFT_INV(SUFFIX, REAL)                                                      \
+    void oracle_realfft_inv_##SUFFIX(REAL *idata, long n)                                     \
+    {                                                                                         \
+        long nby2 = n >> 1, il, ih;                                                           \
+        double cc = 0.5, h1r, h1i, h2r, h2i, h2rwr, h2iwr, h2rwi, h2iwi;                      \
+        double wr, wi, wpr, wpi, tmp1, theta = 2.0 * M_PI / (double)n;                        \
+        REAL *d = idata;                                                                      \
+        if ((n + 2) % 4) d[2 * (n >> 2) + 1] = -d[2 * (n >> 2) + 1];                          \
+        wr = cos(theta);                                                                      \
+        wi = sin(theta);                                                                      \
+        tmp1 = sin(0.5 * theta);                                                              \
+        wpr = -2.0 * tmp1 * tmp1;                                                             \
+        wpi = wi;                                                                             \
+        il = 1;                                                                               \
+        ih = nby2 - il;                                                                       \
+        for (; il <= (n >> 2); il++, ih--) {                                                  \
+            h1r = 0.5 * (d[2 * il] + d[2 * ih]);                                              \
+            h1i = 0.5 * (d[2 * il + 1] - d[2 * ih + 1]);                                      \
+            h2r = -cc * (d[2 * il + 1] + d[2 * ih + 1]);                                      \
+            h2i = cc * (d[2 * il] - d[2 * ih]);                                               \
+            h2rwr = h2r * wr;                                                                 \
+            h2rwi = h2r * wi;                                                                 \
+            h2iwr = h2i * wr;                                                                 \
+            h2iwi = h2i * wi;                                                                 \
+            d[2 * il] = (REAL)(h1r + h2rwr - h2iwi);                                          \
+            d[2 * il + 1] = (REAL)(h1i + h2iwr + h2rwi);                                      \
+            d[2 * ih] = (REAL)(h1r - h2rwr + h2iwi);                                          \
+            d[2 * ih + 1] = (REAL)(-h1i + h2iwr + h2rwi);                                     \
+            tmp1 = wr;                                                                        \
+            wr = tmp1 * wpr - wi * wpi + wr;                                                  \
+            wi = wi * wpr + tmp1 * wpi + wi;                                                  \
+        }                                                                                     \
+        if ((n + 2) % 4) d[2 * (n >> 2) + 1] = -d[2 * (n >> 2) + 1];                          \
+        tmp1 = d[0];                                                                          \
+        d[0] = (REAL)(0.5 * (tmp1 + d[1]));                                                   \
+        d[1] = (REAL)(0.5 * (tmp1 - d[1]));                                                   \
+        oracle_cfft_##SUFFIX(d, (int)nby2, 1);                                                \
+        tmp1 = 2.0 / (double)n;                                                               \
+        for (il = 0; il < n; il++) d[il] = (REAL)(d[il] * tmp1);                              \
+    }
+
+DEFINE_REALFFT_INV(f, float)
+DEFINE_REALFFT_INV(d, double)

@@ -30,9 +30,12 @@ enum { ORACLE_FP32 = 0, ORACLE_FP64 = 1 };
 /* sign -1 = forward (e^{-2 pi i jk/n}), +1 = unnormalised inverse. */
 void oracle_cfft_f(float *data, int n, int sign);
 void oracle_cfft_d(double *data, int n, int sign);
-/* realfft, fastffts.c:163-235: forward (isign -1) only, NR packing. */
+/* realfft, fastffts.c:163-235, NR packing: forward (isign -1) and inverse
+ * (isign +1, scaled by 2/n). */
 void oracle_realfft_fwd_f(float *data, long n);
 void oracle_realfft_fwd_d(double *data, long n);
+void oracle_realfft_inv_f(float *data, long n);
+void oracle_realfft_inv_d(double *data, long n);
 
 /* ---------------- sizing helpers ---------------------------------------- */
 int oracle_next2_to_n(int x);

@@ -75,6 +75,8 @@ def lib():
         L.oracle_cfft_d.argtypes = [P(C.c_double), C.c_int, C.c_int]
         L.oracle_realfft_fwd_f.argtypes = [P(C.c_float), C.c_long]
         L.oracle_realfft_fwd_d.argtypes = [P(C.c_double), C.c_long]
+        L.oracle_realfft_inv_f.argtypes = [P(C.c_float), C.c_long]
+        L.oracle_realfft_inv_d.argtypes = [P(C.c_double), C.c_long]
         for fn in ("oracle_next2_to_n", "oracle_next_good_fftlen", "oracle_fftlen_from_kernwidth"):
             getattr(L, fn).argtypes = [C.c_int]
             getattr(L, fn).restype = C.c_int
@@ -336,6 +338,18 @@ def fresnel(x):
 def median(a):
     a = np.ascontiguousarray(a, dtype=np.float32)
     return lib().oracle_median(fptr(a), len(a))
+
+
+def realfft(x, isign, mode=FP32):
+    """realfft (fastffts.c:163-235) of n floats in place semantics: returns the
+    transformed copy (isign -1: NR-packed spectrum as n floats; +1: inverse,
+    scaled by 2/n)."""
+    dt = np.float64 if mode == FP64 else np.float32
+    a = np.array(x, dtype=dt, copy=True)
+    fn = {(-1, FP32): "oracle_realfft_fwd_f", (-1, FP64): "oracle_realfft_fwd_d",
+          (1, FP32): "oracle_realfft_inv_f", (1, FP64): "oracle_realfft_inv_d"}[(isign, mode)]
+    getattr(lib(), fn)(dptr(a) if mode == FP64 else fptr(a), len(a))
+    return a
 
 
 def cfft(x, sign, mode=FP32):

@@ -114,6 +114,9 @@ def lib():
         L.pzl_last_error.restype = C.c_char_p
         L.pzl_device_count.argtypes = [P(C.c_int)]
         L.pzl_device_memory.argtypes = [C.c_int, P(C.c_longlong), P(C.c_longlong)]
+        L.pzl_realfft.argtypes = [P(C.c_float), C.c_longlong, C.c_int]
+        L.pzl_realfft_device.argtypes = [C.c_int, C.c_void_p, C.c_longlong, C.c_int]
+        L.pzl_realfft_device_batch.argtypes = [C.c_int, C.c_void_p, C.c_longlong, C.c_int, C.c_int]
         L.pzl_geometry_from_config.argtypes = [P(Config), P(Geometry)]
         L.pzl_plan_create.argtypes = [P(Config), C.c_int, P(C.c_void_p)]
         L.pzl_plan_destroy.argtypes = [C.c_void_p]
@@ -180,6 +183,21 @@ def make_config(N, dt, numbins, zmax, wmax, numharm, sigma, wmaxP=True, flo=1.0,
 
 
 RUNTIME_BYTES_PER_SEARCH = 32 << 20  # PZL_RUNTIME_BYTES_PER_SEARCH (pzl_accel.h)
+
+
+def realfft(x, isign=-1):
+    """realfft (fastffts.c:163-235) of n float32 values on the GPU: returns the
+    transformed copy (isign -1: the NR-packed spectrum as n floats, view it as
+    complex64 for the n/2 bins; +1: the inverse, scaled by 2/n)."""
+    a = np.array(x, dtype=np.float32, copy=True)
+    check(lib().pzl_realfft(a.ctypes.data_as(C.POINTER(C.c_float)), len(a), int(isign)))
+    return a
+
+
+def realfft_device(ptr, n, isign=-1, device=0, nseries=1):
+    """The same in place on device memory at 'ptr': nseries series of n floats
+    each, back to back."""
+    check(lib().pzl_realfft_device_batch(device, C.c_void_p(ptr), int(n), int(nseries), int(isign)))
 
 
 def device_memory(device=0):

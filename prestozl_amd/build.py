@@ -17,7 +17,7 @@ LIBDIR = os.path.join(HERE, "lib")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PZL_ARCH", "gfx950")
 
-SOURCES = ["pzl_kernels.hip", "pzl_optimize.hip", "pzl_engine.cpp", "pzl_optimize.cpp", "pzl_geometry.cpp",
+SOURCES = ["pzl_kernels.hip", "pzl_optimize.hip", "pzl_realfft.hip", "pzl_engine.cpp", "pzl_optimize.cpp", "pzl_geometry.cpp",
            "pzl_stats.cpp", "pzl_server.cpp",
            "accel_api.cpp", "accel_output.cpp"]
 GLIB_INC = ["/opt/conda/include/glib-2.0", "/opt/conda/lib/glib-2.0/include"]
@@ -73,6 +73,18 @@ def build_driver(lib=None):
     return out
 
 
+def build_realfft(lib=None):
+    """The realfft command (in-core path of realfft.c) on the GPU transform."""
+    lib = lib or os.path.join(LIBDIR, "libaccelsearch_lib.so")
+    out = os.path.join(LIBDIR, "realfft")
+    cmd = ["gcc", "-O2", "-I" + os.path.join(ROOT, "include"), os.path.join(CSRC, "realfft_main.c"),
+           "-o", out + ".tmp", "-L" + LIBDIR, "-l:" + os.path.basename(lib), "-Wl,-rpath,$ORIGIN",
+           "-Wl,--allow-shlib-undefined"] + GLIB_LINK
+    _run(cmd)
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_oracle():
     _run(["make", "-C", os.path.join(ROOT, "oracle")])
     if os.path.isdir("/root/reference/src"):
@@ -82,6 +94,7 @@ def build_oracle():
 def build_all():
     build_library()
     build_driver()
+    build_realfft()
     build_oracle()
 
 

@@ -157,6 +157,54 @@ constexpr double ct_cos2pi(long m, long R)
 }
 constexpr double ct_sin2pi(long m, long R) { return ct_cos2pi(4 * m - R, 4 * R); }
 
+// cos and sin of 2 pi m / R, m < R, as compile-time tables
+template <int R> struct ct_roots {
+    double c[R], s[R];
+    constexpr ct_roots() : c(), s()
+    {
+        for (int m = 0; m < R; m++) {
+            c[m] = ct_cos2pi(m, R);
+            s[m] = ct_sin2pi(m, R);
+        }
+    }
+};
+
+// DFT of odd size R by the symmetric pairs j, R - j:
+//   X[k] = v0 + sum_j (v_j + v_{R-j}) cos(2 pi j k / R)
+//             + SIGN i sum_j (v_j - v_{R-j}) sin(2 pi j k / R),  j = 1 .. (R-1)/2,
+// and X[R - k] the same with the sine sum's sign reversed.
+template <int R, int SIGN, typename C> PZL_FFT_HD void dft_odd(C *v)
+{
+    typedef decltype(v[0].x) T;
+    constexpr int H = (R - 1) / 2;
+    constexpr ct_roots<R> rt{};
+    C sp[H], dm[H];
+#pragma unroll
+    for (int j = 1; j <= H; j++) {
+        sp[j - 1] = cadd(v[j], v[R - j]);
+        dm[j - 1] = csub(v[j], v[R - j]);
+    }
+    const C x0 = v[0];
+    C y0 = x0;
+#pragma unroll
+    for (int j = 0; j < H; j++) y0 = cadd(y0, sp[j]);
+#pragma unroll
+    for (int k = 1; k <= H; k++) {
+        T ar = x0.x, ai = x0.y, br = 0, bi = 0;
+#pragma unroll
+        for (int j = 1; j <= H; j++) {
+            const T c = (T)rt.c[(j * k) % R], sn = (T)rt.s[(j * k) % R];
+            ar += sp[j - 1].x * c;
+            ai += sp[j - 1].y * c;
+            br += dm[j - 1].x * sn;
+            bi += dm[j - 1].y * sn;
+        }
+        v[k] = C{ar - (T)SIGN * bi, ai + (T)SIGN * br};
+        v[R - k] = C{ar + (T)SIGN * bi, ai - (T)SIGN * br};
+    }
+    v[0] = y0;
+}
+
 // x * exp(SIGN 2 pi i M / R) with the trivial factors done without multiplies
 template <int M, int R, int SIGN, typename C> PZL_FFT_HD C twc(C x)
 {

@@ -134,6 +134,54 @@ def test_realfft_packing():
         assert np.abs(packed[1:] - X[1:n // 2]).max() < 1e-10
 
 
+def test_realfft_roundtrip_known_answer():
+    """The reference's own known answer (tests/test_presto_python.py:10-13):
+    rfft(rfft(x, -1), 1) == x to 1e-6 at N = 20, in float; and at odd n / 2
+    (n % 4 == 2), mixed-radix and power-of-two lengths, float and double."""
+    rng = np.random.default_rng(11)
+    for n in (20, 22, 30, 1000, 3000, 4096):
+        x = rng.standard_normal(n)
+        y = O.realfft(O.realfft(x, -1, O.FP32), 1, O.FP32)
+        assert np.allclose(x, y, atol=1e-6 * np.sqrt(np.log2(n) / np.log2(20)) * 3), n
+        y = O.realfft(O.realfft(x, -1, O.FP64), 1, O.FP64)
+        assert np.abs(x - y).max() < 1e-12, n
+
+
+def test_realfft_inverse_is_numpy_irfft():
+    """realfft(+1) of an NR-packed spectrum is numpy's irfft of the unpacked
+    one: the 2/n scale of an n/2-point inverse FFT is irfft's 1/n.  Bin n/4
+    is where the reference flips signs around its unpacking loop (n % 4 == 0)."""
+    rng = np.random.default_rng(12)
+    for n in (20, 22, 64, 1000, 3000):
+        X = rng.standard_normal(n // 2 + 1) + 1j * rng.standard_normal(n // 2 + 1)
+        X[0] = X[0].real
+        X[n // 2] = X[n // 2].real
+        packed = np.empty(n)
+        packed[0], packed[1] = X[0].real, X[n // 2].real
+        packed[2::2], packed[3::2] = X[1:n // 2].real, X[1:n // 2].imag
+        got = O.realfft(packed, 1, O.FP64)
+        assert np.abs(got - np.fft.irfft(X, n)).max() < 1e-12, n
+
+
+def test_realfft_closed_forms():
+    """cos(2 pi f t / n) -> n/2 at bin f (real); sin -> -n/2 (imaginary); a
+    constant c -> n c in bin 0's real part; (-1)^t -> n in bin 0's imaginary
+    part (the Nyquist slot)."""
+    for n in (24, 1000, 4096):
+        t = np.arange(n)
+        for f in (1, n // 4, n // 2 - 1):
+            a = O.realfft(np.cos(2 * np.pi * f * t / n), -1, O.FP64)
+            b = O.realfft(np.sin(2 * np.pi * f * t / n), -1, O.FP64)
+            ea = np.zeros(n); ea[2 * f] = n / 2
+            eb = np.zeros(n); eb[2 * f + 1] = -n / 2
+            tol = 1e-12 * n  # the double trig recursion of the unpacking over n/4 steps
+            assert np.abs(a - ea).max() < tol and np.abs(b - eb).max() < tol, (n, f)
+        a = O.realfft(np.full(n, 0.75), -1, O.FP64)
+        assert abs(a[0] - 0.75 * n) < tol and np.abs(a[1:]).max() < tol
+        a = O.realfft((-1.0) ** t, -1, O.FP64)
+        assert abs(a[1] - n) < tol and abs(a[0]) < tol and np.abs(a[2:]).max() < tol
+
+
 # ---------------- statistics -------------------------------------------------
 
 def test_median_is_lower_order_statistic():
