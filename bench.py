@@ -218,6 +218,78 @@ def realfft_bench(reps=20, cpu=True):
     return res
 
 
+def dedisp_bench(cpu=True, nsearch=48):
+    """The DM-trial producer on the GPU, device-resident end to end: 64
+    subbands of N = 2^18 samples (dt = 6.4e-4 s, blocks of 16384 as
+    prepsubband_cu reads them) with a sinusoid dispersed at trial 282 of 564,
+    dedispersed into 564 trials (getDedisp per block, prepsubband_cu.c:
+    1117-1164), one batched realfft, then searches (zmax 200, wmax 300, numharm
+    8, sigma 5) of 'nsearch' trials around 282 straight from device memory.
+    Algorithmic bytes of the dedispersion: the subbands read once and the
+    trials written once."""
+    from prestozl_amd import binding as B
+    rng = np.random.default_rng(282)
+    nsub, numdms, worklen, nsamp, dt = 64, 564, 16384, 1 << 18, 6.4e-4
+    f = np.linspace(0.6, 1.0, nsub)
+    d = 1.0 / f ** 2 - 1.0 / f[-1] ** 2
+    offs = np.rint(np.linspace(0.0, 1.0, numdms)[:, None] * d[None, :] / d.max() * (worklen // 2)).astype(np.int32)
+    t = np.arange(nsamp)
+    sub = rng.standard_normal((nsamp, nsub)).astype(np.float32)
+    for s in range(nsub):
+        sub[:, s] += (0.15 * np.cos(2 * np.pi * 37.2 * (t - offs[282, s]) * dt)).astype(np.float32)
+    ds, do, dser = B.DeviceArray(sub.nbytes), B.DeviceArray(offs.nbytes), B.DeviceArray(numdms * nsamp * 4)
+    ds.upload(sub)
+    do.upload(offs)
+    B.dedisperse(ds.ptr, nsamp, nsub, worklen, numdms, do.ptr, dser.ptr)  # warm
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        B.dedisperse(ds.ptr, nsamp, nsub, worklen, numdms, do.ptr, dser.ptr)
+    ms_dd = (time.perf_counter() - t0) * 1e3 / reps
+    check = dser.download((2, nsamp), np.float32, offset=4 * nsamp * 282)
+    t0 = time.perf_counter()
+    B.realfft_device(dser.ptr, nsamp, -1, nseries=numdms)
+    ms_fft = (time.perf_counter() - t0) * 1e3
+    cfg = B.make_config(N=nsamp, dt=dt, numbins=nsamp // 2, zmax=200, wmax=300, numharm=8, sigma=5.0, wmaxP=True)
+    plan = B.Plan(cfg)
+    lo = 282 - nsearch // 2
+    plan.set_spectrum_device(dser.ptr + lo * nsamp * 4, nsamp // 2)
+    plan.search()  # warm
+    best = (0.0, -1, 0.0)
+    t0 = time.perf_counter()
+    for dm in range(lo, lo + nsearch):
+        plan.set_spectrum_device(dser.ptr + dm * nsamp * 4, nsamp // 2)
+        cands, _ = plan.search(stats=False)
+        for c in cands:
+            if c[1] > best[0]:
+                best = (c[1], dm, c[3] / (nsamp * dt))
+    ms_search = (time.perf_counter() - t0) * 1e3 / nsearch
+    plan.close()
+    alg = 4.0 * nsamp * nsub + 4.0 * nsamp * numdms
+    r = {"workload": "64 subbands x N=2^18 (dt=6.4e-4 s) -> 564 DM trials (blocks of 16384), realfft of every trial, "
+                     "search of %d trials from device memory" % nsearch,
+         "ms_dedisperse": ms_dd, "dm_samples_per_s": numdms * nsamp / ms_dd * 1e3,
+         "alg_bytes": alg, "achieved_GBs": alg / (ms_dd * 1e-3) / 1e9, "frac_hbm": alg / (ms_dd * 1e-3) / 1e9 / HBM_PEAK_GBS,
+         "ms_realfft_batch": ms_fft, "ms_search_per_trial": ms_search,
+         "strongest": {"sigma": best[0], "trial": best[1], "freq_hz": best[2]}}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as O
+        t0 = time.perf_counter()
+        got = O.dedisp(sub[worklen * 8:worklen * 9], sub[worklen * 7:worklen * 8], offs[:64])
+        cms = (time.perf_counter() - t0) * 1e3
+        r["cpu_baseline"] = {"ms_per_block_of_64_trials": cms, "cores": 1, "kind": "port",
+                             "sample": "oracle getDedisp on one block (16384 samples, 64 subbands) for 64 trials",
+                             "ms_dedisperse_scaled": cms * (numdms / 64.0) * (nsamp / worklen)}
+        r["bit_identical_block_7"] = bool(np.array_equal(
+            O.dedisp(sub[worklen * 8:worklen * 9], sub[worklen * 7:worklen * 8], offs[282:284]),
+            check[:, worklen * 7:worklen * 8]))
+    ds.free()
+    do.free()
+    dser.free()
+    return r
+
+
 def config3_bench(steps):
     """The batched form at zmax 200, wmax 500, numharm 16 (4096-point
     fundamental): N = 2^23 samples at dt = 1e-4 s with five injected jerked
@@ -292,6 +364,8 @@ def main():
                     help="run only the served-stream leg (--serve requests at --serve-rate) and print its line")
     ap.add_argument("--realfft-only", action="store_true",
                     help="run only the realfft (.dat -> .fft) leg and print its line")
+    ap.add_argument("--dedisp-only", action="store_true",
+                    help="run only the dedispersion -> realfft -> search leg and print its line")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
                     help="run a plain device copy of MIB MiB five times and exit (HBM counter calibration)")
     args = ap.parse_args()
@@ -311,6 +385,9 @@ def main():
         return
     if args.realfft_only:
         print(json.dumps(realfft_bench(cpu=not args.no_cpu)), flush=True)
+        return
+    if args.dedisp_only:
+        print(json.dumps(dedisp_bench(cpu=not args.no_cpu)), flush=True)
         return
     fft, N, dt, opts, sigs = synth.config2(N=1 << args.log2n)
     kw = dict(N=N, dt=dt, numbins=len(fft), zmax=opts["zmax"], wmax=opts["wmax"], numharm=opts["numharm"],
@@ -453,6 +530,7 @@ def main():
         out["dm_trials"] = dm_trials_bench(args.dm_trials, args.serve_inflight)
     if not args.no_legs and not args.rsteps:
         out["realfft"] = realfft_bench(cpu=not args.no_cpu)
+        out["dedisp"] = dedisp_bench(cpu=not args.no_cpu)
     if not args.no_cpu:
         nthreads = args.cpu_threads or min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(fft, kw, args.cpu_rsteps, nthreads,

@@ -282,6 +282,31 @@ int pzl_realfft_device(int device, void *d_data, long long n, int isign);
  * realfft.c:66-251). */
 int pzl_realfft_device_batch(int device, void *d_data, long long n, int nseries, int isign);
 
+/* ---------------- dedispersion: the DM-trial producer ---------------------
+ * prepsubband_cu's GPU kernels (prepsubband_kernels.cu:23-24) on device
+ * memory, synchronous, with their argument meaning and layouts:
+ * DownsampDataArray (:26-46, :79-83): currentdsdata[t][s] = mean of bs
+ * consecutive samples of currentdata[t bs + j][s], t < worklen, s < nsub.
+ * getDedisp (:48-77, :85-90): outdata[dm][x] = sum over subbands s (ascending)
+ * of the sample x + offsets[dm][s] of the block pair (lastdsdata, then
+ * currentdsdata), each [worklen][nsub]; offsets [numdms][nsub] on the device,
+ * each in [0, worklen] (an offset outside is an error; the reference would
+ * read outside its blocks).  Results are bit-identical to the reference's. */
+int pzl_downsamp_data_array(float *currentdsdata_gpu, const float *currentdata_gpu, int worklen, int bs, int nsub);
+int pzl_get_dedisp(const float *currentdsdata_gpu, const float *lastdsdata_gpu, float *outdata_gpu, int worklen,
+                   int nsub, int numdms, const int *offsets_gpu);
+/* The same with output rows out_stride floats apart (>= worklen), so that
+ * consecutive blocks land in one [numdms][series] array. */
+int pzl_get_dedisp_strided(const float *currentdsdata_gpu, const float *lastdsdata_gpu, float *outdata_gpu,
+                           int worklen, int nsub, int numdms, const int *offsets_gpu, long long out_stride);
+/* prepsubband_cu.c's block loop (:1117-1164) over a whole subband series in
+ * device memory: subbands [nsamp][nsub] (nsamp whole blocks of worklen) ->
+ * out [numdms][nsamp]; output block k reads blocks k and k + 1 (zeros after
+ * the last, as the reference pads a short read).  The DM trials stay in HBM
+ * for pzl_realfft_device_batch and the search. */
+int pzl_dedisperse(const float *subbands_gpu, long long nsamp, int nsub, int worklen, int numdms,
+                   const int *offsets_gpu, float *out_gpu);
+
 #ifdef __cplusplus
 }
 #endif

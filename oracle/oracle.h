@@ -37,6 +37,11 @@ void oracle_realfft_fwd_d(double *data, long n);
 void oracle_realfft_inv_f(float *data, long n);
 void oracle_realfft_inv_d(double *data, long n);
 
+/* ---------------- dedispersion (prepsubband_kernels.cu:26-77) ----------- */
+void oracle_downsamp(float *out, const float *in, int worklen, int bs, int nsub);
+void oracle_dedisp(const float *cur, const float *last, float *out, int worklen, int nsub, int numdms,
+                   const int *offsets);
+
 /* ---------------- sizing helpers ---------------------------------------- */
 int oracle_next2_to_n(int x);
 int oracle_next_good_fftlen(int n);                 /* corr_prep.c:3-16 */

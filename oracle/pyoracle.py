@@ -77,6 +77,9 @@ def lib():
         L.oracle_realfft_fwd_d.argtypes = [P(C.c_double), C.c_long]
         L.oracle_realfft_inv_f.argtypes = [P(C.c_float), C.c_long]
         L.oracle_realfft_inv_d.argtypes = [P(C.c_double), C.c_long]
+        L.oracle_downsamp.argtypes = [P(C.c_float), P(C.c_float), C.c_int, C.c_int, C.c_int]
+        L.oracle_dedisp.argtypes = [P(C.c_float), P(C.c_float), P(C.c_float), C.c_int, C.c_int, C.c_int,
+                                    P(C.c_int)]
         for fn in ("oracle_next2_to_n", "oracle_next_good_fftlen", "oracle_fftlen_from_kernwidth"):
             getattr(L, fn).argtypes = [C.c_int]
             getattr(L, fn).restype = C.c_int
@@ -350,6 +353,28 @@ def realfft(x, isign, mode=FP32):
           (1, FP32): "oracle_realfft_inv_f", (1, FP64): "oracle_realfft_inv_d"}[(isign, mode)]
     getattr(lib(), fn)(dptr(a) if mode == FP64 else fptr(a), len(a))
     return a
+
+
+def downsamp(data, bs):
+    """DownsampDataArray (prepsubband_kernels.cu:26-46): data [worklen*bs][nsub]
+    float32 -> [worklen][nsub]."""
+    a = np.ascontiguousarray(data, dtype=np.float32)
+    worklen, nsub = a.shape[0] // bs, a.shape[1]
+    out = np.empty((worklen, nsub), dtype=np.float32)
+    lib().oracle_downsamp(fptr(out), fptr(a), worklen, bs, nsub)
+    return out
+
+
+def dedisp(cur, last, offsets):
+    """getDedisp (prepsubband_kernels.cu:48-77): cur, last [worklen][nsub],
+    offsets [numdms][nsub] -> [numdms][worklen]."""
+    c = np.ascontiguousarray(cur, dtype=np.float32)
+    l = np.ascontiguousarray(last, dtype=np.float32)
+    o = np.ascontiguousarray(offsets, dtype=np.int32)
+    worklen, nsub = c.shape
+    out = np.empty((o.shape[0], worklen), dtype=np.float32)
+    lib().oracle_dedisp(fptr(c), fptr(l), fptr(out), worklen, nsub, o.shape[0], o.ctypes.data_as(C.POINTER(C.c_int)))
+    return out
 
 
 def cfft(x, sign, mode=FP32):

@@ -117,6 +117,11 @@ def lib():
         L.pzl_realfft.argtypes = [P(C.c_float), C.c_longlong, C.c_int]
         L.pzl_realfft_device.argtypes = [C.c_int, C.c_void_p, C.c_longlong, C.c_int]
         L.pzl_realfft_device_batch.argtypes = [C.c_int, C.c_void_p, C.c_longlong, C.c_int, C.c_int]
+        L.pzl_downsamp_data_array.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int]
+        L.pzl_get_dedisp.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p]
+        L.pzl_get_dedisp_strided.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                             C.c_void_p, C.c_longlong]
+        L.pzl_dedisperse.argtypes = [C.c_void_p, C.c_longlong, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
         L.pzl_geometry_from_config.argtypes = [P(Config), P(Geometry)]
         L.pzl_plan_create.argtypes = [P(Config), C.c_int, P(C.c_void_p)]
         L.pzl_plan_destroy.argtypes = [C.c_void_p]
@@ -192,6 +197,65 @@ def realfft(x, isign=-1):
     a = np.array(x, dtype=np.float32, copy=True)
     check(lib().pzl_realfft(a.ctypes.data_as(C.POINTER(C.c_float)), len(a), int(isign)))
     return a
+
+
+class DeviceArray:
+    """A device allocation of the library's HIP runtime (test and bench
+    plumbing for the device-pointer entry points)."""
+
+    def __init__(self, nbytes):
+        self._hip = C.CDLL("libamdhip64.so")
+        lib()
+        self.nbytes = int(nbytes)
+        self._p = C.c_void_p()
+        if self._hip.hipMalloc(C.byref(self._p), C.c_size_t(max(self.nbytes, 1))) != 0:
+            raise PzlError(-2, "hipMalloc of %d bytes failed" % self.nbytes)
+
+    @property
+    def ptr(self):
+        return self._p.value
+
+    def upload(self, a, offset=0):
+        a = np.ascontiguousarray(a)
+        assert offset + a.nbytes <= self.nbytes
+        if self._hip.hipMemcpy(C.c_void_p(self.ptr + offset), a.ctypes.data_as(C.c_void_p), C.c_size_t(a.nbytes), 1):
+            raise PzlError(-2, "hipMemcpy to the device failed")
+
+    def download(self, shape, dtype, offset=0):
+        out = np.empty(shape, dtype=dtype)
+        assert offset + out.nbytes <= self.nbytes
+        if self._hip.hipMemcpy(out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr + offset), C.c_size_t(out.nbytes), 2):
+            raise PzlError(-2, "hipMemcpy from the device failed")
+        return out
+
+    def free(self):
+        if self._p.value:
+            self._hip.hipFree(self._p)
+            self._p = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def downsamp_data_array(d_out, d_in, worklen, bs, nsub):
+    check(lib().pzl_downsamp_data_array(C.c_void_p(d_out), C.c_void_p(d_in), worklen, bs, nsub))
+
+
+def get_dedisp(d_cur, d_last, d_out, worklen, nsub, numdms, d_offsets, out_stride=None):
+    if out_stride is None:
+        check(lib().pzl_get_dedisp(C.c_void_p(d_cur), C.c_void_p(d_last), C.c_void_p(d_out), worklen, nsub, numdms,
+                                   C.c_void_p(d_offsets)))
+    else:
+        check(lib().pzl_get_dedisp_strided(C.c_void_p(d_cur), C.c_void_p(d_last), C.c_void_p(d_out), worklen, nsub,
+                                           numdms, C.c_void_p(d_offsets), int(out_stride)))
+
+
+def dedisperse(d_subbands, nsamp, nsub, worklen, numdms, d_offsets, d_out):
+    check(lib().pzl_dedisperse(C.c_void_p(d_subbands), int(nsamp), nsub, worklen, numdms, C.c_void_p(d_offsets),
+                               C.c_void_p(d_out)))
 
 
 def realfft_device(ptr, n, isign=-1, device=0, nseries=1):

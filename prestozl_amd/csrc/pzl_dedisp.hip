@@ -1,0 +1,197 @@
+// pzl_dedisp.hip -- prepsubband's dedispersion on the GPU (the producer of
+// the DM trials the search reads): DownsampDataArray and getDedisp
+// (prepsubband_kernels.cu:23-90), driven by prepsubband_cu.c:1143-1164.
+//
+//   k_dd_downsamp   ds[t][s] = (sum_{j < bs} in[t bs + j][s]) / bs, one thread
+//                   per output, subbands fastest (coalesced)
+//   k_dd_transpose  the previous and the current block, [t][nsub] each, into
+//                   one subband-major window S[s][0 .. 2 worklen)
+//   k_dd_sum        out[dm][x] = sum_s S[s][x + offsets[dm][s]], subbands in
+//                   ascending order (the reference's float summation order)
+//
+// Both reference kernels read time-major data; the sums are formed in the same
+// order from the same values, so results are bit-identical.
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <vector>
+
+#include "pzl_accel.h"
+#include "pzl_plan.h"
+
+#define DD_NT 256
+#define DD_TILE 32
+#define DD_XPT 4  // outputs per thread in k_dd_sum
+
+namespace {
+
+__global__ __launch_bounds__(DD_NT) void k_dd_downsamp(float *__restrict__ out, const float *__restrict__ in,
+                                                       int worklen, int bs, int nsub)
+{
+    const long long g = (long long)blockIdx.x * DD_NT + threadIdx.x;
+    if (g >= (long long)worklen * nsub) return;
+    const int t = (int)(g / nsub), s = (int)(g - (long long)t * nsub);
+    const float *p = in + (long long)t * bs * nsub + s;
+    float v = 0.0f;
+    for (int j = 0; j < bs; j++) v += p[(long long)j * nsub];
+    out[g] = v / (float)bs;
+}
+
+// S[s][t] = last[t][s] for t < worklen, current[t - worklen][s] after, by
+// 32 x 32 tiles through LDS.
+__global__ __launch_bounds__(DD_TILE * 8) void k_dd_transpose(float *__restrict__ S, const float *__restrict__ last,
+                                                              const float *__restrict__ cur, int worklen, int nsub)
+{
+    __shared__ float tile[DD_TILE][DD_TILE + 1];
+    const int t0 = blockIdx.x * DD_TILE, s0 = blockIdx.y * DD_TILE;
+    const int tx = threadIdx.x % DD_TILE, ty = threadIdx.x / DD_TILE;  // 32 x 8
+    for (int r = ty; r < DD_TILE; r += 8) {
+        const int t = t0 + r, s = s0 + tx;
+        float v = 0.0f;
+        if (t < 2 * worklen && s < nsub)
+            v = t < worklen ? last[(long long)t * nsub + s] : cur[(long long)(t - worklen) * nsub + s];
+        tile[r][tx] = v;
+    }
+    __syncthreads();
+    for (int r = ty; r < DD_TILE; r += 8) {
+        const int s = s0 + r, t = t0 + tx;
+        if (s < nsub && t < 2 * worklen) S[(long long)s * 2 * worklen + t] = tile[tx][r];
+    }
+}
+
+// One DM per blockIdx.y; DD_NT * DD_XPT consecutive outputs per block.  An
+// offset outside [0, worklen] sets *err (the reference would read outside its
+// two blocks) and that DM's outputs are left unwritten.
+__global__ __launch_bounds__(DD_NT) void k_dd_sum(float *__restrict__ out, long long out_stride,
+                                                  const float *__restrict__ S, const int *__restrict__ offsets,
+                                                  int worklen, int nsub, int *__restrict__ err)
+{
+    const int dm = blockIdx.y;
+    const int x0 = blockIdx.x * DD_NT * DD_XPT + threadIdx.x;
+    const int *off = offsets + (long long)dm * nsub;
+    const long long w2 = 2LL * worklen;
+    float v[DD_XPT];
+#pragma unroll
+    for (int k = 0; k < DD_XPT; k++) v[k] = 0.0f;
+    bool bad = false;
+    for (int s = 0; s < nsub; s++) {
+        const int o = off[s];
+        bad |= (o < 0 || o > worklen);
+        const float *row = S + s * w2 + (bad ? 0 : o);
+#pragma unroll
+        for (int k = 0; k < DD_XPT; k++) {
+            const int x = x0 + k * DD_NT;
+            if (x < worklen) v[k] += row[x];
+        }
+    }
+    if (bad) {
+        if (threadIdx.x == 0) atomicOr(err, 1);
+        return;
+    }
+#pragma unroll
+    for (int k = 0; k < DD_XPT; k++) {
+        const int x = x0 + k * DD_NT;
+        if (x < worklen) out[(long long)dm * out_stride + x] = v[k];
+    }
+}
+
+struct DdWork {
+    std::mutex mu;
+    float *S = nullptr;
+    long long cap = 0;
+    int *err = nullptr;
+};
+std::mutex g_dd_mu;
+std::vector<DdWork *> g_dd_work;
+
+DdWork *dd_work(int device)
+{
+    std::lock_guard<std::mutex> lk(g_dd_mu);
+    if ((int)g_dd_work.size() <= device) g_dd_work.resize(device + 1, nullptr);
+    if (!g_dd_work[device]) g_dd_work[device] = new DdWork();
+    return g_dd_work[device];
+}
+
+}  // namespace
+
+using namespace pzl;
+
+extern "C" {
+
+int pzl_downsamp_data_array(float *currentdsdata_gpu, const float *currentdata_gpu, int worklen, int bs, int nsub)
+{
+    if (!currentdsdata_gpu || !currentdata_gpu) return fail(PZL_ERR_ARG, "downsamp: null data");
+    if (worklen < 1 || bs < 1 || nsub < 1) return fail(PZL_ERR_ARG, "downsamp: worklen, bs, nsub must be >= 1");
+    const long long total = (long long)worklen * nsub;
+    if (total * bs >= (1LL << 40)) return fail(PZL_ERR_ARG, "downsamp: block too large");
+    hipLaunchKernelGGL(k_dd_downsamp, dim3((unsigned)((total + DD_NT - 1) / DD_NT)), dim3(DD_NT), 0, nullptr,
+                       currentdsdata_gpu, currentdata_gpu, worklen, bs, nsub);
+    PZL_HIPCHK(hipGetLastError());
+    PZL_HIPCHK(hipStreamSynchronize(nullptr));
+    return 0;
+}
+
+int pzl_get_dedisp_strided(const float *currentdsdata_gpu, const float *lastdsdata_gpu, float *outdata_gpu,
+                           int worklen, int nsub, int numdms, const int *offsets_gpu, long long out_stride)
+{
+    if (!currentdsdata_gpu || !lastdsdata_gpu || !outdata_gpu || !offsets_gpu)
+        return fail(PZL_ERR_ARG, "dedisp: null data");
+    if (worklen < 1 || nsub < 1 || numdms < 1 || numdms > 65535 || out_stride < worklen)
+        return fail(PZL_ERR_ARG, "dedisp: worklen %d, nsub %d, numdms %d (<= 65535), row stride %lld (>= worklen)",
+                    worklen, nsub, numdms, out_stride);
+    if ((long long)nsub * 2 * worklen >= (1LL << 31)) return fail(PZL_ERR_ARG, "dedisp: block too large");
+    int device = 0;
+    PZL_HIPCHK(hipGetDevice(&device));
+    DdWork *w = dd_work(device);
+    std::lock_guard<std::mutex> lk(w->mu);
+    const long long need = (long long)nsub * 2 * worklen;
+    if (need > w->cap) {
+        (void)hipFree(w->S);
+        w->S = nullptr;
+        w->cap = 0;
+        PZL_HIPCHK(hipMalloc(&w->S, sizeof(float) * need));
+        w->cap = need;
+    }
+    if (!w->err) PZL_HIPCHK(hipMalloc(&w->err, sizeof(int)));
+    PZL_HIPCHK(hipMemsetAsync(w->err, 0, sizeof(int), nullptr));
+    hipLaunchKernelGGL(k_dd_transpose, dim3((2 * worklen + DD_TILE - 1) / DD_TILE, (nsub + DD_TILE - 1) / DD_TILE),
+                       dim3(DD_TILE * 8), 0, nullptr, w->S, lastdsdata_gpu, currentdsdata_gpu, worklen, nsub);
+    PZL_HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_dd_sum, dim3((worklen + DD_NT * DD_XPT - 1) / (DD_NT * DD_XPT), numdms), dim3(DD_NT), 0,
+                       nullptr, outdata_gpu, out_stride, w->S, offsets_gpu, worklen, nsub, w->err);
+    PZL_HIPCHK(hipGetLastError());
+    int err = 0;
+    PZL_HIPCHK(hipMemcpy(&err, w->err, sizeof(int), hipMemcpyDeviceToHost));
+    if (err) return fail(PZL_ERR_ARG, "dedisp: an offset lies outside [0, worklen = %d]", worklen);
+    return 0;
+}
+
+int pzl_get_dedisp(const float *currentdsdata_gpu, const float *lastdsdata_gpu, float *outdata_gpu, int worklen,
+                   int nsub, int numdms, const int *offsets_gpu)
+{
+    return pzl_get_dedisp_strided(currentdsdata_gpu, lastdsdata_gpu, outdata_gpu, worklen, nsub, numdms, offsets_gpu,
+                                  worklen);
+}
+
+int pzl_dedisperse(const float *subbands_gpu, long long nsamp, int nsub, int worklen, int numdms,
+                   const int *offsets_gpu, float *out_gpu)
+{
+    if (!subbands_gpu || !out_gpu || !offsets_gpu) return fail(PZL_ERR_ARG, "dedisperse: null data");
+    if (worklen < 1 || nsamp < worklen || nsamp % worklen)
+        return fail(PZL_ERR_ARG, "dedisperse: %lld samples are not whole blocks of %d", nsamp, worklen);
+    const long long nblk = nsamp / worklen, blk = (long long)worklen * nsub;
+    float *zero = nullptr;
+    PZL_HIPCHK(hipMalloc(&zero, sizeof(float) * blk));
+    hipError_t e = hipMemset(zero, 0, sizeof(float) * blk);
+    int rc = 0;
+    for (long long k = 0; k < nblk && rc == 0 && e == hipSuccess; k++) {
+        const float *last = subbands_gpu + k * blk;
+        const float *cur = k + 1 < nblk ? subbands_gpu + (k + 1) * blk : zero;
+        rc = pzl_get_dedisp_strided(cur, last, out_gpu + k * worklen, worklen, nsub, numdms, offsets_gpu, nsamp);
+    }
+    (void)hipFree(zero);
+    if (e != hipSuccess) return fail(PZL_ERR_HIP, "dedisperse: %s", hipGetErrorString(e));
+    return rc;
+}
+
+}  // extern "C"

@@ -182,6 +182,83 @@ def test_realfft_closed_forms():
         assert abs(a[1] - n) < tol and abs(a[0]) < tol and np.abs(a[2:]).max() < tol
 
 
+# ---------------- dedispersion (prepsubband_kernels.cu:26-77) ----------------
+
+def test_downsamp_closed_form():
+    """A ramp in[t][s] = t nsub + s averaged over bs samples: the mean of bs
+    consecutive t is (t' bs + (bs - 1) / 2) nsub + s (exact in float here); bs
+    = 1 is the identity."""
+    worklen, nsub = 50, 12
+    for bs in (1, 2, 4, 5):
+        t = np.arange(worklen * bs)[:, None]
+        x = (t * nsub + np.arange(nsub)[None, :]).astype(np.float32)
+        ds = O.downsamp(x, bs)
+        tp = np.arange(worklen)[:, None]
+        want = ((tp * bs + (bs - 1) / 2.0) * nsub + np.arange(nsub)[None, :]).astype(np.float32)
+        assert np.array_equal(ds, want), bs
+
+
+def _offsets(numdms, nsub, worklen, rng):
+    # delays grow towards low frequency (subband 0) and vanish at the top
+    # subband, as prepsubband_cu.c:456-466 builds them
+    dmfac = rng.uniform(0.0, 1.0, numdms)
+    f = np.linspace(0.6, 1.0, nsub)  # subband 0 lowest
+    d = (1.0 / f ** 2 - 1.0 / f[-1] ** 2)
+    return np.rint(dmfac[:, None] * d[None, :] / d.max() * (worklen - 1)).astype(np.int32)
+
+
+def test_dedisp_impulse_across_the_block_boundary():
+    """A pulse dispersed with the offsets of one DM (in the previous block for
+    some subbands, the current for others) sums to nsub at its own DM and
+    output time, and no other DM of different offsets reaches nsub there."""
+    rng = np.random.default_rng(5)
+    worklen, nsub, numdms = 400, 16, 9
+    offs = _offsets(numdms, nsub, worklen, rng)
+    dm, x0 = 4, 250
+    last = np.zeros((worklen, nsub), np.float32)
+    cur = np.zeros((worklen, nsub), np.float32)
+    for s in range(nsub):
+        t = x0 + offs[dm, s]
+        (last if t < worklen else cur)[t % worklen if t >= worklen else t, s] = 1.0
+    out = O.dedisp(cur, last, offs)
+    assert out[dm, x0] == nsub
+    assert out[dm].sum() == nsub  # every subband's impulse lands once in this DM's series
+    for d in range(numdms):
+        if not np.array_equal(offs[d], offs[dm]):
+            assert out[d, x0] < nsub
+
+
+def test_dedisp_constants_and_zero_offsets():
+    rng = np.random.default_rng(6)
+    worklen, nsub, numdms = 300, 10, 5
+    offs = _offsets(numdms, nsub, worklen, rng)
+    ones = np.ones((worklen, nsub), np.float32)
+    assert np.array_equal(O.dedisp(ones, ones, offs), np.full((numdms, worklen), nsub, np.float32))
+    last = rng.integers(-50, 50, (worklen, nsub)).astype(np.float32)
+    cur = rng.integers(-50, 50, (worklen, nsub)).astype(np.float32)
+    out = O.dedisp(cur, last, np.zeros((2, nsub), np.int32))
+    assert np.array_equal(out[0], last.sum(axis=1)) and np.array_equal(out[1], out[0])
+    out = O.dedisp(cur, last, np.full((1, nsub), worklen, np.int32))  # offset = worklen reads the current block
+    assert np.array_equal(out[0], cur.sum(axis=1))
+
+
+def test_dedisp_blocks_equal_the_whole_series():
+    """prepsubband_cu.c feeds consecutive blocks as (last, current): block k of
+    the output is sum_s series[s][k worklen + x + off_s] over the whole series
+    (integer-valued data: exact in any order)."""
+    rng = np.random.default_rng(7)
+    worklen, nsub, numdms, nblk = 128, 8, 6, 5
+    offs = _offsets(numdms, nsub, worklen, rng)
+    series = rng.integers(-20, 20, ((nblk + 1) * worklen, nsub)).astype(np.float32)
+    for k in range(nblk):
+        last, cur = series[k * worklen:(k + 1) * worklen], series[(k + 1) * worklen:(k + 2) * worklen]
+        out = O.dedisp(cur, last, offs)
+        for d in range(numdms):
+            idx = k * worklen + np.arange(worklen)[:, None] + offs[d][None, :]
+            want = series[idx, np.arange(nsub)[None, :]].sum(axis=1)
+            assert np.array_equal(out[d], want), (k, d)
+
+
 # ---------------- statistics -------------------------------------------------
 
 def test_median_is_lower_order_statistic():
