@@ -60,11 +60,37 @@ def _c3_small():
     return fft, kw, sigs
 
 
+def _c4_shapes(which):
+    """Two of the served stream's shapes: (zmax, wmax, numharm) = (50, 0, 8)
+    and (200, 150, 16) (wmax 150 rounds up to 160), N = 2^21, dt = 1e-4 s, one
+    injected jerked sinusoid."""
+    from prestozl_amd import synth
+    N, dt = 1 << 21, 1e-4
+    x = synth.timeseries(N, dt, [(0.05, 211.3, 30.0, -40.0)], seed=41)
+    fft = synth.realfft_packed(x)
+    zmax, wmax, nh = (50, 0, 8) if which == "c4a" else (200, 150, 16)
+    kw = dict(N=N, dt=dt, numbins=len(fft), zmax=zmax, wmax=wmax, numharm=nh, sigma=5.0, wmaxP=wmax > 0,
+              batchsize=4)
+    return fft, kw
+
+
+def _c5_trial():
+    """Config 5's DM trial 282 (N = 2^18, dt = 6.4e-4 s; zmax 200, wmax 300, 8 harmonics)."""
+    from prestozl_amd import synth
+    fft, N, dt = synth.dm_trial(282)
+    kw = dict(N=N, dt=dt, numbins=len(fft), zmax=200, wmax=300, numharm=8, sigma=5.0, wmaxP=True, batchsize=8)
+    return fft, kw
+
+
 def _shape(config1_data, which):
     if which == "c1":
         return _c1(config1_data)
     if which == "c2":
         return _c2_small()
+    if which in ("c4a", "c4b"):
+        return _c4_shapes(which)
+    if which == "c5":
+        return _c5_trial()
     fft, kw, _ = _c3_small()
     return fft, kw
 
@@ -132,7 +158,7 @@ def test_planes(config1_data, which):
     plan.close()
 
 
-@pytest.mark.parametrize("which", ["c1", "c2", "c3"])
+@pytest.mark.parametrize("which", ["c1", "c2", "c3", "c4a", "c4b", "c5"])
 def test_summed_powers(config1_data, which):
     """Final summed fundamental plane vs the FP32 oracle (tolerance TOL_POW)
     and the FP64 correlation-and-sum oracle (max and RMS error reported)."""
