@@ -258,6 +258,11 @@ int pzl_server_create(int device, long long hbm_budget, int max_inflight, void *
  * request completes.  Fails if the request alone exceeds the budget. */
 int pzl_server_submit(void *server, const PzlConfig *cfg, const float *host_fft, long long numbins,
                       long long *req_id);
+/* The same with the spectrum in device memory of the server's device (a DM
+ * trial from pzl_dedisperse + pzl_realfft_device_batch, say); it stays the
+ * caller's and must stay valid until the request is done. */
+int pzl_server_submit_device(void *server, const PzlConfig *cfg, const void *dev_fft, long long numbins,
+                             long long *req_id);
 /* Block until the request is done; copy its candidates and timing and forget it. */
 int pzl_server_wait(void *server, long long req_id, PzlCand *cands, long long maxcands, long long *ncands,
                     PzlRequestStats *stats);

@@ -141,6 +141,7 @@ def lib():
         L.pzl_autotune.argtypes = [C.c_void_p, C.c_int, C.c_char_p, C.c_int]
         L.pzl_server_create.argtypes = [C.c_int, C.c_longlong, C.c_int, P(C.c_void_p)]
         L.pzl_server_submit.argtypes = [C.c_void_p, P(Config), P(C.c_float), C.c_longlong, P(C.c_longlong)]
+        L.pzl_server_submit_device.argtypes = [C.c_void_p, P(Config), C.c_void_p, C.c_longlong, P(C.c_longlong)]
         L.pzl_server_wait.argtypes = [C.c_void_p, C.c_longlong, P(Cand), C.c_longlong, P(C.c_longlong),
                                       P(RequestStats)]
         L.pzl_server_poll.argtypes = [C.c_void_p, P(C.c_longlong), C.c_int, P(C.c_int)]
@@ -505,6 +506,13 @@ class Server:
         rid = C.c_longlong(0)
         check(lib().pzl_server_submit(self.h, C.byref(cfg), _fptr(a), len(a) // 2, C.byref(rid)))
         self._keep[rid.value] = a  # the spectrum stays valid until the request is done
+        return rid.value
+
+    def submit_device(self, cfg, ptr, numbins):
+        """A spectrum of numbins complex bins in device memory at 'ptr' (kept
+        valid by the caller until the request is done)."""
+        rid = C.c_longlong(0)
+        check(lib().pzl_server_submit_device(self.h, C.byref(cfg), C.c_void_p(ptr), int(numbins), C.byref(rid)))
         return rid.value
 
     def wait(self, rid, maxcands=1 << 16):

@@ -66,6 +66,7 @@ struct Request {
     long long id = 0;
     PzlConfig cfg{};
     const float *fft = nullptr;  // caller-owned until the request completes
+    bool on_device = false;      // fft is device memory of the server's device
     long long numbins = 0;
     long long footprint = 0;
     double cells = 0;
@@ -166,7 +167,9 @@ struct Server {
                     resident_bytes += p.bytes;
                 }
             }
-            if (rc == 0) rc = pzl_set_spectrum(plan, r->fft, r->numbins);
+            if (rc == 0)
+                rc = r->on_device ? pzl_set_spectrum_device(plan, r->fft, r->numbins)
+                                  : pzl_set_spectrum(plan, r->fft, r->numbins);
             PzlStats st{};
             long long n = 0;
             if (rc == 0) rc = pzl_search(plan, 0, -1, nullptr, 0, &n, &st);
@@ -233,16 +236,18 @@ int pzl_server_create(int device, long long hbm_budget, int max_inflight, void *
     return 0;
 }
 
-int pzl_server_submit(void *srv, const PzlConfig *cfg, const float *host_fft, long long numbins, long long *req_id)
+static int submit(void *srv, const PzlConfig *cfg, const float *fft, bool on_device, long long numbins,
+                  long long *req_id)
 {
     Server *s = (Server *)srv;
-    if (!s || !cfg || !host_fft || numbins <= 0) return fail(PZL_ERR_ARG, "bad request");
+    if (!s || !cfg || !fft || numbins <= 0) return fail(PZL_ERR_ARG, "bad request");
     PzlGeometry g;
     int rc = pzl_geometry_from_config(cfg, &g);
     if (rc) return rc;
     auto r = std::make_shared<Request>();
     r->cfg = *cfg;
-    r->fft = host_fft;
+    r->fft = fft;
+    r->on_device = on_device;
     r->numbins = numbins;
     // plan buffers, resident spectrum and kernel bank (counted per request,
     // although requests of one shape share a cached bank), and the runtime's
@@ -262,6 +267,17 @@ int pzl_server_submit(void *srv, const PzlConfig *cfg, const float *host_fft, lo
     s->cv.notify_all();
     if (req_id) *req_id = r->id;
     return 0;
+}
+
+int pzl_server_submit(void *srv, const PzlConfig *cfg, const float *host_fft, long long numbins, long long *req_id)
+{
+    return submit(srv, cfg, host_fft, false, numbins, req_id);
+}
+
+int pzl_server_submit_device(void *srv, const PzlConfig *cfg, const void *dev_fft, long long numbins,
+                             long long *req_id)
+{
+    return submit(srv, cfg, (const float *)dev_fft, true, numbins, req_id);
 }
 
 int pzl_server_wait(void *srv, long long req_id, PzlCand *cands, long long maxcands, long long *ncands,

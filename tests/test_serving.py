@@ -88,3 +88,28 @@ def test_request_above_budget_is_refused(config1_data):
     with pytest.raises(B.PzlError):
         srv.submit(cfg, fft)
     srv.close()
+
+
+def test_device_spectra_in_mix_equal_host(config1_data):
+    """Spectra already in device memory (submitted with
+    pzl_server_submit_device), served in a mix with host-memory requests of
+    the same shapes, give the same candidates as alone."""
+    from prestozl_amd import binding as B
+    reqs = _requests(config1_data)[:6]
+    alone = [_alone(cfg, fft) for cfg, fft in reqs]
+    devs = []
+    for cfg, fft in reqs:
+        a = np.ascontiguousarray(fft, dtype=np.complex64)
+        d = B.DeviceArray(a.nbytes)
+        d.upload(a)
+        devs.append(d)
+    srv = B.Server(budget=0, max_inflight=3)
+    ids = []
+    for k, ((cfg, fft), d) in enumerate(zip(reqs, devs)):
+        ids.append(srv.submit_device(cfg, d.ptr, len(fft)) if k % 2 == 0 else srv.submit(cfg, fft))
+    got = [srv.wait(i) for i in ids]
+    srv.close()
+    for k, ((cands, _), ref) in enumerate(zip(got, alone)):
+        assert cands == ref, k
+    for d in devs:
+        d.free()
