@@ -233,6 +233,44 @@ int tuned_variant(const Plan &p, const SizeClass &c)
     return it == g_tuned.end() ? 0 : it->second;
 }
 
+// Pinned hit-record buffers outlive their plans: pinning host memory is slow
+// (on the order of 0.15 s per GiB) and waits for the device, so a plan takes
+// buffers from this cache before pinning new memory, and a destroyed plan
+// leaves its buffers here, up to kPinCacheLimit bytes.
+std::mutex g_pin_mu;
+std::multimap<long long, PzlHostHit *> g_pin_cache;  // capacity in records -> buffer
+long long g_pin_cache_bytes = 0;
+constexpr long long kPinCacheLimit = 8LL << 30;
+
+// The smallest cached buffer of at least 'want' records and at most four times
+// that, or null; *cap gets its capacity.
+PzlHostHit *pin_take(long long want, long long *cap)
+{
+    std::lock_guard<std::mutex> lk(g_pin_mu);
+    auto it = g_pin_cache.lower_bound(want);
+    if (it == g_pin_cache.end() || it->first > 4 * want) return nullptr;
+    *cap = it->first;
+    PzlHostHit *ptr = it->second;
+    g_pin_cache_bytes -= (long long)sizeof(PzlHostHit) * it->first;
+    g_pin_cache.erase(it);
+    return ptr;
+}
+
+void pin_put(PzlHostHit *ptr, long long cap)
+{
+    if (!ptr) return;
+    {
+        std::lock_guard<std::mutex> lk(g_pin_mu);
+        const long long bytes = (long long)sizeof(PzlHostHit) * cap;
+        if (g_pin_cache_bytes + bytes <= kPinCacheLimit) {
+            g_pin_cache.emplace(cap, ptr);
+            g_pin_cache_bytes += bytes;
+            return;
+        }
+    }
+    (void)hipHostFree(ptr);
+}
+
 void free_plan(Plan *p)
 {
     if (!p) return;
@@ -256,7 +294,7 @@ void free_plan(Plan *p)
     }
     if (p->h_counts) (void)hipHostFree(p->h_counts);
     for (auto &hb : p->hpool) {
-        if (hb.ptr) (void)hipHostFree(hb.ptr);
+        pin_put(hb.ptr, hb.cap);
         if (hb.ev) (void)hipEventDestroy(hb.ev);
     }
     for (int k = 0; k < Plan::kSets; k++)
@@ -556,7 +594,7 @@ struct SearchState {
             }
             if (p->hpool_bytes + add > p->hpool_limit) {
                 for (auto &hb : p->hpool) {
-                    if (hb.ptr) (void)hipHostFree(hb.ptr);
+                    pin_put(hb.ptr, hb.cap);
                     hb.ptr = nullptr;
                     hb.cap = 0;
                 }
@@ -574,12 +612,15 @@ struct SearchState {
                 }
             }
             auto &hb = p->hpool[k];
-            if (hipHostMalloc((void **)&hb.ptr, sizeof(PzlHostHit) * want, hipHostMallocDefault) != hipSuccess) {
+            long long cap = want;
+            hb.ptr = pin_take(want, &cap);
+            if (!hb.ptr &&
+                hipHostMalloc((void **)&hb.ptr, sizeof(PzlHostHit) * want, hipHostMallocDefault) != hipSuccess) {
                 hb.ptr = nullptr;
                 return -1;
             }
-            hb.cap = want;
-            p->hpool_bytes += add;
+            hb.cap = cap;
+            p->hpool_bytes += (long long)sizeof(PzlHostHit) * cap;
             hb.busy = true;
             return k;
         }
