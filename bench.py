@@ -218,7 +218,7 @@ def realfft_bench(reps=20, cpu=True):
     return res
 
 
-def dedisp_bench(cpu=True, nsearch=48):
+def dedisp_bench(cpu=True, nsearch=48, serve=True):
     """The DM-trial producer on the GPU, device-resident end to end: 64
     subbands of N = 2^18 samples (dt = 6.4e-4 s, blocks of 16384 as
     prepsubband_cu reads them) with a sinusoid dispersed at trial 282 of 564,
@@ -267,6 +267,24 @@ def dedisp_bench(cpu=True, nsearch=48):
     ms_search = (time.perf_counter() - t0) * 1e3 / nsearch
     plan.close()
     # every trial through the request loop, straight from device memory
+    served = None
+    if serve:
+        served = serve_trials(B, cfg, dser, numdms, nsamp)
+    alg = 4.0 * nsamp * nsub + 4.0 * nsamp * numdms
+    r = {"workload": "64 subbands x N=2^18 (dt=6.4e-4 s) -> 564 DM trials (blocks of 16384), realfft of every trial, "
+                     "search of %d trials from device memory" % nsearch,
+         "ms_dedisperse": ms_dd, "dm_samples_per_s": numdms * nsamp / ms_dd * 1e3,
+         "alg_bytes": alg, "achieved_GBs": alg / (ms_dd * 1e-3) / 1e9, "frac_hbm": alg / (ms_dd * 1e-3) / 1e9 / HBM_PEAK_GBS,
+         "ms_realfft_batch": ms_fft, "ms_search_per_trial": ms_search,
+         "strongest": {"sigma": best[0], "trial": best[1], "freq_hz": best[2]}}
+    if served:
+        served["chain_s"] = (ms_dd + ms_fft) / 1e3 + served["wall_s"]
+        r["served_all_trials"] = served
+    return _dedisp_cpu(r, cpu, sub, offs, worklen, check, ds, do, dser)
+
+
+def serve_trials(B, cfg, dser, numdms, nsamp):
+    """Every DM trial through the request loop from device memory."""
     srv = B.Server(budget=0, max_inflight=3)
     t0 = time.perf_counter()
     ids = [srv.submit_device(cfg, dser.ptr + dm * nsamp * 4, nsamp // 2) for dm in range(numdms)]
@@ -275,19 +293,17 @@ def dedisp_bench(cpu=True, nsearch=48):
     srv.close()
     lat = [x[1]["ms_total"] for x in res]
     served_best = max(range(numdms), key=lambda i: max([c[1] for c in res[i][0]] or [0.0]))
-    alg = 4.0 * nsamp * nsub + 4.0 * nsamp * numdms
-    r = {"workload": "64 subbands x N=2^18 (dt=6.4e-4 s) -> 564 DM trials (blocks of 16384), realfft of every trial, "
-                     "search of %d trials from device memory" % nsearch,
-         "ms_dedisperse": ms_dd, "dm_samples_per_s": numdms * nsamp / ms_dd * 1e3,
-         "alg_bytes": alg, "achieved_GBs": alg / (ms_dd * 1e-3) / 1e9, "frac_hbm": alg / (ms_dd * 1e-3) / 1e9 / HBM_PEAK_GBS,
-         "ms_realfft_batch": ms_fft, "ms_search_per_trial": ms_search,
-         "strongest": {"sigma": best[0], "trial": best[1], "freq_hz": best[2]},
-         "served_all_trials": {"trials": numdms, "wall_s": wall, "trials_per_s": numdms / wall,
-                               "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
-                               "strongest_trial": served_best,
-                               "chain_s": (ms_dd + ms_fft) / 1e3 + wall,
-                               "note": "dedispersion + batched realfft + all 564 searches (3 in flight), every "
-                                       "spectrum read from device memory"}}
+    return {"trials": numdms, "wall_s": wall, "trials_per_s": numdms / wall,
+            "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+            "strongest_trial": served_best,
+            "note": "dedispersion + batched realfft + all 564 searches (3 in flight), every spectrum read from "
+                    "device memory"}
+
+
+def _dedisp_cpu(r, cpu, sub, offs, worklen, check, ds, do, dser):
+    """The oracle's dedispersion of one block (the CPU baseline) and a
+    bit-identity check of block 7 for two trials; frees the device arrays."""
+    numdms, nsamp = offs.shape[0], sub.shape[0]
     if cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import pyoracle as O
@@ -382,6 +398,8 @@ def main():
                     help="run only the realfft (.dat -> .fft) leg and print its line")
     ap.add_argument("--dedisp-only", action="store_true",
                     help="run only the dedispersion -> realfft -> search leg and print its line")
+    ap.add_argument("--dedisp-no-serve", action="store_true",
+                    help="with --dedisp-only: leave out the request-loop search of every trial")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
                     help="run a plain device copy of MIB MiB five times and exit (HBM counter calibration)")
     args = ap.parse_args()
@@ -403,7 +421,7 @@ def main():
         print(json.dumps(realfft_bench(cpu=not args.no_cpu)), flush=True)
         return
     if args.dedisp_only:
-        print(json.dumps(dedisp_bench(cpu=not args.no_cpu)), flush=True)
+        print(json.dumps(dedisp_bench(cpu=not args.no_cpu, serve=not args.dedisp_no_serve)), flush=True)
         return
     fft, N, dt, opts, sigs = synth.config2(N=1 << args.log2n)
     kw = dict(N=N, dt=dt, numbins=len(fft), zmax=opts["zmax"], wmax=opts["wmax"], numharm=opts["numharm"],

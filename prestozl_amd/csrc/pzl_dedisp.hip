@@ -7,7 +7,8 @@
 //   k_dd_transpose  the previous and the current block, [t][nsub] each, into
 //                   one subband-major window S[s][0 .. 2 worklen)
 //   k_dd_sum        out[dm][x] = sum_s S[s][x + offsets[dm][s]], subbands in
-//                   ascending order (the reference's float summation order)
+//                   ascending order (the reference's float summation order),
+//                   output tiles mapped XCD by XCD
 //
 // Both reference kernels read time-major data; the sums are formed in the same
 // order from the same values, so results are bit-identical.
@@ -21,7 +22,7 @@
 
 #define DD_NT 256
 #define DD_TILE 32
-#define DD_XPT 4  // outputs per thread in k_dd_sum
+#define DD_XPT 4  // consecutive outputs per thread in k_dd_sum (one 16-byte load)
 
 namespace {
 
@@ -59,15 +60,34 @@ __global__ __launch_bounds__(DD_TILE * 8) void k_dd_transpose(float *__restrict_
     }
 }
 
-// One DM per blockIdx.y; DD_NT * DD_XPT consecutive outputs per block.  An
-// offset outside [0, worklen] sets *err (the reference would read outside its
-// two blocks) and that DM's outputs are left unwritten.
+// out[dm][x] for DD_NT * DD_XPT consecutive x of one DM per workgroup.  The
+// block pair's subband-major window (nsub x 2 worklen floats) is larger than
+// one XCD's L2, and workgroups are dispatched to the 8 XCDs in turn: the
+// 1-D grid is mapped so that each XCD takes a contiguous range of output
+// tiles for every DM, and its part of the window stays in its L2.  Subbands
+// are summed in ascending order (the reference's order and values: the same
+// bits).  An offset outside [0, worklen] sets *err and that DM's outputs are
+// left unwritten.
 __global__ __launch_bounds__(DD_NT) void k_dd_sum(float *__restrict__ out, long long out_stride,
                                                   const float *__restrict__ S, const int *__restrict__ offsets,
-                                                  int worklen, int nsub, int *__restrict__ err)
+                                                  int worklen, int nsub, int ntiles, int numdms,
+                                                  int *__restrict__ err)
 {
-    const int dm = blockIdx.y;
-    const int x0 = blockIdx.x * DD_NT * DD_XPT + threadIdx.x;
+    const int w = blockIdx.x, nb = gridDim.x;
+    int tile, dm;
+    if (ntiles % 8 == 0 && nb % 8 == 0) {
+        const int per = ntiles / 8, j = w / 8;  // the j-th workgroup of XCD w % 8
+        tile = (w % 8) * per + j % per;
+        dm = j / per;
+    } else {
+        tile = w % ntiles;
+        dm = w / ntiles;
+    }
+    if (dm >= numdms) return;
+    // DD_XPT consecutive outputs per thread, read with 16-byte loads per
+    // subband (4-byte aligned: the offsets are arbitrary)
+    const int x0 = tile * DD_NT * DD_XPT + threadIdx.x * DD_XPT;
+    const bool full = x0 + DD_XPT <= worklen;
     const int *off = offsets + (long long)dm * nsub;
     const long long w2 = 2LL * worklen;
     float v[DD_XPT];
@@ -77,11 +97,21 @@ __global__ __launch_bounds__(DD_NT) void k_dd_sum(float *__restrict__ out, long 
     for (int s = 0; s < nsub; s++) {
         const int o = off[s];
         bad |= (o < 0 || o > worklen);
-        const float *row = S + s * w2 + (bad ? 0 : o);
+        const float *row = S + s * w2 + (bad ? 0 : o) + x0;
+        if (full) {
 #pragma unroll
-        for (int k = 0; k < DD_XPT; k++) {
-            const int x = x0 + k * DD_NT;
-            if (x < worklen) v[k] += row[x];
+            for (int k = 0; k < DD_XPT; k += 4) {
+                float4 q;
+                __builtin_memcpy(&q, row + k, sizeof(q));
+                v[k] += q.x;
+                v[k + 1] += q.y;
+                v[k + 2] += q.z;
+                v[k + 3] += q.w;
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < DD_XPT; k++)
+                if (x0 + k < worklen) v[k] += row[k];
         }
     }
     if (bad) {
@@ -89,10 +119,8 @@ __global__ __launch_bounds__(DD_NT) void k_dd_sum(float *__restrict__ out, long 
         return;
     }
 #pragma unroll
-    for (int k = 0; k < DD_XPT; k++) {
-        const int x = x0 + k * DD_NT;
-        if (x < worklen) out[(long long)dm * out_stride + x] = v[k];
-    }
+    for (int k = 0; k < DD_XPT; k++)
+        if (x0 + k < worklen) out[(long long)dm * out_stride + x0 + k] = v[k];
 }
 
 struct DdWork {
@@ -100,6 +128,8 @@ struct DdWork {
     float *S = nullptr;
     long long cap = 0;
     int *err = nullptr;
+    float *zero = nullptr;  // a block of zeros after the last one of a series
+    long long zcap = 0;
 };
 std::mutex g_dd_mu;
 std::vector<DdWork *> g_dd_work;
@@ -131,19 +161,29 @@ int pzl_downsamp_data_array(float *currentdsdata_gpu, const float *currentdata_g
     return 0;
 }
 
-int pzl_get_dedisp_strided(const float *currentdsdata_gpu, const float *lastdsdata_gpu, float *outdata_gpu,
-                           int worklen, int nsub, int numdms, const int *offsets_gpu, long long out_stride)
+// Enqueue the transpose and the sums of one block pair on the null stream
+// (the workspace and the error flag are the caller's, under w->mu).
+static int dedisp_enqueue(DdWork *w, const float *cur, const float *last, float *out, int worklen, int nsub,
+                          int numdms, const int *offsets, long long out_stride)
 {
-    if (!currentdsdata_gpu || !lastdsdata_gpu || !outdata_gpu || !offsets_gpu)
-        return fail(PZL_ERR_ARG, "dedisp: null data");
-    if (worklen < 1 || nsub < 1 || numdms < 1 || numdms > 65535 || out_stride < worklen)
-        return fail(PZL_ERR_ARG, "dedisp: worklen %d, nsub %d, numdms %d (<= 65535), row stride %lld (>= worklen)",
-                    worklen, nsub, numdms, out_stride);
+    hipLaunchKernelGGL(k_dd_transpose, dim3((2 * worklen + DD_TILE - 1) / DD_TILE, (nsub + DD_TILE - 1) / DD_TILE),
+                       dim3(DD_TILE * 8), 0, nullptr, w->S, last, cur, worklen, nsub);
+    PZL_HIPCHK(hipGetLastError());
+    const int ntiles = (worklen + DD_NT * DD_XPT - 1) / (DD_NT * DD_XPT);
+    hipLaunchKernelGGL(k_dd_sum, dim3((unsigned)ntiles * numdms), dim3(DD_NT), 0, nullptr, out, out_stride, w->S,
+                       offsets, worklen, nsub, ntiles, numdms, w->err);
+    PZL_HIPCHK(hipGetLastError());
+    return 0;
+}
+
+// Argument checks and the workspace for blocks of (worklen, nsub).
+static int dedisp_prepare(DdWork *w, int worklen, int nsub, int numdms, long long out_stride)
+{
+    if (worklen < 1 || nsub < 1 || numdms < 1 || (long long)numdms * ((worklen + 1023) / 1024) >= (1LL << 31) ||
+        out_stride < worklen)
+        return fail(PZL_ERR_ARG, "dedisp: worklen %d, nsub %d, numdms %d, row stride %lld (>= worklen)", worklen, nsub,
+                    numdms, out_stride);
     if ((long long)nsub * 2 * worklen >= (1LL << 31)) return fail(PZL_ERR_ARG, "dedisp: block too large");
-    int device = 0;
-    PZL_HIPCHK(hipGetDevice(&device));
-    DdWork *w = dd_work(device);
-    std::lock_guard<std::mutex> lk(w->mu);
     const long long need = (long long)nsub * 2 * worklen;
     if (need > w->cap) {
         (void)hipFree(w->S);
@@ -154,16 +194,31 @@ int pzl_get_dedisp_strided(const float *currentdsdata_gpu, const float *lastdsda
     }
     if (!w->err) PZL_HIPCHK(hipMalloc(&w->err, sizeof(int)));
     PZL_HIPCHK(hipMemsetAsync(w->err, 0, sizeof(int), nullptr));
-    hipLaunchKernelGGL(k_dd_transpose, dim3((2 * worklen + DD_TILE - 1) / DD_TILE, (nsub + DD_TILE - 1) / DD_TILE),
-                       dim3(DD_TILE * 8), 0, nullptr, w->S, lastdsdata_gpu, currentdsdata_gpu, worklen, nsub);
-    PZL_HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_dd_sum, dim3((worklen + DD_NT * DD_XPT - 1) / (DD_NT * DD_XPT), numdms), dim3(DD_NT), 0,
-                       nullptr, outdata_gpu, out_stride, w->S, offsets_gpu, worklen, nsub, w->err);
-    PZL_HIPCHK(hipGetLastError());
+    return 0;
+}
+
+static int dedisp_finish(DdWork *w, int worklen)
+{
     int err = 0;
     PZL_HIPCHK(hipMemcpy(&err, w->err, sizeof(int), hipMemcpyDeviceToHost));
     if (err) return fail(PZL_ERR_ARG, "dedisp: an offset lies outside [0, worklen = %d]", worklen);
     return 0;
+}
+
+int pzl_get_dedisp_strided(const float *currentdsdata_gpu, const float *lastdsdata_gpu, float *outdata_gpu,
+                           int worklen, int nsub, int numdms, const int *offsets_gpu, long long out_stride)
+{
+    if (!currentdsdata_gpu || !lastdsdata_gpu || !outdata_gpu || !offsets_gpu)
+        return fail(PZL_ERR_ARG, "dedisp: null data");
+    int device = 0;
+    PZL_HIPCHK(hipGetDevice(&device));
+    DdWork *w = dd_work(device);
+    std::lock_guard<std::mutex> lk(w->mu);
+    int rc = dedisp_prepare(w, worklen, nsub, numdms, out_stride);
+    if (!rc)
+        rc = dedisp_enqueue(w, currentdsdata_gpu, lastdsdata_gpu, outdata_gpu, worklen, nsub, numdms, offsets_gpu,
+                            out_stride);
+    return rc ? rc : dedisp_finish(w, worklen);
 }
 
 int pzl_get_dedisp(const float *currentdsdata_gpu, const float *lastdsdata_gpu, float *outdata_gpu, int worklen,
@@ -179,19 +234,29 @@ int pzl_dedisperse(const float *subbands_gpu, long long nsamp, int nsub, int wor
     if (!subbands_gpu || !out_gpu || !offsets_gpu) return fail(PZL_ERR_ARG, "dedisperse: null data");
     if (worklen < 1 || nsamp < worklen || nsamp % worklen)
         return fail(PZL_ERR_ARG, "dedisperse: %lld samples are not whole blocks of %d", nsamp, worklen);
+    int device = 0;
+    PZL_HIPCHK(hipGetDevice(&device));
+    DdWork *w = dd_work(device);
+    std::lock_guard<std::mutex> lk(w->mu);
+    int rc = dedisp_prepare(w, worklen, nsub, numdms, nsamp);
+    if (rc) return rc;
     const long long nblk = nsamp / worklen, blk = (long long)worklen * nsub;
-    float *zero = nullptr;
-    PZL_HIPCHK(hipMalloc(&zero, sizeof(float) * blk));
-    hipError_t e = hipMemset(zero, 0, sizeof(float) * blk);
-    int rc = 0;
-    for (long long k = 0; k < nblk && rc == 0 && e == hipSuccess; k++) {
-        const float *last = subbands_gpu + k * blk;
-        const float *cur = k + 1 < nblk ? subbands_gpu + (k + 1) * blk : zero;
-        rc = pzl_get_dedisp_strided(cur, last, out_gpu + k * worklen, worklen, nsub, numdms, offsets_gpu, nsamp);
+    if (blk > w->zcap) {
+        (void)hipFree(w->zero);
+        w->zero = nullptr;
+        w->zcap = 0;
+        PZL_HIPCHK(hipMalloc(&w->zero, sizeof(float) * blk));
+        PZL_HIPCHK(hipMemset(w->zero, 0, sizeof(float) * blk));
+        w->zcap = blk;
     }
-    (void)hipFree(zero);
-    if (e != hipSuccess) return fail(PZL_ERR_HIP, "dedisperse: %s", hipGetErrorString(e));
-    return rc;
+    // every block pair enqueued in order (the window workspace is reused in
+    // stream order), one wait and one check at the end
+    for (long long k = 0; k < nblk && rc == 0; k++) {
+        const float *last = subbands_gpu + k * blk;
+        const float *cur = k + 1 < nblk ? subbands_gpu + (k + 1) * blk : w->zero;
+        rc = dedisp_enqueue(w, cur, last, out_gpu + k * worklen, worklen, nsub, numdms, offsets_gpu, nsamp);
+    }
+    return rc ? rc : dedisp_finish(w, worklen);
 }
 
 }  // extern "C"
