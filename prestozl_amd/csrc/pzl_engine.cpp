@@ -464,6 +464,22 @@ int launch_sum_part(Plan &p, long long r0, int nb, int set, int slot, float *d_s
     return 0;
 }
 
+int sync_all(Plan &p);
+
+// Drop the captured correlation graphs (they hold the variants chosen at
+// capture time): the next full batch of each set captures anew.
+int drop_graphs(Plan &p)
+{
+    int rc = sync_all(p);
+    if (rc) return rc;
+    for (int k = 0; k < Plan::kSets; k++)
+        if (p.gexec[k]) {
+            (void)hipGraphExecDestroy(p.gexec[k]);
+            p.gexec[k] = nullptr;
+        }
+    return 0;
+}
+
 // The correlation part of a full batch as one graph launch: the graph of a
 // buffer set (segments and every correlation class) is captured once, with the
 // batch's first rstep read from device memory, and replayed for every later
@@ -1242,6 +1258,10 @@ int pzl_autotune(void *plan, int reps, char *report, int report_len)
     if (!p->d_fft) return fail(PZL_ERR_NOSPECTRUM, "no spectrum set");
     PZL_HIPCHK(hipSetDevice(p->device));
     if (reps < 1) reps = 1;
+    {
+        int grc = drop_graphs(*p);  // the variants may change below
+        if (grc) return grc;
+    }
     const long long nr = p->g.nrsteps;
     const long long r0 = std::max<long long>(0, std::min<long long>(nr - p->B, nr / 2));
     const int nb = (int)std::min<long long>(p->B, nr - r0);
@@ -1258,6 +1278,7 @@ int pzl_autotune(void *plan, int reps, char *report, int report_len)
     PZL_HIPCHK(hipEventCreate(&e1));
     std::string rep;
     char line[256];
+    std::vector<std::pair<float, int>> lead_times;  // (ms, variant) of the longest FFT length
     for (auto &c : p->classes) {
         const PzlCorrArgs ca = corr_args(*p, c, nb, 0);
         const int nv = pzl_corr_num_variants(c.fftlen);
@@ -1283,6 +1304,7 @@ int pzl_autotune(void *plan, int reps, char *report, int report_len)
             snprintf(line, sizeof line, "fftlen %d variant %d (%s): %.4f ms\n", c.fftlen, v,
                      pzl_corr_variant_name(c.fftlen, v), ms);
             rep += line;
+            if (&c == &p->classes[0]) lead_times.emplace_back(ms, v);
             if (ms < best_ms) {
                 best_ms = ms;
                 best = v;
@@ -1329,6 +1351,41 @@ int pzl_autotune(void *plan, int reps, char *report, int report_len)
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    // In the search the longest length's correlation shares the CUs with the
+    // harmonic sum and the shorter lengths: the variants of it within 8% of
+    // the fastest alone are timed on a band of the real pipeline (best of two
+    // searches of 8 batches), and the fastest there is kept.
+    std::sort(lead_times.begin(), lead_times.end());
+    const long long band = 8LL * p->B;
+    if (lead_times.size() >= 2 && nr >= band) {
+        SizeClass &c = p->classes[0];
+        const long long lo = (nr - band) / 2;
+        float best_ms = 1e30f;
+        int best = c.variant;
+        for (size_t i = 0; i < lead_times.size() && i < 5; i++) {
+            if (lead_times[i].first > 1.08f * lead_times[0].first) break;
+            c.variant = lead_times[i].second;
+            float ms = 1e30f;
+            for (int k = 0; k < 2; k++) {
+                long long n = 0;
+                const auto t0 = std::chrono::steady_clock::now();
+                rc = pzl_search(plan, lo, lo + band, nullptr, 0, &n, nullptr);
+                if (rc) return rc;
+                ms = std::min(ms, (float)std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() -
+                                                                                  t0).count());
+            }
+            snprintf(line, sizeof line, "fftlen %d variant %d in a search band of %lld rsteps: %.3f ms\n", c.fftlen,
+                     c.variant, band, ms);
+            rep += line;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = c.variant;
+            }
+        }
+        c.variant = best;
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tuned[tune_key(*p, c)] = best;
+    }
     if (report && report_len > 0) {
         snprintf(report, (size_t)report_len, "%s", rep.c_str());
     }
@@ -1432,6 +1489,10 @@ int pzl_plan_set_variant(void *plan, int fftlen, int variant)
         return fail(PZL_ERR_ARG, "FFT length %d has no variant %d", fftlen, variant);
     for (auto &c : p->classes)
         if (c.fftlen == fftlen) {
+            if (c.variant != variant) {
+                int rc = drop_graphs(*p);
+                if (rc) return rc;
+            }
             c.variant = variant;
             return 0;
         }
