@@ -138,6 +138,68 @@ __device__ inline long long rf_group(int b, int nb)
     return (nb % 8) ? b : (long long)(b % 8) * (nb / 8) + b / 8;
 }
 
+// The same pass with L and Ns known at compile time (powers of two): the
+// butterfly indices are shifts and masks.  The sequence count stays a
+// run-time value.
+template <int SIGN, int R, int L, int Ns>
+__device__ __forceinline__ void rf_pass_ct(float2 *buf, int nseq, int sstride, const float2 *__restrict__ tw, int tid)
+{
+    constexpr int MAXB = (RF_MAXE / R + RF_NT - 1) / RF_NT;
+    constexpr int nbf = L / R, step = L / (Ns * R);
+    const int total = nbf * nseq;
+    float2 v[MAXB][R];
+#pragma unroll
+    for (int b = 0; b < MAXB; b++) {
+        const int g = tid + b * RF_NT;
+        if (g < total) {
+            const int s = g / nbf, j = g % nbf, jm = j % Ns;
+            const float2 *q = buf + s * sstride;
+            v[b][0] = q[j];
+#pragma unroll
+            for (int r = 1; r < R; r++) {
+                float2 t = tw[jm * r * step];
+                if (SIGN > 0) t.y = -t.y;
+                v[b][r] = cmul(q[j + r * nbf], t);
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < MAXB; b++) {
+        const int g = tid + b * RF_NT;
+        if (g < total) {
+            const int s = g / nbf, j = g % nbf, jm = j % Ns, jq = j / Ns;
+            dftR<R, SIGN>(v[b]);
+            float2 *q = buf + s * sstride;
+            const int o = jq * Ns * R + jm;
+#pragma unroll
+            for (int r = 0; r < R; r++) q[o + r * Ns] = v[b][r];
+        }
+    }
+    __syncthreads();
+}
+
+template <int SIGN, int L, int Ns = 1>
+__device__ __forceinline__ void rf_fft_ct(float2 *buf, int nseq, int sstride, const float2 *__restrict__ tw, int tid)
+{
+    if constexpr (Ns < L) {
+        constexpr int rem = L / Ns;
+        constexpr int R = rem % 16 == 0 ? 16 : rem % 8 == 0 ? 8 : rem % 4 == 0 ? 4 : 2;
+        rf_pass_ct<SIGN, R, L, Ns>(buf, nseq, sstride, tw, tid);
+        rf_fft_ct<SIGN, L, Ns * R>(buf, nseq, sstride, tw, tid);
+    }
+}
+
+// FFT length L of the four-step kernels: 0 for the run-time pass loop, else a
+// power of two compiled into the pass sequence.
+template <int SIGN, int L>
+__device__ __forceinline__ void rf_fft_any(float2 *buf, int n, int nseq, int sstride, const float2 *__restrict__ tw,
+                                           int tid)
+{
+    if constexpr (L == 0) rf_fft<SIGN>(buf, n, nseq, sstride, tw, tid);
+    else rf_fft_ct<SIGN, L>(buf, nseq, sstride, tw, tid);
+}
+
 __device__ inline float2 rf_scale(float2 v, double scale)
 {
     return float2{(float)((double)v.x * scale), (float)((double)v.y * scale)};
@@ -190,7 +252,7 @@ __global__ __launch_bounds__(RF_NT) void k_rf_single(float2 *__restrict__ data, 
 
 // Columns j2b .. j2b + C - 1 of the [M1][M2] view of 'in': FFT of length M1
 // down each, times exp(SIGN 2 pi i k1 j2 / M), into 'out' at the same places.
-template <int SIGN>
+template <int SIGN, int L>
 __global__ __launch_bounds__(RF_NT) void k_rf_cols(const float2 *__restrict__ in, float2 *__restrict__ out, int M1,
                                                    int M2, int C, const float2 *__restrict__ tw,
                                                    const double2 *__restrict__ tlo, const double2 *__restrict__ thi)
@@ -224,7 +286,7 @@ __global__ __launch_bounds__(RF_NT) void k_rf_cols(const float2 *__restrict__ in
         }
     }
     __syncthreads();
-    rf_fft<SIGN>(rbuf, M1, C, sstride, tw, tid);
+    rf_fft_any<SIGN, L>(rbuf, M1, C, sstride, tw, tid);
 #pragma unroll 4
     for (int e = tid; e < total; e += RF_NT) {
         const int k1 = dc.div(e), c = e - k1 * C;
@@ -237,7 +299,7 @@ __global__ __launch_bounds__(RF_NT) void k_rf_cols(const float2 *__restrict__ in
 
 // Rows k1b .. k1b + C2 - 1 of the [M1][M2] view of 'in': FFT of length M2
 // along each, element k2 of row k1 to out[k1 + M1 k2], times 'scale'.
-template <int SIGN>
+template <int SIGN, int L>
 __global__ __launch_bounds__(RF_NT) void k_rf_rows(const float2 *__restrict__ in, float2 *__restrict__ out, int M1,
                                                    int M2, int C2, const float2 *__restrict__ tw, double scale)
 {
@@ -268,7 +330,7 @@ __global__ __launch_bounds__(RF_NT) void k_rf_rows(const float2 *__restrict__ in
         }
     }
     __syncthreads();
-    rf_fft<SIGN>(rbuf, M2, C2, sstride, tw, tid);
+    rf_fft_any<SIGN, L>(rbuf, M2, C2, sstride, tw, tid);
 #pragma unroll 4
     for (int e = tid; e < total; e += RF_NT) {
         const int k2 = dc2.div(e), c = e - k2 * C2;
@@ -398,6 +460,38 @@ RfWork *work_for(int device)
 
 using namespace pzl;
 
+// The column / row kernels for FFT length L: the compiled pass sequence for the
+// powers of two 256 .. 8192, the run-time pass loop otherwise.
+template <int SIGN, typename... A>
+static hipError_t launch_cols(int L, dim3 grid, size_t lds, hipStream_t st, A... a)
+{
+    switch (L) {
+    case 256: hipLaunchKernelGGL((k_rf_cols<SIGN, 256>), grid, dim3(RF_NT), lds, st, a...); break;
+    case 512: hipLaunchKernelGGL((k_rf_cols<SIGN, 512>), grid, dim3(RF_NT), lds, st, a...); break;
+    case 1024: hipLaunchKernelGGL((k_rf_cols<SIGN, 1024>), grid, dim3(RF_NT), lds, st, a...); break;
+    case 2048: hipLaunchKernelGGL((k_rf_cols<SIGN, 2048>), grid, dim3(RF_NT), lds, st, a...); break;
+    case 4096: hipLaunchKernelGGL((k_rf_cols<SIGN, 4096>), grid, dim3(RF_NT), lds, st, a...); break;
+    case 8192: hipLaunchKernelGGL((k_rf_cols<SIGN, 8192>), grid, dim3(RF_NT), lds, st, a...); break;
+    default: hipLaunchKernelGGL((k_rf_cols<SIGN, 0>), grid, dim3(RF_NT), lds, st, a...); break;
+    }
+    return hipGetLastError();
+}
+
+template <int SIGN, typename... A>
+static hipError_t launch_rows(int L, dim3 grid, size_t lds, hipStream_t st, A... a)
+{
+    switch (L) {
+    case 256: hipLaunchKernelGGL((k_rf_rows<SIGN, 256>), grid, dim3(RF_NT), lds, st, a...); break;
+    case 512: hipLaunchKernelGGL((k_rf_rows<SIGN, 512>), grid, dim3(RF_NT), lds, st, a...); break;
+    case 1024: hipLaunchKernelGGL((k_rf_rows<SIGN, 1024>), grid, dim3(RF_NT), lds, st, a...); break;
+    case 2048: hipLaunchKernelGGL((k_rf_rows<SIGN, 2048>), grid, dim3(RF_NT), lds, st, a...); break;
+    case 4096: hipLaunchKernelGGL((k_rf_rows<SIGN, 4096>), grid, dim3(RF_NT), lds, st, a...); break;
+    case 8192: hipLaunchKernelGGL((k_rf_rows<SIGN, 8192>), grid, dim3(RF_NT), lds, st, a...); break;
+    default: hipLaunchKernelGGL((k_rf_rows<SIGN, 0>), grid, dim3(RF_NT), lds, st, a...); break;
+    }
+    return hipGetLastError();
+}
+
 // The transforms of nb series of n floats each, back to back at d (device
 // memory of the current device).
 static int realfft_on_device(float *d_data, long long n, int nb, int isign, hipStream_t st, RfWork *w)
@@ -448,15 +542,11 @@ static int realfft_on_device(float *d_data, long long n, int nb, int isign, hipS
         const int C = per_block(M2, M1), C2 = per_block(M1, M2);
         const size_t lds1 = sizeof(float2) * (size_t)C * (M1 + 1), lds2 = sizeof(float2) * (size_t)C2 * (M2 + 1);
         if (isign == -1) {
-            hipLaunchKernelGGL(k_rf_cols<-1>, dim3(M2 / C, nb), dim3(RF_NT), lds1, st, d, ws, M1, M2, C, tw1, tlo,
-                               thi);
-            hipLaunchKernelGGL(k_rf_rows<-1>, dim3(M1 / C2, nb), dim3(RF_NT), lds2, st, ws, d, M1, M2, C2, tw2,
-                               scale);
+            PZL_HIPCHK(launch_cols<-1>(M1, dim3(M2 / C, nb), lds1, st, d, ws, M1, M2, C, tw1, tlo, thi));
+            PZL_HIPCHK(launch_rows<-1>(M2, dim3(M1 / C2, nb), lds2, st, ws, d, M1, M2, C2, tw2, scale));
         } else {
-            hipLaunchKernelGGL(k_rf_cols<1>, dim3(M2 / C, nb), dim3(RF_NT), lds1, st, d, ws, M1, M2, C, tw1, tlo,
-                               thi);
-            hipLaunchKernelGGL(k_rf_rows<1>, dim3(M1 / C2, nb), dim3(RF_NT), lds2, st, ws, d, M1, M2, C2, tw2,
-                               scale);
+            PZL_HIPCHK(launch_cols<1>(M1, dim3(M2 / C, nb), lds1, st, d, ws, M1, M2, C, tw1, tlo, thi));
+            PZL_HIPCHK(launch_rows<1>(M2, dim3(M1 / C2, nb), lds2, st, ws, d, M1, M2, C2, tw2, scale));
         }
     }
     PZL_HIPCHK(hipGetLastError());
