@@ -1315,6 +1315,7 @@ int pzl_autotune(void *plan, int reps, char *report, int report_len)
         g_tuned[tune_key(*p, c)] = best;
     }
     // harmonic-sum workgroup order, on the planes the correlations just wrote
+    std::vector<std::pair<float, int>> sum_times;  // (ms, form = order | rpt log2 << 1)
     {
         PZL_HIPCHK(hipMemsetAsync(p->d_counts, 0, 2 * sizeof(unsigned long long), p->st));
         float best_ms = 1e30f;
@@ -1337,6 +1338,7 @@ int pzl_autotune(void *plan, int reps, char *report, int report_len)
             ms /= reps;
             snprintf(line, sizeof line, "harmonic sum order %d, %d r per thread: %.4f ms\n", order, rpt, ms);
             rep += line;
+            sum_times.emplace_back(ms, form);
             if (ms < best_ms) {
                 best_ms = ms;
                 best = order;
@@ -1385,6 +1387,37 @@ int pzl_autotune(void *plan, int reps, char *report, int report_len)
         c.variant = best;
         std::lock_guard<std::mutex> lk(g_tune_mu);
         g_tuned[tune_key(*p, c)] = best;
+    }
+    // likewise the harmonic-sum forms within 8% of the fastest alone
+    std::sort(sum_times.begin(), sum_times.end());
+    if (sum_times.size() >= 2 && nr >= band && sum_times[1].first <= 1.08f * sum_times[0].first) {
+        const long long lo = (nr - band) / 2;
+        float best_ms = 1e30f;
+        int best_order = p->sum_order, best_rpt = p->sum_rpt;
+        for (size_t i = 0; i < sum_times.size() && i < 3; i++) {
+            if (sum_times[i].first > 1.08f * sum_times[0].first) break;
+            p->sum_order = sum_times[i].second & 1;
+            p->sum_rpt = 1 << (sum_times[i].second >> 1);
+            float ms = 1e30f;
+            for (int k = 0; k < 2; k++) {
+                long long n = 0;
+                const auto t0 = std::chrono::steady_clock::now();
+                rc = pzl_search(plan, lo, lo + band, nullptr, 0, &n, nullptr);
+                if (rc) return rc;
+                ms = std::min(ms, (float)std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() -
+                                                                                  t0).count());
+            }
+            snprintf(line, sizeof line, "harmonic sum order %d, %d r per thread in a search band of %lld rsteps: %.3f ms\n",
+                     p->sum_order, p->sum_rpt, band, ms);
+            rep += line;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best_order = p->sum_order;
+                best_rpt = p->sum_rpt;
+            }
+        }
+        p->sum_order = best_order;
+        p->sum_rpt = best_rpt;
     }
     if (report && report_len > 0) {
         snprintf(report, (size_t)report_len, "%s", rep.c_str());
