@@ -220,6 +220,42 @@ def test_search_config1(config1_data):
     plan.close()
 
 
+@pytest.mark.parametrize("option", ["nph", "flo_fhi", "numharm1", "lobin"])
+def test_search_options(config1_data, option):
+    """Search options the drop-in takes (accelsearch_cmd.h; create_accelobs,
+    accel_utils.c:1636-2037), each over config 1 against the oracle: photon
+    normalisation (-photon: powers / nph instead of the block median), an
+    -flo/-fhi band, a single harmonic, and a spectrum file that starts at bin
+    4000 (the .fft's lobin; the injected signal sits near bin 8286)."""
+    fft, N, dt, opts = config1_data
+    kw = dict(N=N, dt=dt, numbins=len(fft), zmax=opts["zmax"], wmax=opts["wmax"], numharm=opts["numharm"],
+              sigma=opts["sigma"], wmaxP=True, batchsize=8)
+    data = fft
+    if option == "nph":
+        kw["nph"] = 1.3
+    elif option == "flo_fhi":
+        kw.update(flo=50.0, fhi=300.0)
+    elif option == "numharm1":
+        kw.update(numharm=1, sigma=3.0)
+    else:
+        data = fft[4000:]
+        kw.update(numbins=len(data), lobin=4000)
+    plan = _plan(kw, data)
+    obs, ctx32 = _oracle(kw, O.FP32)
+    cg, _ = plan.search()
+    hg, pg = plan.hits()
+    co, ho, _ = ctx32.search(data)
+    near, worst, common = _compare_hits(plan, hg, pg, ho, list(obs.powcut))
+    print("%s: %d gpu hits, %d oracle hits, %d near-threshold diffs, hit power err %.3g; %d/%d cands"
+          % (option, len(hg), len(ho), near, worst, len(cg), len(co)))
+    assert worst < TOL_POW
+    assert common > 0
+    key = lambda c: (round(c[3], 6), round(c[4], 6), round(c[5], 6), c[2])
+    if near == 0:
+        assert [key(c) for c in cg] == [key(c) for c in co]
+    plan.close()
+
+
 def test_search_config2_band():
     """Config-2 geometry (zmax 200, wmax 300, 16 harmonics) on a band of rsteps."""
     fft, kw = _c2_small()
