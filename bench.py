@@ -63,6 +63,20 @@ def cpu_optimize_sample(fft, cands, ncand=8):
                       "(mean %.1f harmonics)" % (len(sample), np.mean([c[3] for c in sample]) if sample else 0)}
 
 
+def cpu_maximize_sample(fft, points, npts=64):
+    """The oracle's max_rzw_arr (one harmonic, one core) on the first 'npts'
+    harmonic points of the -noharmpolish optimisation."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+    sample = points[:npts]
+    t0 = time.perf_counter()
+    for r, z, w in sample:
+        O.max_rzw_arr(fft, r, z, w)
+    dt = time.perf_counter() - t0
+    return {"ms_per_point": dt * 1e3 / max(len(sample), 1), "cores": 1, "kind": "port",
+            "sample": "oracle max_rzw_arr + derivatives on the first %d harmonic points" % len(sample)}
+
+
 def copy_calibrate(mib):
     """A plain copy kernel of known size: under rocprofv3 --pmc its FETCH_SIZE
     and WRITE_SIZE per dispatch give the counters' scale (bytes moved / KiB read)."""
@@ -512,6 +526,15 @@ def main():
                                 "mean_numharm": float(np.mean([c[3] for c in opt_cands])),
                                 "note": "optimize_accelcand with harmonic polishing of every search candidate "
                                         "(a superset of CPU2's harmonic-filtered list)"}
+        # -noharmpolish: every harmonic of every candidate on its own
+        nhp_pts = [(r * (ii + 1), z * (ii + 1), w * (ii + 1)) for r, z, w, nh in opt_cands for ii in range(nh)]
+        plan.maximize(nhp_pts[:16])  # warm
+        ts = time.perf_counter()
+        plan.maximize(nhp_pts)
+        tmax = time.perf_counter() - ts
+        out["cpu2_noharmpolish"] = {"points": len(nhp_pts), "ms": tmax * 1e3, "points_per_s": len(nhp_pts) / tmax,
+                                    "note": "optimize_accelcand -noharmpolish (max_rzw_arr per harmonic) of every "
+                                            "search candidate's harmonics"}
     # variants of the execution: graph replay of every full batch (on top of the
     # default overlap), and everything in one stream without overlap
     if args.no_legs:
@@ -570,6 +593,10 @@ def main():
         out["cpu_baseline"] = cpu_baseline(fft, kw, args.cpu_rsteps, nthreads,
                                            g.corr_uselen * g.numz * max(g.numw, 1), g.nrsteps // 2)
         out["cpu_baseline"]["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
+        if "cpu2_noharmpolish" in out:
+            out["cpu_baseline"]["cpu2_noharmpolish"] = cpu_maximize_sample(fft, nhp_pts)
+            out["cpu2_noharmpolish"]["gpu_over_cpu"] = (out["cpu_baseline"]["cpu2_noharmpolish"]["ms_per_point"] *
+                                                       out["cpu2_noharmpolish"]["points_per_s"] / 1e3)
         if "cpu2_optimize" in out:
             out["cpu_baseline"]["cpu2_optimize"] = cpu_optimize_sample(fft, opt_cands)
             out["cpu2_optimize"]["gpu_over_cpu"] = (out["cpu_baseline"]["cpu2_optimize"]["ms_per_candidate"] *

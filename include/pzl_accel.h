@@ -225,6 +225,17 @@ int pzl_optimize(void *plan, PzlOptCand *cands, long long n);
 /* With a host spectrum of numbins complex bins (interleaved re, im). */
 int pzl_optimize_spectrum(int device, const float *host_fft, long long numbins, PzlOptCand *cands,
                           long long n);
+/* One harmonic per entry (numharm must be 1), as optimize_accelcand's
+ * -noharmpolish branch calls per harmonic (accel_utils.c:610-652): for w > 0
+ * searches max_rzw_arr (maximize_rzw.c:3-50; fdot_only 0): the first
+ * raw-power maximum of the 9 x 9 x 24 grid over z +- ACCEL_DZ, w +- ACCEL_DW
+ * and three bins from int(r) - 1, derivatives there with the local power of
+ * (r, z, w); for w = 0 searches max_rz_arr (maximize_rz.c:25-85; fdot_only 1):
+ * the (r, z) simplex on the raw power.  Out: (ropt, zopt, wopt), derivs[0],
+ * pows[0] = the maximum's raw power (the functions' return value). */
+int pzl_maximize(void *plan, PzlOptCand *cands, long long n);
+int pzl_maximize_spectrum(int device, const float *host_fft, long long numbins, PzlOptCand *cands,
+                          long long n);
 
 /* ---------------- request loop (continuous batching on one GPU) ----------
  * Replaces the reference's multi-process pipeline

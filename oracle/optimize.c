@@ -16,6 +16,7 @@
 #define NUMFINTBINS 16
 #define NUMLOCPOWAVG 20
 #define DELTAAVGBINS 5
+#define ACCEL_DR 0.5
 #define ACCEL_RDR 2
 #define ACCEL_DZ 2
 #define ACCEL_DW 20
@@ -513,4 +514,74 @@ void oracle_max_rz_arr_harmonics(const float *data, int numdata, int num_harmoni
         oracle_get_derivs3d(data, numdata, *rout * n, *zout * n, 0.0, locpow[ii], &derivs[ii]);
     }
     free(locpow);
+}
+
+/* max_rzw_arr (maximize_rzw.c:3-50): one harmonic, the 9 x 9 x 24 volume over
+ * z +- ACCEL_DZ, w +- ACCEL_DW and three bins from int(rin) - 1, its first
+ * raw-power maximum, and the derivatives there with the local power of the
+ * input point.  Returns that (float) power. */
+double oracle_max_rzw_arr(const float *data, int numdata, double rin, double zin, double win, double *rout,
+                          double *zout, double *wout, OracleRDerivs *derivs)
+{
+    const float locpow = (float)oracle_get_localpower3d(data, numdata, rin, zin, win);
+    float maxpow = 0.0f;
+    const int extra = 10, startbin = (int)rin - 1, interpfac = 4;
+    const int numbetween = interpfac * ACCEL_RDR, numr = 3 * numbetween, numz = 2 * interpfac + 1, numw = numz;
+    const int khw = oracle_w_resp_halfwidth_acc(fabs(zin) + 2, fabs(win) + 20, 0);
+    const int fftlen = oracle_next2_to_n(numbetween * (2 * khw + extra));
+    int nextbin, ii, jj, kk, wind = 0, zind = 0, rind = 0;
+    float *vol = (float *)malloc(sizeof(float) * 2 * (size_t)numw * numz * fftlen);
+    const int ngood = oracle_corr_rzw_vol(data, numdata, numbetween, startbin, zin - ACCEL_DZ, zin + ACCEL_DZ, numz,
+                                          win - ACCEL_DW, win + ACCEL_DW, numw, fftlen, &nextbin, vol);
+    for (ii = 0; ii < numw; ii++)
+        for (jj = 0; jj < numz; jj++)
+            for (kk = 0; kk < numr && kk < ngood; kk++) {
+                const float pw = power_f(vol + 2 * (((size_t)ii * numz + jj) * ngood + kk));
+                if (pw > maxpow) {
+                    maxpow = pw;
+                    wind = ii;
+                    zind = jj;
+                    rind = kk;
+                }
+            }
+    free(vol);
+    *rout = startbin + rind * ACCEL_DR / (double)interpfac;
+    *zout = zin - ACCEL_DZ + zind * ACCEL_DZ / (double)interpfac;
+    *wout = win - ACCEL_DW + wind * ACCEL_DW / (double)interpfac;
+    oracle_get_derivs3d(data, numdata, *rout, *zout, *wout, locpow, derivs);
+    return maxpow;
+}
+
+/* max_rz_arr (maximize_rz.c:25-85): the simplex of max_rz_arr_harmonics on one
+ * harmonic's raw power (power_call_rz, maximize_rz.c:10-23, divides by
+ * nothing), the local power and derivatives at the peak.  Returns the peak's
+ * raw power. */
+double oracle_max_rz_arr(const float *data, int numdata, double rin, double zin, double *rout, double *zout,
+                         OracleRDerivs *derivs)
+{
+    double y[3], x[3][2];
+    const double step = 0.4;
+    float one = 1.0f;
+    RzFunc F = {data, numdata, 1, oracle_z_resp_halfwidth_acc(fabs(zin) + 4.0, 0), &one};
+    int ii;
+    x[0][0] = rin - step;
+    x[0][1] = zin / ZSCALE - step;
+    x[1][0] = rin - step;
+    x[1][1] = zin / ZSCALE + step;
+    x[2][0] = rin + step;
+    x[2][1] = zin / ZSCALE;
+    for (ii = 0; ii < 3; ii++) y[ii] = rz_func(&F, x[ii]);
+    amoeba(&F, x, y, 1.0e-7);
+    F.khw = oracle_z_resp_halfwidth_acc(fabs(x[0][1]) + 4.0, 1);
+    x[1][0] = x[0][0] + 0.01;
+    x[1][1] = x[0][1];
+    x[2][0] = x[0][0];
+    x[2][1] = x[0][1] + 0.01;
+    for (ii = 0; ii < 3; ii++) y[ii] = rz_func(&F, x[ii]);
+    amoeba(&F, x, y, 1.0e-10);
+    *rout = x[0][0];
+    *zout = x[0][1] * ZSCALE;
+    const float lp = (float)oracle_get_localpower3d(data, numdata, *rout, *zout, 0.0);
+    oracle_get_derivs3d(data, numdata, *rout, *zout, 0.0, lp, derivs);
+    return -y[0];
 }

@@ -89,7 +89,13 @@ void oracle_max_rzw_arr_harmonics(const float *data, int numdata, int num_harmon
                                   double *powers);                 /* maximize_rzw.c:85-191 */
 void oracle_max_rz_arr_harmonics(const float *data, int numdata, int num_harmonics, double rin, double zin,
                                  double *rout, double *zout, OracleRDerivs *derivs,
-                                 double *powers);                  /* maximize_rz.c:141-220, amoeba.c */
+                                 double *powers);
+/* max_rzw_arr (maximize_rzw.c:3-50) and max_rz_arr (maximize_rz.c:25-85): one
+ * harmonic each; both return the maximum's raw power */
+double oracle_max_rzw_arr(const float *data, int numdata, double rin, double zin, double win, double *rout,
+                          double *zout, double *wout, OracleRDerivs *derivs);
+double oracle_max_rz_arr(const float *data, int numdata, double rin, double zin, double *rout, double *zout,
+                         OracleRDerivs *derivs);                  /* maximize_rz.c:141-220, amoeba.c */
 int oracle_calc_props(const OracleRDerivs *d, double r, double z, double w,
                       OracleFourierProps *res);                     /* characteristics.c:193-265 */
 

@@ -139,6 +139,12 @@ def lib():
         L.oracle_max_rz_arr_harmonics.argtypes = [P(C.c_float), C.c_int, C.c_int, C.c_double, C.c_double,
                                                   P(C.c_double), P(C.c_double), P(RDerivs), P(C.c_double)]
         L.oracle_calc_props.restype = C.c_int
+        L.oracle_max_rzw_arr.argtypes = [P(C.c_float), C.c_int, C.c_double, C.c_double, C.c_double,
+                                         P(C.c_double), P(C.c_double), P(C.c_double), P(RDerivs)]
+        L.oracle_max_rzw_arr.restype = C.c_double
+        L.oracle_max_rz_arr.argtypes = [P(C.c_float), C.c_int, C.c_double, C.c_double, P(C.c_double),
+                                        P(C.c_double), P(RDerivs)]
+        L.oracle_max_rz_arr.restype = C.c_double
         _lib = L
     return _lib
 
@@ -204,6 +210,23 @@ def max_rz_arr_harmonics(data, numharm, r, z):
     pows = (C.c_double * numharm)()
     lib().oracle_max_rz_arr_harmonics(fptr(a), len(a) // 2, numharm, r, z, C.byref(ro), C.byref(zo), ders, pows)
     return ro.value, zo.value, list(ders), list(pows)
+
+
+def max_rzw_arr(data, r, z, w):
+    """(r, z, w, RDerivs, raw power) of one harmonic's 9 x 9 x 24 volume
+    (maximize_rzw.c:3)."""
+    a = _c64(data)
+    ro, zo, wo, d = C.c_double(), C.c_double(), C.c_double(), RDerivs()
+    p = lib().oracle_max_rzw_arr(fptr(a), len(a) // 2, r, z, w, C.byref(ro), C.byref(zo), C.byref(wo), C.byref(d))
+    return ro.value, zo.value, wo.value, d, p
+
+
+def max_rz_arr(data, r, z):
+    """(r, z, RDerivs, raw power) of one harmonic's simplex (maximize_rz.c:25)."""
+    a = _c64(data)
+    ro, zo, d = C.c_double(), C.c_double(), RDerivs()
+    p = lib().oracle_max_rz_arr(fptr(a), len(a) // 2, r, z, C.byref(ro), C.byref(zo), C.byref(d))
+    return ro.value, zo.value, d, p
 
 
 def calc_props(derivs, r, z, w):

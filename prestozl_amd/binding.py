@@ -157,6 +157,8 @@ def lib():
         L.pzl_plan_sum_form.argtypes = [C.c_void_p, P(C.c_int), P(C.c_int)]
         L.pzl_optimize.argtypes = [C.c_void_p, P(OptCand), C.c_longlong]
         L.pzl_optimize_spectrum.argtypes = [C.c_int, P(C.c_float), C.c_longlong, P(OptCand), C.c_longlong]
+        L.pzl_maximize.argtypes = [C.c_void_p, P(OptCand), C.c_longlong]
+        L.pzl_maximize_spectrum.argtypes = [C.c_int, P(C.c_float), C.c_longlong, P(OptCand), C.c_longlong]
         L.pzl_plan_set_sum_form.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.pzl_corr_num_variants.argtypes = [C.c_int]
         L.pzl_corr_variant_name.argtypes = [C.c_int, C.c_int]
@@ -375,6 +377,13 @@ class Plan:
         check(lib().pzl_optimize(self.h, arr, len(cands)))
         return _opt_result(arr, len(cands))
 
+    def maximize(self, points, fdot_only=False):
+        """-noharmpolish: one harmonic per (r, z, w) point with the plan's
+        resident spectrum (max_rzw_arr / max_rz_arr)."""
+        arr = _opt_array([(r, z, w, 1) for r, z, w in points], fdot_only)
+        check(lib().pzl_maximize(self.h, arr, len(points)))
+        return _opt_result(arr, len(points))
+
     def fftlens(self):
         """The FFT lengths of the plan's harmonics (one kernel variant each)."""
         return sorted({h.fftlen for h in self.harms})
@@ -487,6 +496,19 @@ def optimize_spectrum(fft, cands, device=0, fdot_only=False):
     arr = _opt_array(cands, fdot_only)
     check(lib().pzl_optimize_spectrum(int(device), _fptr(a), len(a) // 2, arr, len(cands)))
     return _opt_result(arr, len(cands))
+
+
+def maximize_spectrum(fft, points, device=0, fdot_only=False):
+    """One harmonic per (r, z, w) point, r in bins of 'fft', as
+    optimize_accelcand's -noharmpolish branch: max_rzw_arr (maximize_rzw.c:3)
+    or, fdot_only, max_rz_arr (maximize_rz.c:25).  [{r, z, w, derivs: [d],
+    pows: [raw maximum power]}, ...]."""
+    a = np.ascontiguousarray(fft)
+    if a.dtype == np.complex64:
+        a = a.view(np.float32)
+    arr = _opt_array([(r, z, w, 1) for r, z, w in points], fdot_only)
+    check(lib().pzl_maximize_spectrum(int(device), _fptr(a), len(a) // 2, arr, len(points)))
+    return _opt_result(arr, len(points))
 
 
 class Server:

@@ -264,43 +264,70 @@ void calc_props(rderivs data, double r, double z, double w, fourierprops *result
 }
 
 // All candidates of a list at once (the loop of accelsearch_lib.c:921-930 as
-// one GPU batch): the rzw volume for w > 0 searches, the (r, z) simplex for
-// w = 0 ones.  Returns 0, or -1 when this path cannot optimise them
-// (-noharmpolish): the caller then keeps the search-grid values.
+// one GPU batch).  With harmonic polishing: the rzw volume of the summed
+// harmonics for w > 0 searches, the (r, z) simplex for w = 0 ones
+// (accel_utils.c:584-608).  With -noharmpolish: every harmonic of every
+// candidate on its own, max_rzw_arr or max_rz_arr at (r, z, w) * (ii + 1)
+// (accel_utils.c:610-652).  Returns 0, or -1 when the spectrum is not in
+// memory (the reference then reads the file, max_rzw_file).
 int pzl_optimize_accelcands(GSList *list, accelobs *obs)
 {
-    if (!obs->use_harmonic_polishing || !(obs->mmap_file || obs->dat_input)) return -1;
+    if (!(obs->mmap_file || obs->dat_input)) return -1;
+    const bool polish = obs->use_harmonic_polishing != 0;
     std::vector<PzlOptCand> oc;
     std::vector<accelcand *> cs;
     for (GSList *p = list; p; p = p->next) {
         accelcand *c = (accelcand *)p->data;
-        PzlOptCand o;
-        memset(&o, 0, sizeof o);
-        o.r = c->r - obs->lobin;
-        o.z = c->z;
-        o.w = c->w;
-        o.numharm = c->numharm;
-        o.fdot_only = obs->numw == 0;
-        oc.push_back(o);
         cs.push_back(c);
+        for (int ii = 0; ii < (polish ? 1 : c->numharm); ii++) {
+            PzlOptCand o;
+            memset(&o, 0, sizeof o);
+            if (polish) {
+                o.r = c->r - obs->lobin;
+                o.z = c->z;
+                o.w = c->w;
+                o.numharm = c->numharm;
+            } else {
+                o.r = c->r * (ii + 1) - obs->lobin;
+                o.z = c->z * (ii + 1);
+                o.w = c->w * (ii + 1);
+                o.numharm = 1;
+            }
+            o.fdot_only = obs->numw == 0;
+            oc.push_back(o);
+        }
     }
     if (oc.empty()) return 0;
-    if (pzl_optimize_spectrum(0, (const float *)obs->fft, obs->numbins, oc.data(), (long long)oc.size()) != 0) {
+    const int rc = polish ? pzl_optimize_spectrum(0, (const float *)obs->fft, obs->numbins, oc.data(),
+                                                  (long long)oc.size())
+                          : pzl_maximize_spectrum(0, (const float *)obs->fft, obs->numbins, oc.data(),
+                                                  (long long)oc.size());
+    if (rc != 0) {
         printf("\nGPU candidate optimisation failed: %s\n\n", pzl_last_error());
         exit(1);
     }
-    for (size_t k = 0; k < cs.size(); k++) {
-        accelcand *c = cs[k];
-        const PzlOptCand &o = oc[k];
+    size_t k = 0;
+    for (accelcand *c : cs) {
         alloc_cand_arrays(c);
         for (int ii = 0; ii < c->numharm; ii++) {
-            c->hirs[ii] = (o.ropt + obs->lobin) * (ii + 1);
-            c->hizs[ii] = o.zopt * (ii + 1);
-            c->hiws[ii] = obs->numw ? o.wopt * (ii + 1) : 0.0;
-            const PzlRDerivs &d = o.derivs[ii];
+            const PzlOptCand &o = polish ? oc[k] : oc[k + ii];
+            if (polish) {
+                c->hirs[ii] = (o.ropt + obs->lobin) * (ii + 1);
+                c->hizs[ii] = o.zopt * (ii + 1);
+                c->hiws[ii] = obs->numw ? o.wopt * (ii + 1) : 0.0;
+            } else {
+                c->hirs[ii] = o.ropt + obs->lobin;
+                c->hizs[ii] = o.zopt;
+                // w = 0: max_rz_arr sets no w (the reference leaves the
+                // uncleared gen_dvect value); 0 here
+                c->hiws[ii] = obs->numw ? o.wopt : 0.0;
+            }
+            const int h = polish ? ii : 0;
+            const PzlRDerivs &d = o.derivs[h];
             c->derivs[ii] = rderivs{d.pow, d.phs, d.dpow, d.dphs, d.d2pow, d.d2phs, d.locpow};
-            c->pows[ii] = o.pows[ii];
+            c->pows[ii] = o.pows[h];
         }
+        k += polish ? 1 : (size_t)c->numharm;
         c->sigma = (float)pzl_candidate_sigma(c->power, c->numharm,
                                               (double)obs->numindep[twon_to_index(c->numharm)]);
     }
@@ -313,7 +340,7 @@ void optimize_accelcand(accelcand *cand, accelobs *obs)
     one.data = cand;
     one.next = nullptr;
     if (pzl_optimize_accelcands(&one, obs) != 0) {
-        printf("\noptimize_accelcand: only harmonic polishing is done on the GPU (not -noharmpolish)\n\n");
+        printf("\noptimize_accelcand: the spectrum is not in memory\n\n");
         exit(1);
     }
 }

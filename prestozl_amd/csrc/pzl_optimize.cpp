@@ -127,6 +127,7 @@ struct OptBuffers {
     int *err = nullptr;
     float *powv = nullptr, *tail = nullptr, *locpow = nullptr;
     int *job0 = nullptr, *nharm = nullptr, *best = nullptr;
+    float *bestv = nullptr;
     size_t nv_cap = 0, nc_cap = 0;
     ~OptBuffers()
     {
@@ -141,6 +142,7 @@ struct OptBuffers {
         (void)hipFree(job0);
         (void)hipFree(nharm);
         (void)hipFree(best);
+        (void)hipFree(bestv);
     }
     int interp(size_t n)
     {
@@ -182,11 +184,14 @@ struct OptBuffers {
             (void)hipFree(job0);
             (void)hipFree(nharm);
             (void)hipFree(best);
+            (void)hipFree(bestv);
             job0 = nharm = best = nullptr;
+            bestv = nullptr;
             nc_cap = 0;
             PZL_HIPCHK(hipMalloc(&job0, sizeof(int) * nc));
             PZL_HIPCHK(hipMalloc(&nharm, sizeof(int) * nc));
             PZL_HIPCHK(hipMalloc(&best, sizeof(int) * nc));
+            PZL_HIPCHK(hipMalloc(&bestv, sizeof(float) * nc));
             nc_cap = nc;
         }
         return 0;
@@ -208,10 +213,16 @@ int run_interp(OptBuffers &B, const float2 *d_data, int numdata, const std::vect
     return 0;
 }
 
-// One pass over candidates [c0, c1).
+// One pass over candidates [c0, c1).  single: every candidate is one harmonic
+// searched as max_rzw_arr does (maximize_rzw.c:3-50): a 9 x 9 grid over z +-
+// ACCEL_DZ, w +- ACCEL_DW, the raw power's maximum, the input point's local
+// power in the derivatives.
 int optimize_chunk(OptBuffers &B, const float2 *d_data, int numdata, PzlOptCand *cands, long long c0, long long c1,
-                   hipStream_t st)
+                   bool single, hipStream_t st)
 {
+    const int nz = single ? 2 * PZL_OPT_INTERPFAC + 1 : PZL_OPT_NUMZ;  // maximize_rzw.c:21
+    const int nw = single ? 2 * PZL_OPT_INTERPFAC + 1 : PZL_OPT_NUMW;
+    const double zwid = single ? 1.0 : PZL_OPT_NUMWID;  // half-range in units of ACCEL_DZ, ACCEL_DW
     const long long nc = c1 - c0;
     std::vector<PzlVolJob> vj;
     std::vector<int> job0((size_t)nc), nharm((size_t)nc);
@@ -238,12 +249,13 @@ int optimize_chunk(OptBuffers &B, const float2 *d_data, int numdata, PzlOptCand 
             J.lobin = (int)lobin;
             J.fftlen = pzl_next2_to_n(PZL_OPT_NB * (2 * khw_n + 10));
             // corr_rzw_vol (rzwinterp.c:34-60) on the z and w ranges of :130-135
-            const double zlo = zh - PZL_OPT_NUMWID * hfrac * kDZ, zhi = zh + PZL_OPT_NUMWID * hfrac * kDZ;
-            const double wlo = wh - PZL_OPT_NUMWID * hfrac * kDW, whi = wh + PZL_OPT_NUMWID * hfrac * kDW;
+            // (max_rzw_arr: zin -+ ACCEL_DZ, win -+ ACCEL_DW, maximize_rzw.c:27-30)
+            const double zlo = zh - zwid * hfrac * kDZ, zhi = zh + zwid * hfrac * kDZ;
+            const double wlo = wh - zwid * hfrac * kDW, whi = wh + zwid * hfrac * kDW;
             J.zlo = zlo;
             J.wlo = wlo;
-            J.dz = (zhi - zlo) / (PZL_OPT_NUMZ - 1);
-            J.dw = (whi - wlo) / (PZL_OPT_NUMW - 1);
+            J.dz = (zhi - zlo) / (nz - 1);
+            J.dw = (whi - wlo) / (nw - 1);
             const double maxz = (fabs(zlo) < fabs(zhi)) ? zhi : zlo;
             const double maxw = (fabs(wlo) < fabs(whi)) ? whi : wlo;
             J.khw = pzl_w_resp_halfwidth_acc(maxz, maxw, false);
@@ -265,8 +277,8 @@ int optimize_chunk(OptBuffers &B, const float2 *d_data, int numdata, PzlOptCand 
                 return fail(PZL_ERR_UNSUPPORTED, "candidate at r %.3f z %.2f w %.2f: harmonic %d needs an FFT of %d",
                             C.r, C.z, C.w, n, J.fftlen);
             ldsn = std::max(ldsn, numkern);
-            for (int i = 0; i < PZL_OPT_NUMW; i++)
-                for (int j = 0; j < PZL_OPT_NUMZ; j++) {
+            for (int i = 0; i < nw; i++)
+                for (int j = 0; j < nz; j++) {
                     const int npts = w_response_npts(zlo + j * J.dz, wlo + i * J.dw, numkern, PZL_OPT_NB);
                     if (npts > PZL_OPT_RESP_FFT_MAX)
                         return fail(PZL_ERR_UNSUPPORTED,
@@ -285,23 +297,56 @@ int optimize_chunk(OptBuffers &B, const float2 *d_data, int numdata, PzlOptCand 
     // volumes (independent of the local powers, which only scale the sum)
     PZL_HIPCHK(hipMemcpyAsync(B.vjobs, vj.data(), sizeof(PzlVolJob) * nv, hipMemcpyHostToDevice, st));
     PZL_HIPCHK(hipMemsetAsync(B.err, 0, sizeof(int), st));
-    PZL_HIPCHK(pzl_launch_opt_volume(d_data, numdata, B.vjobs, (int)nv, ldsn, B.tw, B.powv, B.tail, B.err, st));
+    PZL_HIPCHK(pzl_launch_opt_volume(d_data, numdata, B.vjobs, (int)nv, ldsn, nz, nw, B.tw, B.powv, B.tail, B.err,
+                                     st));
     std::vector<float2> ans;
     rc = run_interp(B, d_data, numdata, lj, ans, st);
     if (rc) return rc;
-    std::vector<float> locpow(nv);
-    for (size_t j = 0; j < nv; j++) locpow[j] = (float)localpower_from(ans.data() + lj0[j], ljn[j]);
+    // the sum's local powers (1: max_rzw_arr compares raw powers)
+    std::vector<float> locpow(nv), inpow(nv);
+    for (size_t j = 0; j < nv; j++) {
+        inpow[j] = (float)localpower_from(ans.data() + lj0[j], ljn[j]);
+        locpow[j] = single ? 1.0f : inpow[j];
+    }
     std::vector<int> best((size_t)nc);
+    std::vector<float> bestv((size_t)nc);
     PZL_HIPCHK(hipMemcpyAsync(B.locpow, locpow.data(), sizeof(float) * nv, hipMemcpyHostToDevice, st));
     PZL_HIPCHK(hipMemcpyAsync(B.job0, job0.data(), sizeof(int) * nc, hipMemcpyHostToDevice, st));
     PZL_HIPCHK(hipMemcpyAsync(B.nharm, nharm.data(), sizeof(int) * nc, hipMemcpyHostToDevice, st));
-    PZL_HIPCHK(pzl_launch_opt_sum(B.powv, B.tail, B.vjobs, B.locpow, B.job0, B.nharm, (int)nc, B.best, st));
+    PZL_HIPCHK(pzl_launch_opt_sum(B.powv, B.tail, B.vjobs, B.locpow, B.job0, B.nharm, (int)nc, nz * nw, B.best,
+                                  B.bestv, st));
     PZL_HIPCHK(hipMemcpyAsync(best.data(), B.best, sizeof(int) * nc, hipMemcpyDeviceToHost, st));
+    PZL_HIPCHK(hipMemcpyAsync(bestv.data(), B.bestv, sizeof(float) * nc, hipMemcpyDeviceToHost, st));
     int verr = 0;
     PZL_HIPCHK(hipMemcpyAsync(&verr, B.err, sizeof(int), hipMemcpyDeviceToHost, st));
     PZL_HIPCHK(hipStreamSynchronize(st));
     if (verr) return fail(PZL_ERR_UNSUPPORTED, "a volume row's response exceeded the launch's LDS");
 
+    if (single) {
+        // max_rzw_arr's maximum (maximize_rzw.c:46-48) and the derivatives
+        // there with the input point's local power (:4, :52)
+        std::vector<PzlInterpJob> dj;
+        for (long long c = 0; c < nc; c++) {
+            PzlOptCand &C = cands[c0 + c];
+            const int idx = best[c];
+            const int wind = idx / (nz * PZL_OPT_NUMR), zind = (idx / PZL_OPT_NUMR) % nz, rind = idx % PZL_OPT_NUMR;
+            C.ropt = hhlobin[c] + rind * 0.5 / (double)PZL_OPT_INTERPFAC;
+            C.zopt = C.z - kDZ + zind * kDZ / (double)PZL_OPT_INTERPFAC;
+            C.wopt = C.w - kDW + wind * kDW / (double)PZL_OPT_INTERPFAC;
+            derivs_jobs(C.ropt, C.zopt, C.wopt, dj);
+        }
+        rc = check_interp_jobs(dj, 0);
+        if (rc) return rc;
+        rc = run_interp(B, d_data, numdata, dj, ans, st);
+        if (rc) return rc;
+        const size_t per = dj.size() / (size_t)(nc ? nc : 1);
+        for (long long c = 0; c < nc; c++) {
+            PzlOptCand &C = cands[c0 + c];
+            C.derivs[0] = derivs_from(ans.data() + per * c, inpow[c]);
+            C.pows[0] = bestv[c];
+        }
+        return 0;
+    }
     // the maximum (maximize_rzw.c:177-184), then every harmonic's local power
     // and derivatives there (:186-193)
     std::vector<PzlInterpJob> dj;
@@ -516,14 +561,17 @@ void rz_point_jobs(const Simplex &S, const double *pt, std::vector<PzlInterpJob>
     }
 }
 
-int optimize_rz(OptBuffers &B, const float2 *d_data, int numdata, PzlOptCand *cands, long long nc, hipStream_t st)
+// single: one harmonic each, on its raw power as max_rz_arr does
+// (maximize_rz.c:25-85; power_call_rz, :10-23, divides by nothing).
+int optimize_rz(OptBuffers &B, const float2 *d_data, int numdata, PzlOptCand *cands, long long nc, bool single,
+                hipStream_t st)
 {
     std::vector<Simplex> S((size_t)nc);
     std::vector<PzlInterpJob> jobs;
     std::vector<float2> ans;
     std::vector<int> l0, ln;
     // local powers at the start (maximize_rz.c:157-161)
-    for (long long c = 0; c < nc; c++)
+    for (long long c = 0; c < nc && !single; c++)
         for (int ii = 0; ii < cands[c].numharm; ii++) {
             const int n = ii + 1;
             l0.push_back((int)jobs.size());
@@ -538,7 +586,8 @@ int optimize_rz(OptBuffers &B, const float2 *d_data, int numdata, PzlOptCand *ca
         Simplex &X = S[c];
         const PzlOptCand &C = cands[c];
         X.nh = C.numharm;
-        for (int ii = 0; ii < C.numharm; ii++, k++) X.locpow.push_back((float)localpower_from(ans.data() + l0[k], ln[k]));
+        for (int ii = 0; ii < C.numharm; ii++, k++)
+            X.locpow.push_back(single ? 1.0f : (float)localpower_from(ans.data() + l0[k], ln[k]));
         X.khw = pzl_z_resp_halfwidth_acc(fabs(C.z * C.numharm) + 4.0, false);
         const double step = 0.4;  // maximize_rz.c:165-172
         const double init[3][2] = {{C.r - step, C.z / kZScale - step},
@@ -610,13 +659,14 @@ int optimize_rz(OptBuffers &B, const float2 *d_data, int numdata, PzlOptCand *ca
     return 0;
 }
 
-int optimize_on(const float2 *d_data, long long numbins, PzlOptCand *cands, long long n, hipStream_t st)
+int optimize_on(const float2 *d_data, long long numbins, PzlOptCand *cands, long long n, bool single,
+                hipStream_t st)
 {
     if (numbins <= 0 || numbins > INT_MAX) return fail(PZL_ERR_ARG, "spectrum of %lld bins", numbins);
     for (long long c = 0; c < n; c++) {
         const PzlOptCand &C = cands[c];
         if (C.numharm < 1 || C.numharm > PZL_MAX_NUMHARM || !(C.r >= 0.0) || !std::isfinite(C.z) ||
-            !std::isfinite(C.w))
+            !std::isfinite(C.w) || (single && C.numharm != 1))
             return fail(PZL_ERR_ARG, "candidate %lld: r %.3f, numharm %d", c, C.r, C.numharm);
     }
     OptBuffers B;
@@ -629,7 +679,7 @@ int optimize_on(const float2 *d_data, long long numbins, PzlOptCand *cands, long
             rz_at.push_back(c);
         }
     if (!rz.empty()) {
-        const int rc = optimize_rz(B, d_data, (int)numbins, rz.data(), (long long)rz.size(), st);
+        const int rc = optimize_rz(B, d_data, (int)numbins, rz.data(), (long long)rz.size(), single, st);
         if (rc) return rc;
         for (size_t k = 0; k < rz.size(); k++) cands[rz_at[k]] = rz[k];
     }
@@ -645,7 +695,7 @@ int optimize_on(const float2 *d_data, long long numbins, PzlOptCand *cands, long
     while (c0 < nv) {
         long long c1 = c0, jobs = 0;
         while (c1 < nv && (c1 == c0 || jobs + vw[c1].numharm <= kChunkJobs)) jobs += vw[c1++].numharm;
-        const int rc = optimize_chunk(B, d_data, (int)numbins, vw.data(), c0, c1, st);
+        const int rc = optimize_chunk(B, d_data, (int)numbins, vw.data(), c0, c1, single, st);
         if (rc) return rc;
         c0 = c1;
     }
@@ -664,10 +714,12 @@ int pzl_optimize(void *plan, PzlOptCand *cands, long long n)
     if (!p->d_fft) return fail(PZL_ERR_NOSPECTRUM, "no spectrum set");
     PZL_HIPCHK(hipSetDevice(p->device));
     PZL_HIPCHK(hipStreamSynchronize(p->st));
-    return optimize_on(p->d_fft, p->fft_bins, cands, n, p->st);
+    return optimize_on(p->d_fft, p->fft_bins, cands, n, false, p->st);
 }
 
-int pzl_optimize_spectrum(int device, const float *host_fft, long long numbins, PzlOptCand *cands, long long n)
+namespace {
+int on_host_spectrum(int device, const float *host_fft, long long numbins, PzlOptCand *cands, long long n,
+                     bool single)
 {
     if (!host_fft || numbins <= 0 || (n > 0 && !cands)) return fail(PZL_ERR_ARG, "bad arguments");
     PZL_HIPCHK(hipSetDevice(device));
@@ -676,12 +728,33 @@ int pzl_optimize_spectrum(int device, const float *host_fft, long long numbins, 
     PZL_HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     hipError_t e = hipMalloc(&d, sizeof(float2) * numbins);
     if (e == hipSuccess) e = hipMemcpyAsync(d, host_fft, sizeof(float2) * numbins, hipMemcpyHostToDevice, st);
-    int rc = e == hipSuccess ? optimize_on(d, numbins, cands, n, st)
+    int rc = e == hipSuccess ? optimize_on(d, numbins, cands, n, single, st)
                              : fail(PZL_ERR_HIP, "spectrum upload: %s", hipGetErrorString(e));
     (void)hipStreamSynchronize(st);
     (void)hipFree(d);
     (void)hipStreamDestroy(st);
     return rc;
+}
+}  // namespace
+
+int pzl_optimize_spectrum(int device, const float *host_fft, long long numbins, PzlOptCand *cands, long long n)
+{
+    return on_host_spectrum(device, host_fft, numbins, cands, n, false);
+}
+
+int pzl_maximize(void *plan, PzlOptCand *cands, long long n)
+{
+    Plan *p = (Plan *)plan;
+    if (!p || (n > 0 && !cands)) return fail(PZL_ERR_ARG, "null argument");
+    if (!p->d_fft) return fail(PZL_ERR_NOSPECTRUM, "no spectrum set");
+    PZL_HIPCHK(hipSetDevice(p->device));
+    PZL_HIPCHK(hipStreamSynchronize(p->st));
+    return optimize_on(p->d_fft, p->fft_bins, cands, n, true, p->st);
+}
+
+int pzl_maximize_spectrum(int device, const float *host_fft, long long numbins, PzlOptCand *cands, long long n)
+{
+    return on_host_spectrum(device, host_fft, numbins, cands, n, true);
 }
 
 }  // extern "C"

@@ -40,11 +40,17 @@ hipError_t pzl_launch_opt_interp(const float2 *data, int numdata, const PzlInter
 hipError_t pzl_launch_opt_twiddles(double2 *tw, hipStream_t st);
 // ldsn (<= PZL_OPT_RESP_FFT_MAX): at least the largest numkern (16 khw) of the
 // jobs and the largest chirp length of their rows' w responses; a row that
-// needs more ORs 1 into *err
+// needs more ORs 1 into *err.  Each job's grid has nz z by nw w rows
+// (PZL_OPT_NUMZ x PZL_OPT_NUMW for the harmonic sum, 9 x 9 for one harmonic's
+// max_rzw_arr); powv holds [job][w][z][PZL_OPT_NUMR], tail [job][w][z][PZL_OPT_TAIL].
 hipError_t pzl_launch_opt_volume(const float2 *data, int numdata, const PzlVolJob *jobs, int njobs, int ldsn,
-                                 const double2 *tw, float *powv, float *tail, int *err, hipStream_t st);
+                                 int nz, int nw, const double2 *tw, float *powv, float *tail, int *err,
+                                 hipStream_t st);
+// best[c]: the flat (w, z, r) index of candidate c's maximum; bestv[c] (bestv
+// may be null) its value
 hipError_t pzl_launch_opt_sum(const float *powv, const float *tail, const PzlVolJob *jobs, const float *locpow,
-                              const int *job0, const int *nharm, int ncand, int *best, hipStream_t st);
+                              const int *job0, const int *nharm, int ncand, int nzw, int *best, float *bestv,
+                              hipStream_t st);
 #ifdef __cplusplus
 }
 #endif

@@ -208,7 +208,7 @@ __global__ __launch_bounds__(OPT_NT, 2) void k_opt_volrow(const float2 *__restri
                                                        const PzlVolJob *__restrict__ jobs,
                                                        const double2 *__restrict__ tw, int ldsn,
                                                        float *__restrict__ powv, float *__restrict__ tail,
-                                                       int *__restrict__ err)
+                                                       int *__restrict__ err, int nz, int nzw)
 {
     extern __shared__ double2 sbuf[];
     constexpr int NOUT = PZL_OPT_NUMR + PZL_OPT_TAIL;  // 32 outputs per row
@@ -217,9 +217,8 @@ __global__ __launch_bounds__(OPT_NT, 2) void k_opt_volrow(const float2 *__restri
     __shared__ int lst[5][NOUT];    // output slots of each residue pass
     __shared__ int lcnt[5];
     const int tid = threadIdx.x, lane = tid % GL, grp = tid / GL;
-    constexpr int NZW = PZL_OPT_NUMZ * PZL_OPT_NUMW;
-    const int job = blockIdx.x / NZW, ij = blockIdx.x % NZW;
-    const int wi = ij / PZL_OPT_NUMZ, zj = ij % PZL_OPT_NUMZ;
+    const int job = blockIdx.x / nzw, ij = blockIdx.x % nzw;
+    const int wi = ij / nz, zj = ij % nz;
     const PzlVolJob &J = jobs[job];
     const int khw = J.khw, beginbin = J.lobin - khw, nterm = 2 * khw;
     const double z = J.zlo + zj * J.dz, w = J.wlo + wi * J.dw;
@@ -281,8 +280,8 @@ __global__ __launch_bounds__(OPT_NT, 2) void k_opt_volrow(const float2 *__restri
             if (o >= 0 && lane == 0) {
                 const float fr = (float)ar, fi = (float)ai;
                 const float pw = fr * fr + fi * fi;
-                if (o < PZL_OPT_NUMR) powv[((long long)job * NZW + ij) * PZL_OPT_NUMR + o] = pw;
-                else tail[((long long)job * NZW + ij) * PZL_OPT_TAIL + (o - PZL_OPT_NUMR)] = pw;
+                if (o < PZL_OPT_NUMR) powv[((long long)job * nzw + ij) * PZL_OPT_NUMR + o] = pw;
+                else tail[((long long)job * nzw + ij) * PZL_OPT_TAIL + (o - PZL_OPT_NUMR)] = pw;
             }
         }
     };
@@ -333,16 +332,17 @@ __global__ __launch_bounds__(OPT_NT, 2) void k_opt_volrow(const float2 *__restri
 // (w, z, r) order with maxpow starting at 0 (:161-175).  The volume is one
 // contiguous [w][z][r] block in the reference, so a negative r index reads
 // the end of the preceding row; before the first row (memory outside the
-// block there) the power is taken as 0.
+// block there) the power is taken as 0.  With a local power of 1 and one
+// harmonic this is max_rzw_arr's raw-power maximum (maximize_rzw.c:31-45).
+// The grid has nzw (w, z) rows; bestv (if not null) gets the maximum.
 __global__ __launch_bounds__(OPT_NT) void k_opt_sum(const float *__restrict__ powv,
                                                     const float *__restrict__ tail,
                                                     const PzlVolJob *__restrict__ jobs,
                                                     const float *__restrict__ locpow,
                                                     const int *__restrict__ job0, const int *__restrict__ nharm,
-                                                    int *__restrict__ best)
+                                                    int *__restrict__ best, float *__restrict__ bestv, int nzw)
 {
-    constexpr int NZW = PZL_OPT_NUMZ * PZL_OPT_NUMW;
-    constexpr int NCELL = PZL_OPT_NUMZ * PZL_OPT_NUMW * PZL_OPT_NUMR;
+    const int NZW = nzw, NCELL = nzw * PZL_OPT_NUMR;
     __shared__ float rv[OPT_NT];
     __shared__ int ri[OPT_NT];
     const int tid = threadIdx.x, c = blockIdx.x;
@@ -379,7 +379,10 @@ __global__ __launch_bounds__(OPT_NT) void k_opt_sum(const float *__restrict__ po
         }
         __syncthreads();
     }
-    if (tid == 0) best[c] = rv[0] > 0.0f ? ri[0] : 0;
+    if (tid == 0) {
+        best[c] = rv[0] > 0.0f ? ri[0] : 0;
+        if (bestv) bestv[c] = rv[0];
+    }
 }
 
 extern "C" hipError_t pzl_launch_opt_interp(const float2 *data, int numdata, const PzlInterpJob *jobs, int njobs,
@@ -397,21 +400,24 @@ extern "C" hipError_t pzl_launch_opt_twiddles(double2 *tw, hipStream_t st)
 }
 
 extern "C" hipError_t pzl_launch_opt_volume(const float2 *data, int numdata, const PzlVolJob *jobs, int njobs,
-                                            int ldsn, const double2 *tw, float *powv, float *tail, int *err,
-                                            hipStream_t st)
+                                            int ldsn, int nz, int nw, const double2 *tw, float *powv, float *tail,
+                                            int *err, hipStream_t st)
 {
     if (njobs <= 0) return hipSuccess;
-    if (ldsn <= 0 || ldsn > PZL_OPT_RESP_FFT_MAX) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_opt_volrow, dim3(njobs * PZL_OPT_NUMZ * PZL_OPT_NUMW), dim3(OPT_NT),
-                       sizeof(double2) * (size_t)ldsn, st, data, numdata, jobs, tw, ldsn, powv, tail, err);
+    if (ldsn <= 0 || ldsn > PZL_OPT_RESP_FFT_MAX || nz < 1 || nw < 1 || nz > PZL_OPT_NUMZ || nw > PZL_OPT_NUMW)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_opt_volrow, dim3(njobs * nz * nw), dim3(OPT_NT), sizeof(double2) * (size_t)ldsn, st, data,
+                       numdata, jobs, tw, ldsn, powv, tail, err, nz, nz * nw);
     return hipGetLastError();
 }
 
 extern "C" hipError_t pzl_launch_opt_sum(const float *powv, const float *tail, const PzlVolJob *jobs,
                                          const float *locpow, const int *job0, const int *nharm, int ncand,
-                                         int *best, hipStream_t st)
+                                         int nzw, int *best, float *bestv, hipStream_t st)
 {
     if (ncand <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_opt_sum, dim3(ncand), dim3(OPT_NT), 0, st, powv, tail, jobs, locpow, job0, nharm, best);
+    if (nzw < 1 || nzw > PZL_OPT_NUMZ * PZL_OPT_NUMW) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_opt_sum, dim3(ncand), dim3(OPT_NT), 0, st, powv, tail, jobs, locpow, job0, nharm, best,
+                       bestv, nzw);
     return hipGetLastError();
 }

@@ -355,3 +355,43 @@ def test_driver_jerk_search_outputs(tmp_path):
         assert abs(p["phs"] - e.phs) < 1e-4
         assert p["pur"] == pytest.approx(e.pur, rel=3e-2)
         assert abs(p["cen"] - e.cen) < 1e-3
+
+
+@pytest.mark.gpu
+def test_driver_noharmpolish_outputs(tmp_path):
+    """CPU2 with -noharmpolish (accel_utils.c:610-652): every harmonic of
+    every candidate maximised on its own, max_rzw_arr at (r, z, w) * (ii + 1).
+    The .cand records hold the fundamental's properties from its own maximum,
+    and each harmonics-table row's 'r' is that harmonic's own maximum, as the
+    oracle's max_rzw_arr gives them."""
+    from prestozl_amd import synth
+    N, dt = 1 << 20, 1e-4
+    sigs = [(0.06, 123.4, 8.0, 30.0), (0.04, 2 * 123.4, 16.0, 60.0)]
+    x = synth.timeseries(N, dt, sigs, seed=21)
+    fft = synth.realfft_packed(x)
+    root = D.setup_files(tmp_path, fft, N, dt, name="nhp")
+    r = subprocess.run([DRIVER, "-zmax", "20", "-wmax", "40", "-numharm", "2", "-sigma", "3.0", "-noharmpolish",
+                        root + ".fft"], capture_output=True, text=True, timeout=300, cwd=str(tmp_path))
+    print(r.stdout[-1500:])
+    assert r.returncode == 0, r.stdout + r.stderr
+    base = root + "_ACCEL_20_JERK_40"
+    props = np.fromfile(base + ".cand", dtype=FOURIERPROPS)
+    fund, harm = _accel_tables(base)[:2]
+    numharm = [int(row.split()[4]) for row in fund]
+    assert len(props) == len(fund) > 0 and len(harm) == sum(numharm)
+    hrows = iter(harm)
+    for p, nh in zip(props, numharm):
+        for ii in range(nh):
+            ro, zo, wo, d, maxpow = O.max_rzw_arr(fft, p["r"] * (ii + 1), p["z"] * (ii + 1), p["w"] * (ii + 1))
+            tok = next(hrows).split()
+            t0 = 1 if ii == 0 else 0  # the candidate number opens its first row
+            assert int(tok[t0]) == ii + 1
+            txt = tok[t0 + 4].split("(")[0]  # printed to the digits of its error
+            hr = float(txt)
+            dec = len(txt.split(".")[1]) if "." in txt else 0
+            print("cand r %.3f harm %d: table r %s, oracle %.4f" % (p["r"], ii + 1, txt, ro))
+            assert abs(hr - ro) <= 0.5 * 10.0 ** -dec + 1e-9
+            if ii == 0:
+                e = O.calc_props(d, p["r"], p["z"], p["w"])
+                for k in ("pow", "rawpow", "sig", "locpow"):
+                    assert p[k] == pytest.approx(getattr(e, k), rel=1e-4), k

@@ -150,3 +150,71 @@ def test_rz_simplex_finds_injection(numharm):
     assert pows[0] == pytest.approx((N / 2) ** 2, rel=2e-2)
     assert ders[0].pow == pytest.approx(pows[0], rel=1e-2)
     assert p.pur == pytest.approx(1.0, abs=0.01)
+
+
+def _single_grid_brute_force(data, rin, zin, win):
+    """max_rzw_arr's grid by direct interpolation: r = int(rin) - 1 + k / 8
+    (k < 24), z = zin - 2 + j / 2, w = win - 20 + 5 i (i, j < 9), with the
+    kernel half-width of the grid's extreme z and w (LOWACC)."""
+    zlo, zhi, wlo, whi = zin - 2.0, zin + 2.0, win - 20.0, win + 20.0
+    khw = O.w_resp_halfwidth_acc(zhi if abs(zlo) < abs(zhi) else zlo, whi if abs(wlo) < abs(whi) else wlo, False)
+    sb = int(rin) - 1
+    p = np.zeros((9, 9, 24))
+    for i in range(9):
+        for j in range(9):
+            for k in range(24):
+                a = O.rzw_interp(data, sb + k / 8.0, zlo + j * (zhi - zlo) / 8, wlo + i * (whi - wlo) / 8, khw)
+                p[i, j, k] = abs(a) ** 2
+    return p, sb
+
+
+@pytest.mark.parametrize("dz,dw", [(0.3, -4.0), (3.0, 0.0), (-0.6, 31.0)])
+def test_single_harmonic_volume_is_its_grid_maximum(dz, dw):
+    """max_rzw_arr (maximize_rzw.c:3): the raw-power maximum of a 9 x 9 x 24
+    grid over z +- 2, w +- 20 and three bins from int(r) - 1, found here by
+    brute force over direct interpolations.  An injection beyond the z (or w)
+    range puts the maximum on the range's edge (a 17-point, +- 4 grid would
+    follow it out); the power is raw (N A / 2)^2-scale, not local-power
+    normalised; the derivatives carry the local power of the input point."""
+    f0, z0, w = 4321.2, 7.0, 40.0
+    data = chirp([(1.0, f0, z0, w)], noise=2.0, seed=21)
+    r, z, ww = presto_rzw(f0, z0, w)
+    rin, zin, win = r + 0.3, z - dz, ww - dw
+    ro, zo, wo, d, maxpow = O.max_rzw_arr(data, rin, zin, win)
+    p, sb = _single_grid_brute_force(data, rin, zin, win)
+    i, j, k = np.unravel_index(np.argmax(p), p.shape)
+    srt = np.sort(p.ravel())
+    assert maxpow == pytest.approx(p.max(), rel=3e-3)
+    if srt[-2] < 0.995 * srt[-1]:
+        assert (ro, zo, wo) == pytest.approx((sb + k / 8.0, zin - 2 + j / 2.0, win - 20 + 5.0 * i), abs=1e-9)
+    else:
+        assert p[int(round((wo - win + 20) / 5)), int(round((zo - zin + 2) * 2)), int(round((ro - sb) * 8))] \
+            >= 0.995 * p.max()
+    if abs(dz) > 2.0:
+        assert zo == pytest.approx(zin + np.sign(dz) * 2.0)
+    elif abs(dw) > 20.0:
+        assert wo == pytest.approx(win + np.sign(dw) * 20.0)
+    else:
+        assert abs(ro - r) <= 0.125 + 1e-9 and abs(zo - z) <= 0.5 + 0.2
+    assert maxpow > 0.5 * (N / 2) ** 2
+    assert d.locpow == np.float32(O.localpower3d(data, rin, zin, win))
+    assert d.locpow != np.float32(O.localpower3d(data, ro, zo, wo))
+
+
+@pytest.mark.parametrize("dr,dz", [(0.2, 0.7), (-0.4, -1.3)])
+def test_single_harmonic_simplex_is_raw_power_peak(dr, dz):
+    """max_rz_arr (maximize_rz.c:25): the simplex of max_rz_arr_harmonics on
+    the raw power of one harmonic lands on the same peak, and returns that
+    peak's raw power (the coherent (N A / 2)^2, not a local-power ratio) with
+    the derivatives at the peak."""
+    f0, z0 = 3310.6, -5.2
+    data = chirp([(1.0, f0, z0, 0.0)], noise=0.3, seed=4)
+    r, z = f0 + 0.5 * z0, z0
+    ro, zo, d, pw = O.max_rz_arr(data, r + dr, z + dz)
+    rh, zh, dh, ph = O.max_rz_arr_harmonics(data, 1, r + dr, z + dz)
+    assert abs(ro - r) < 2e-3 and abs(zo - z) < 2e-2
+    assert abs(ro - rh) < 1e-5 and abs(zo - zh) < 1e-4
+    assert pw == pytest.approx((N / 2) ** 2, rel=2e-2)
+    assert pw == pytest.approx(ph[0], rel=1e-6)
+    assert d.locpow == np.float32(O.localpower3d(data, ro, zo, 0.0))
+    assert d.pow == pytest.approx(pw, rel=1e-2)
