@@ -905,7 +905,8 @@ void accelsearch_CPU2(GSList **cands, accelobs *obs, infodata *idata, Cmdline *c
     if (numcands) {
         *cands = sort_accelcands(*cands);
         if (cmd->numharm > 1 && !cmd->noharmremoveP) *cands = eliminate_harmonics(*cands, &numcands);
-        if (pzl_optimize_accelcands(*cands, obs) == 0) {
+        if (pzl_optimize_accelcands(*cands, obs) != 0) die(1, "\nThe spectrum is not in memory\n\n");
+        {
             // the fundamentals' properties, with the search-grid errors
             // (accelsearch_lib.c:932-948), then the text and binary outputs
             std::vector<fourierprops> props((size_t)numcands);
@@ -924,20 +925,6 @@ void accelsearch_CPU2(GSList **cands, accelobs *obs, infodata *idata, Cmdline *c
                 die(1, "\nCannot write '%s'\n\n", obs->candnm);
             fclose(f);
             printf("\n\n");
-        } else {
-            // -noharmpolish: not optimised here; the search-grid list is
-            // written instead
-            std::string fn = std::string(obs->accelnm) + ".pzlcands";
-            FILE *f = fopen(fn.c_str(), "w");
-            if (f) {
-                fprintf(f, "# sigma  power  numharm  r  z  w  (search-grid values, not optimised)\n");
-                for (GSList *p = *cands; p; p = p->next) {
-                    const accelcand *c = (const accelcand *)p->data;
-                    fprintf(f, "%.6f %.4f %d %.4f %.4f %.4f\n", c->sigma, c->power, c->numharm, c->r, c->z, c->w);
-                }
-                fclose(f);
-            }
-            printf("Search-grid candidates (not optimised) are in '%s'.\n\n", fn.c_str());
         }
     } else {
         printf("No candidates above sigma = %.2f were found.\n\n", obs->sigma);
