@@ -8,7 +8,9 @@
 //                   one subband-major window S[s][0 .. 2 worklen)
 //   k_dd_sum        out[dm][x] = sum_s S[s][x + offsets[dm][s]], subbands in
 //                   ascending order (the reference's float summation order),
-//                   output tiles mapped XCD by XCD
+//                   8 DMs per workgroup sharing each subband's window in
+//                   LDS, output tiles mapped XCD by XCD (k_dd_sum_direct:
+//                   one DM per workgroup, for blocks of many subbands)
 //
 // Both reference kernels read time-major data; the sums are formed in the same
 // order from the same values, so results are bit-identical.
@@ -22,7 +24,11 @@
 
 #define DD_NT 256
 #define DD_TILE 32
-#define DD_XPT 4  // consecutive outputs per thread in k_dd_sum (one 16-byte load)
+#define DD_XPT 4      // outputs per thread in k_dd_sum, DD_NT apart
+#define DD_DG 8       // DMs per k_dd_sum workgroup
+#define DD_SPREAD 256  // widest offset spread of a group's subband served from LDS
+#define DD_NSUB 256    // most subbands whose offsets k_dd_sum keeps in LDS
+#define DD_XPD 4       // consecutive outputs per thread in k_dd_sum_direct
 
 namespace {
 
@@ -60,15 +66,12 @@ __global__ __launch_bounds__(DD_TILE * 8) void k_dd_transpose(float *__restrict_
     }
 }
 
-// out[dm][x] for DD_NT * DD_XPT consecutive x of one DM per workgroup.  The
-// block pair's subband-major window (nsub x 2 worklen floats) is larger than
-// one XCD's L2, and workgroups are dispatched to the 8 XCDs in turn: the
-// 1-D grid is mapped so that each XCD takes a contiguous range of output
-// tiles for every DM, and its part of the window stays in its L2.  Subbands
-// are summed in ascending order (the reference's order and values: the same
-// bits).  An offset outside [0, worklen] sets *err and that DM's outputs are
-// left unwritten.
-__global__ __launch_bounds__(DD_NT) void k_dd_sum(float *__restrict__ out, long long out_stride,
+// out[dm][x] for DD_NT * DD_XPD consecutive x of one DM per workgroup, for
+// blocks of more than DD_NSUB subbands: every DM reads its subbands' windows
+// directly (one 16-byte load per thread and subband), the 1-D grid mapped XCD
+// by XCD as below.  Subbands in ascending order; a bad offset sets *err and
+// that DM's outputs are left unwritten.
+__global__ __launch_bounds__(DD_NT) void k_dd_sum_direct(float *__restrict__ out, long long out_stride,
                                                   const float *__restrict__ S, const int *__restrict__ offsets,
                                                   int worklen, int nsub, int ntiles, int numdms,
                                                   int *__restrict__ err)
@@ -84,15 +87,15 @@ __global__ __launch_bounds__(DD_NT) void k_dd_sum(float *__restrict__ out, long 
         dm = w / ntiles;
     }
     if (dm >= numdms) return;
-    // DD_XPT consecutive outputs per thread, read with 16-byte loads per
+    // DD_XPD consecutive outputs per thread, read with 16-byte loads per
     // subband (4-byte aligned: the offsets are arbitrary)
-    const int x0 = tile * DD_NT * DD_XPT + threadIdx.x * DD_XPT;
-    const bool full = x0 + DD_XPT <= worklen;
+    const int x0 = tile * DD_NT * DD_XPD + threadIdx.x * DD_XPD;
+    const bool full = x0 + DD_XPD <= worklen;
     const int *off = offsets + (long long)dm * nsub;
     const long long w2 = 2LL * worklen;
-    float v[DD_XPT];
+    float v[DD_XPD];
 #pragma unroll
-    for (int k = 0; k < DD_XPT; k++) v[k] = 0.0f;
+    for (int k = 0; k < DD_XPD; k++) v[k] = 0.0f;
     bool bad = false;
     for (int s = 0; s < nsub; s++) {
         const int o = off[s];
@@ -100,7 +103,7 @@ __global__ __launch_bounds__(DD_NT) void k_dd_sum(float *__restrict__ out, long 
         const float *row = S + s * w2 + (bad ? 0 : o) + x0;
         if (full) {
 #pragma unroll
-            for (int k = 0; k < DD_XPT; k += 4) {
+            for (int k = 0; k < DD_XPD; k += 4) {
                 float4 q;
                 __builtin_memcpy(&q, row + k, sizeof(q));
                 v[k] += q.x;
@@ -110,7 +113,7 @@ __global__ __launch_bounds__(DD_NT) void k_dd_sum(float *__restrict__ out, long 
             }
         } else {
 #pragma unroll
-            for (int k = 0; k < DD_XPT; k++)
+            for (int k = 0; k < DD_XPD; k++)
                 if (x0 + k < worklen) v[k] += row[k];
         }
     }
@@ -119,8 +122,144 @@ __global__ __launch_bounds__(DD_NT) void k_dd_sum(float *__restrict__ out, long 
         return;
     }
 #pragma unroll
-    for (int k = 0; k < DD_XPT; k++)
+    for (int k = 0; k < DD_XPD; k++)
         if (x0 + k < worklen) out[(long long)dm * out_stride + x0 + k] = v[k];
+}
+
+// out[dm][x] for DD_DG consecutive DMs and DD_NT * DD_XPT outputs per
+// workgroup (thread t takes x0 + k DD_NT + t, k < DD_XPT: consecutive lanes,
+// consecutive words).  Neighbouring DMs' offsets differ little, so per subband
+// the union of the group's windows, S[s][x0 + lo .. x0 + X + hi) with lo / hi
+// the group's smallest / largest offset, is read from global memory once into
+// LDS (two buffers alternate: one barrier per subband) and every DM of the
+// group reads its own shifted part of it; the next subband's window is read
+// into registers while one is summed.  A subband whose offsets spread wider
+// than DD_SPREAD is read from global memory directly.  The block pair's
+// window is larger than one XCD's L2, and workgroups are dispatched to the 8
+// XCDs in turn: the 1-D grid is mapped so that each XCD takes a contiguous
+// range of output tiles for every DM group, and its part of the window stays
+// in its L2.  Subbands are summed in ascending order (the reference's order
+// and values: the same bits).  An offset outside [0, worklen] sets *err and
+// that DM's outputs are left unwritten.
+__global__ __launch_bounds__(DD_NT) void k_dd_sum(float *__restrict__ out, long long out_stride,
+                                                  const float *__restrict__ S, const int *__restrict__ offsets,
+                                                  int worklen, int nsub, int ntiles, int numdms,
+                                                  int *__restrict__ err)
+{
+    constexpr int X = DD_NT * DD_XPT, WIN = X + DD_SPREAD;
+    __shared__ float win[2][WIN];
+    const int w = blockIdx.x, nb = gridDim.x, tid = threadIdx.x;
+    int tile, grp;
+    if (ntiles % 8 == 0 && nb % 8 == 0) {
+        const int per = ntiles / 8, j = w / 8;  // the j-th workgroup of XCD w % 8
+        tile = (w % 8) * per + j % per;
+        grp = j / per;
+    } else {
+        tile = w % ntiles;
+        grp = w / ntiles;
+    }
+    const int dm0 = grp * DD_DG;
+    if (dm0 >= numdms || nsub > DD_NSUB) return;  // the launch sends larger blocks to k_dd_sum_direct
+    const int ndm = min(DD_DG, numdms - dm0);
+    const int x0 = tile * X;
+    const long long w2 = 2LL * worklen;
+    float v[DD_DG][DD_XPT];
+#pragma unroll
+    for (int d = 0; d < DD_DG; d++)
+#pragma unroll
+        for (int k = 0; k < DD_XPT; k++) v[d][k] = 0.0f;
+    // the group's offsets (a bad one as 0) and each subband's smallest and
+    // largest, in LDS
+    __shared__ int soff[DD_DG][DD_NSUB], slo[DD_NSUB], shi[DD_NSUB];
+    __shared__ unsigned sbad;
+    if (tid == 0) sbad = 0;
+    __syncthreads();
+    for (int i = tid; i < DD_DG * nsub; i += DD_NT) {
+        const int d = i / nsub, s = i - d * nsub;
+        int od = d < ndm ? offsets[(long long)(dm0 + d) * nsub + s] : 0;
+        if (od < 0 || od > worklen) {
+            atomicOr(&sbad, 1u << d);
+            od = 0;
+        }
+        soff[d][s] = od;
+    }
+    __syncthreads();
+    for (int s = tid; s < nsub; s += DD_NT) {
+        int lo = worklen, hi = 0;
+        for (int d = 0; d < ndm; d++) {
+            lo = min(lo, soff[d][s]);
+            hi = max(hi, soff[d][s]);
+        }
+        slo[s] = lo;
+        shi[s] = hi;
+    }
+    __syncthreads();
+    const unsigned bad = sbad;  // bit d: DM dm0 + d has an offset outside [0, worklen]
+    // subband s + 1's window (LDS path) is fetched into registers while
+    // subband s is summed: the global loads' latency overlaps the sums
+    constexpr int NPRE = WIN / DD_NT;
+    float pre[NPRE];
+    int lon = 0, hin = 0;
+    auto fetch = [&](int s) {
+        lon = slo[s];
+        hin = shi[s];
+        if (hin - lon <= DD_SPREAD) {
+            // positions x0 + lo + j, j < X + hi - lo (0 beyond the row's end,
+            // which no written output reads)
+            const int span = X + hin - lon;
+            const long long base = (long long)x0 + lon;
+            const float *row = S + s * w2;
+#pragma unroll
+            for (int i = 0; i < NPRE; i++) {
+                const int j = tid + i * DD_NT;
+                pre[i] = (j < span && base + j < w2) ? row[base + j] : 0.0f;
+            }
+        }
+    };
+    fetch(0);
+    int buf = 0;
+    for (int s = 0; s < nsub; s++) {
+        const int lo = lon;
+        const bool shared = hin - lon <= DD_SPREAD;
+        if (shared) {
+#pragma unroll
+            for (int i = 0; i < NPRE; i++) win[buf][tid + i * DD_NT] = pre[i];
+            __syncthreads();
+        }
+        if (s + 1 < nsub) fetch(s + 1);
+        if (shared) {
+#pragma unroll
+            for (int d = 0; d < DD_DG; d++)
+                if (d < ndm) {
+                    const float *wd = &win[buf][tid + soff[d][s] - lo];
+#pragma unroll
+                    for (int k = 0; k < DD_XPT; k++) v[d][k] += wd[k * DD_NT];
+                }
+            buf ^= 1;
+        } else {
+            const float *row = S + s * w2;
+#pragma unroll
+            for (int d = 0; d < DD_DG; d++)
+                if (d < ndm) {
+                    const int od = soff[d][s];
+#pragma unroll
+                    for (int k = 0; k < DD_XPT; k++) {
+                        const int x = x0 + k * DD_NT + tid;
+                        if (x < worklen) v[d][k] += row[od + x];
+                    }
+                }
+        }
+    }
+    if (bad && tid == 0) atomicOr(err, 1);
+#pragma unroll
+    for (int d = 0; d < DD_DG; d++) {
+        if (d >= ndm || (bad >> d) & 1u) continue;
+#pragma unroll
+        for (int k = 0; k < DD_XPT; k++) {
+            const int x = x0 + k * DD_NT + tid;
+            if (x < worklen) out[(long long)(dm0 + d) * out_stride + x] = v[d][k];
+        }
+    }
 }
 
 struct DdWork {
@@ -170,8 +309,15 @@ static int dedisp_enqueue(DdWork *w, const float *cur, const float *last, float 
                        dim3(DD_TILE * 8), 0, nullptr, w->S, last, cur, worklen, nsub);
     PZL_HIPCHK(hipGetLastError());
     const int ntiles = (worklen + DD_NT * DD_XPT - 1) / (DD_NT * DD_XPT);
-    hipLaunchKernelGGL(k_dd_sum, dim3((unsigned)ntiles * numdms), dim3(DD_NT), 0, nullptr, out, out_stride, w->S,
-                       offsets, worklen, nsub, ntiles, numdms, w->err);
+    const int ngroups = (numdms + DD_DG - 1) / DD_DG;
+    if (nsub <= DD_NSUB) {
+        hipLaunchKernelGGL(k_dd_sum, dim3((unsigned)ntiles * ngroups), dim3(DD_NT), 0, nullptr, out, out_stride,
+                           w->S, offsets, worklen, nsub, ntiles, numdms, w->err);
+    } else {
+        const int nt = (worklen + DD_NT * DD_XPD - 1) / (DD_NT * DD_XPD);
+        hipLaunchKernelGGL(k_dd_sum_direct, dim3((unsigned)nt * numdms), dim3(DD_NT), 0, nullptr, out, out_stride,
+                           w->S, offsets, worklen, nsub, nt, numdms, w->err);
+    }
     PZL_HIPCHK(hipGetLastError());
     return 0;
 }

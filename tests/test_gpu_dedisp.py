@@ -31,7 +31,7 @@ def test_downsamp_bit_identical(bs):
     assert np.array_equal(got, O.downsamp(x, bs))
 
 
-@pytest.mark.parametrize("worklen,nsub,numdms", [(1000, 16, 7), (4096, 64, 33), (2500, 96, 5)])
+@pytest.mark.parametrize("worklen,nsub,numdms", [(1000, 16, 7), (4096, 64, 33), (2500, 96, 5), (600, 300, 5)])
 def test_dedisp_bit_identical(worklen, nsub, numdms):
     from prestozl_amd import binding as B
     rng = np.random.default_rng(worklen + nsub)
@@ -48,6 +48,26 @@ def test_dedisp_bit_identical(worklen, nsub, numdms):
     B.get_dedisp(dc.ptr, dl.ptr, dout.ptr, worklen, nsub, numdms, do.ptr)
     got = dout.download((numdms, worklen), np.float32)
     assert np.array_equal(got, O.dedisp(cur, last, offs))
+
+
+@pytest.mark.parametrize("worklen,nsub,numdms", [(3000, 24, 19), (1024, 8, 8)])
+def test_dedisp_scattered_offsets_bit_identical(worklen, nsub, numdms):
+    """Unsorted offsets over the whole [0, worklen]: DM groups whose offsets in
+    a subband spread wider than the shared window read that subband directly,
+    others share it; both give the oracle's sums."""
+    from prestozl_amd import binding as B
+    rng = np.random.default_rng(worklen + 7)
+    last = rng.standard_normal((worklen, nsub)).astype(np.float32)
+    cur = rng.standard_normal((worklen, nsub)).astype(np.float32)
+    offs = rng.integers(0, worklen + 1, (numdms, nsub)).astype(np.int32)
+    offs[:, ::3] = rng.integers(0, 40, (numdms, len(range(0, nsub, 3))))  # narrow subbands too
+    dl, dc = B.DeviceArray(last.nbytes), B.DeviceArray(cur.nbytes)
+    do, dout = B.DeviceArray(offs.nbytes), B.DeviceArray(numdms * worklen * 4)
+    dl.upload(last)
+    dc.upload(cur)
+    do.upload(offs)
+    B.get_dedisp(dc.ptr, dl.ptr, dout.ptr, worklen, nsub, numdms, do.ptr)
+    assert np.array_equal(dout.download((numdms, worklen), np.float32), O.dedisp(cur, last, offs))
 
 
 def test_dedisp_strided_rows_and_bad_offsets():
