@@ -395,3 +395,26 @@ def test_driver_noharmpolish_outputs(tmp_path):
                 e = O.calc_props(d, p["r"], p["z"], p["w"])
                 for k in ("pow", "rawpow", "sig", "locpow"):
                     assert p[k] == pytest.approx(getattr(e, k), rel=1e-4), k
+
+
+@pytest.mark.gpu
+def test_driver_noharmpolish_fdot_only(tmp_path, config1_data):
+    """-noharmpolish in a w = 0 search (config 1): max_rz_arr per harmonic
+    (maximize_rz.c:25-85).  Each .cand record holds the fundamental's
+    properties at its own simplex optimum, as the oracle's max_rz_arr gives
+    them (the simplexes agree to the flatness of the peak: 5e-3 in power)."""
+    fft, N, dt, opts = config1_data
+    root = D.setup_files(tmp_path, fft, N, dt, name="nhp0")
+    r = subprocess.run([DRIVER, "-zmax", "20", "-wmax", "0", "-numharm", "4", "-sigma", "2.0", "-noharmpolish",
+                        root + ".fft"], capture_output=True, text=True, timeout=300, cwd=str(tmp_path))
+    print(r.stdout[-1500:])
+    assert r.returncode == 0, r.stdout + r.stderr
+    base = root + "_ACCEL_20"
+    props = np.fromfile(base + ".cand", dtype=FOURIERPROPS)
+    fund, harm = _accel_tables(base)[:2]
+    numharm = [int(row.split()[4]) for row in fund]
+    assert len(props) == len(fund) > 0 and len(harm) == sum(numharm)
+    for p in props[:12]:
+        ro, zo, d, pw = O.max_rz_arr(fft, p["r"], p["z"])
+        q = O.calc_props(d, p["r"], p["z"], 0.0)
+        assert p["pow"] == pytest.approx(q.pow, rel=5e-3) and p["rawpow"] == pytest.approx(q.rawpow, rel=5e-3)
