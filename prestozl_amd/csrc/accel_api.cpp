@@ -906,26 +906,24 @@ void accelsearch_CPU2(GSList **cands, accelobs *obs, infodata *idata, Cmdline *c
         *cands = sort_accelcands(*cands);
         if (cmd->numharm > 1 && !cmd->noharmremoveP) *cands = eliminate_harmonics(*cands, &numcands);
         if (pzl_optimize_accelcands(*cands, obs) != 0) die(1, "\nThe spectrum is not in memory\n\n");
-        {
-            // the fundamentals' properties, with the search-grid errors
-            // (accelsearch_lib.c:932-948), then the text and binary outputs
-            std::vector<fourierprops> props((size_t)numcands);
-            int ii = 0;
-            for (GSList *p = *cands; p; p = p->next, ii++) {
-                const accelcand *c = (const accelcand *)p->data;
-                calc_props(c->derivs[0], c->r, c->z, c->w, &props[ii]);
-                props[ii].rerr = (float)(PZL_DR) / c->numharm;
-                props[ii].zerr = (float)(PZL_DZ) / c->numharm;
-                props[ii].werr = (float)(PZL_DW) / c->numharm;
-            }
-            output_fundamentals(props.data(), *cands, obs, idata);
-            output_harmonics(*cands, obs, idata);
-            FILE *f = fopen(obs->candnm, "wb");
-            if (!f || fwrite(props.data(), sizeof(fourierprops), (size_t)numcands, f) != (size_t)numcands)
-                die(1, "\nCannot write '%s'\n\n", obs->candnm);
-            fclose(f);
-            printf("\n\n");
+        // the fundamentals' properties, with the search-grid errors
+        // (accelsearch_lib.c:932-948), then the text and binary outputs
+        std::vector<fourierprops> props((size_t)numcands);
+        int ii = 0;
+        for (GSList *p = *cands; p; p = p->next, ii++) {
+            const accelcand *c = (const accelcand *)p->data;
+            calc_props(c->derivs[0], c->r, c->z, c->w, &props[ii]);
+            props[ii].rerr = (float)(PZL_DR) / c->numharm;
+            props[ii].zerr = (float)(PZL_DZ) / c->numharm;
+            props[ii].werr = (float)(PZL_DW) / c->numharm;
         }
+        output_fundamentals(props.data(), *cands, obs, idata);
+        output_harmonics(*cands, obs, idata);
+        FILE *f = fopen(obs->candnm, "wb");
+        if (!f || fwrite(props.data(), sizeof(fourierprops), (size_t)numcands, f) != (size_t)numcands)
+            die(1, "\nCannot write '%s'\n\n", obs->candnm);
+        fclose(f);
+        printf("\n\n");
     } else {
         printf("No candidates above sigma = %.2f were found.\n\n", obs->sigma);
     }
