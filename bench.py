@@ -422,7 +422,8 @@ def main():
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each rstep and each file is "
-              "independent; see DESIGN.md for how the search shards).", file=sys.stderr)
+              "independent: one process per GPU, each searching its own files or rstep range, would scale "
+              "the search).", file=sys.stderr)
         sys.exit(2)
 
     from prestozl_amd import binding as B
