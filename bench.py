@@ -25,6 +25,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3        # MI355X published FP32 (vector, packed) peak
+FP32_NONPACKED_TFLOPS = 78.6    # 256 CUs x 64 lanes x 2 (FMA) x 2.4 GHz without packed FP32
 
 
 def hip_sync():
@@ -34,18 +36,44 @@ def hip_sync():
         pass
 
 
-def cpu_baseline(fft, kw, rsteps, nthreads, cells_per_rstep, start):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(fft, kw, rsteps, nthreads, cells_per_rstep, start, rsteps_1core=16):
+    """The oracle (PRESTO C's per-rstep loop restated, FP32) on a band of
+    rsteps from the middle of the search: on 'nthreads' host threads (OpenMP
+    over kernel rows, as -ncpus does) and on one thread."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as O
     obs = O.create_accelobs(O.make_params(**kw))
-    ctx = O.Ctx(obs, O.FP32, nthreads=nthreads)
-    t0 = time.perf_counter()
-    ctx.search(fft, start, start + rsteps, maxhits=1 << 16)
-    dt = time.perf_counter() - t0
-    ctx.close()
-    return {"value": cells_per_rstep * rsteps / dt, "unit": "cells/s", "cores": nthreads, "kind": "port",
-            "sample": "oracle FP32 (PRESTO C restated, OpenMP over kernel rows) on rsteps [%d, %d) of the same "
-                      "spectrum (%.1f s), kernel-bank build excluded as on the GPU" % (start, start + rsteps, dt)}
+
+    def run(nt, n):
+        ctx = O.Ctx(obs, O.FP32, nthreads=nt)
+        t0 = time.perf_counter()
+        ctx.search(fft, start, start + n, maxhits=1 << 16)
+        dt = time.perf_counter() - t0
+        ctx.close()
+        return dt
+
+    dt = run(nthreads, rsteps)
+    dt1 = run(1, rsteps_1core) if rsteps_1core else None
+    out = {"value": cells_per_rstep * rsteps / dt, "unit": "cells/s", "cores": nthreads, "kind": "port",
+           "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
+           "sample": "oracle FP32 (PRESTO C restated, OpenMP over kernel rows) on rsteps [%d, %d) of the same "
+                     "spectrum (%.1f s), kernel-bank build excluded as on the GPU" % (start, start + rsteps, dt)}
+    if dt1:
+        out["single_core"] = {"value": cells_per_rstep * rsteps_1core / dt1, "unit": "cells/s", "cores": 1,
+                              "sample": "rsteps [%d, %d) on one thread (%.1f s)" % (start, start + rsteps_1core,
+                                                                                     dt1)}
+    return out
 
 
 def cpu_optimize_sample(fft, cands, ncand=8):
@@ -414,6 +442,9 @@ def main():
                     help="run only the dedispersion -> realfft -> search leg and print its line")
     ap.add_argument("--dedisp-no-serve", action="store_true",
                     help="with --dedisp-only: leave out the request-loop search of every trial")
+    ap.add_argument("--overlap", type=int, default=-1,
+                    help="profiling runs: 0 = one stream, 1 = harmonic sums on a side stream, 2 = also the "
+                         "shorter FFT lengths (the default)")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
                     help="run a plain device copy of MIB MiB five times and exit (HBM counter calibration)")
     args = ap.parse_args()
@@ -450,6 +481,8 @@ def main():
         lo = g.nrsteps // 2
         hi = lo + args.rsteps
     tune = [] if args.no_tune else plan.autotune(reps=3)
+    if args.overlap >= 0:
+        plan.set_overlap(args.overlap)
     for _ in range(args.warmup):
         plan.search(lo, hi, stats=False)
     hip_sync()
@@ -467,15 +500,23 @@ def main():
     cls = {}
     for st in stats_all:
         for c in st["classes"]:
-            d = cls.setdefault(c["fftlen"], dict(ms=0.0, launches=0, bytes=0.0))
+            d = cls.setdefault(c["fftlen"], dict(ms=0.0, launches=0, bytes=0.0, flops=0.0, bank=0.0))
             d["ms"] += c["ms"]
             d["launches"] += c["launches"]
             d["bytes"] += c["bytes"]
+            d["flops"] += c["flops"]
+            d["bank"] = c["bank_bytes"]
     dom_len = max(cls, key=lambda k: cls[k]["ms"])
     dom = cls[dom_len]
     avg_ms = dom["ms"] / dom["launches"]
+    # algorithmic bytes per launch: the plane rows written and the segments read
+    # by one launch, plus the class's kernel rows amortised over the search's
+    # launches (the bank is streamed once per search)
     bytes_per_launch = dom["bytes"] / dom["launches"]
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    flops_per_launch = dom["flops"] / dom["launches"]
+    tflops = flops_per_launch / (avg_ms * 1e-3) / 1e12
+    traffic = measured_traffic("k_corr<%d>" % dom_len)
     out = {
         "metric": "f-fdot-fdotdot cells/s at zmax200/wmax300/16 harm; p50/p99 latency per search",
         "value": value,
@@ -498,8 +539,18 @@ def main():
                        "samples": len(lat)},
         "roofline": {"bound": "hbm", "kernel": "k_corr<%d>" % dom_len, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic("k_corr<%d>" % dom_len),
-                     "avg_launch_ms": avg_ms, "alg_bytes_per_launch": bytes_per_launch},
+                     "traffic": traffic,
+                     "avg_launch_ms": avg_ms, "alg_bytes_per_launch": bytes_per_launch,
+                     "bank_bytes_reread_per_launch": dom["bank"],
+                     "traffic_over_alg": traffic / bytes_per_launch if traffic else None,
+                     "binding": "FP32 vector issue (VALU), not HBM: see compute",
+                     "compute": {"flops_per_launch": flops_per_launch, "achieved": tflops, "unit": "TFLOP/s",
+                                 "peak": FP32_PEAK_TFLOPS, "frac": tflops / FP32_PEAK_TFLOPS,
+                                 "peak_nonpacked": FP32_NONPACKED_TFLOPS,
+                                 "frac_nonpacked": tflops / FP32_NONPACKED_TFLOPS,
+                                 "note": "nominal 5 N log2 N per inverse FFT + 6 N per product + 3 per power; "
+                                         "packed FP32 is disabled in this build, so one FP32 FMA per lane and "
+                                         "cycle (78.6 TFLOP/s) is the issue ceiling"}},
         "search_roofline": {"bytes_alg_per_search": stats_all[0]["bytes_alg"],
                             "achieved_GBs": stats_all[0]["bytes_alg"] * args.steps / total / 1e9,
                             "frac": stats_all[0]["bytes_alg"] * args.steps / total / 1e9 / HBM_PEAK_GBS},

@@ -28,6 +28,7 @@ extern "C" {
 #define PZL_ERR_HIP (-2)
 #define PZL_ERR_UNSUPPORTED (-3)
 #define PZL_ERR_NOSPECTRUM (-4)
+#define PZL_ERR_TIMEOUT (-5)
 
 /* Search request: the fields of Cmdline (accelsearch_cmd.h:11-87) and
  * infodata (makeinf.h:23-56) that create_accelobs (accel_utils.c:1636-2037)
@@ -86,9 +87,15 @@ typedef struct PzlStats {
     int class_fftlen[16];
     double class_ms[16];          /* summed duration of that class's launches */
     long long class_launches[16];
-    double class_bytes[16];       /* algorithmic bytes of that class's launches */
+    double class_bytes[16];       /* algorithmic bytes of that class's launches: plane rows
+                                     written and segments read per rstep, the class's kernel
+                                     rows once per search */
     double ms_graph;              /* summed duration of graph-replayed batches */
     long long graph_launches;
+    double class_bank_bytes[16];  /* the class's kernel rows (read once per launch) */
+    double class_flops[16];       /* nominal FP32 operations of the class's launches:
+                                     5 N log2 N per inverse FFT, 6 N per product with
+                                     conj(kernel), 3 per stored power */
 } PzlStats;
 
 const char *pzl_last_error(void);
@@ -131,6 +138,11 @@ int pzl_set_spectrum_device(void *plan, const void *dev_fft, long long numbins);
  * (accel_utils.c:382-480).  Writes min(ncands, maxcands) candidates. */
 int pzl_search(void *plan, long long rstep_lo, long long rstep_hi, PzlCand *cands, long long maxcands,
                long long *ncands, PzlStats *stats);
+/* A deadline for the plan's searches: once 'ms' milliseconds have passed
+ * since this call, pzl_search launches no further batch, waits for the ones
+ * in flight and returns PZL_ERR_TIMEOUT (its candidate list is then partial
+ * and not returned).  ms <= 0 clears the deadline. */
+int pzl_plan_set_deadline(void *plan, double ms);
 
 /* Autotuner: times every correlation-kernel variant (FFT radices, threads and
  * kernel rows per workgroup) of each FFT length of the plan on one batch of
@@ -171,6 +183,9 @@ int pzl_get_cands(void *plan, PzlCand *cands, long long maxcands, long long *nca
 /* Hits of the last pzl_search in PRESTO C scan order: key encodes
  * (((rstep*stages + stage)*numw + w)*numz + z)*corr_uselen + r. */
 int pzl_get_hits(void *plan, long long *keys, float *powers, long long maxhits, long long *nhits);
+/* How many hits a search keeps for pzl_get_hits (default 2^22; a search with
+ * more keeps none and pzl_get_hits fails). */
+int pzl_plan_set_keep_hits(void *plan, long long limit);
 
 /* Parity hooks: one FFT'd kernel (fftlen complex64) of harmonic idx, and the
  * power plane ([numw][numz][numrs]) of (rstep, harmonic) as
@@ -257,6 +272,8 @@ typedef struct PzlServerStats {
     long long budget, admitted_bytes, peak_admitted_bytes, resident_bytes;
     long long queued, completed, pooled_plans;
     int inflight, peak_inflight;
+    long long peak_resident_bytes;  /* pooled plans (busy or idle) plus plans being created */
+    long long timed_out;            /* requests failed by their deadline */
 } PzlServerStats;
 
 /* A request is admitted by its footprint: the geometry's device_bytes plus
@@ -274,6 +291,15 @@ int pzl_server_submit(void *server, const PzlConfig *cfg, const float *host_fft,
  * caller's and must stay valid until the request is done. */
 int pzl_server_submit_device(void *server, const PzlConfig *cfg, const void *dev_fft, long long numbins,
                              long long *req_id);
+/* The same with a deadline: 'deadline_ms' after submission the request
+ * fails with PZL_ERR_TIMEOUT -- still queued, it is dropped; running, its
+ * search stops launching batches, drains the ones in flight and returns --
+ * and its slot and admitted bytes are freed for the next request.  The other
+ * requests are not affected.  deadline_ms <= 0: none.  The reference
+ * pipeline's per-thread timeout and task re-queue
+ * (bin/accelsearch_pipeline_cu.py:440-490) leave the retry to the caller here. */
+int pzl_server_submit_deadline(void *server, const PzlConfig *cfg, const void *fft, int on_device,
+                               long long numbins, double deadline_ms, long long *req_id);
 /* Block until the request is done; copy its candidates and timing and forget it. */
 int pzl_server_wait(void *server, long long req_id, PzlCand *cands, long long maxcands, long long *ncands,
                     PzlRequestStats *stats);

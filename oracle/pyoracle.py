@@ -320,7 +320,12 @@ class Ctx:
         pows = pows[: h.numw * h.numz * numrs].reshape(h.numw, h.numz, numrs)
         return pows, rinds, zinds, meta
 
-    def search(self, fft, rstep_lo=0, rstep_hi=-1, maxcands=1 << 16, maxhits=1 << 20, sum_rstep=-1):
+    def search(self, fft, rstep_lo=0, rstep_hi=-1, maxcands=1 << 16, maxhits=1 << 20, sum_rstep=-1,
+               hits_array=False):
+        """oracle_search over rsteps [rstep_lo, rstep_hi): (candidates, hits,
+        summed plane of sum_rstep).  hits: a list of (rstep, stage, wi, zi, ri,
+        power, sigma) tuples, or with hits_array a structured numpy array of
+        those fields.  More than maxhits hits raise an error."""
         obs = self.obs
         cands = (Cand * maxcands)()
         hits = (Hit * maxhits)()
@@ -336,8 +341,15 @@ class Ctx:
                                 C.byref(snumrs))
         clist = [(cands[i].power, cands[i].sigma, cands[i].numharm, cands[i].r, cands[i].z, cands[i].w)
                  for i in range(min(n, maxcands))]
-        hlist = [(hits[i].rstep, hits[i].stage, hits[i].wi, hits[i].zi, hits[i].ri, hits[i].power,
-                  hits[i].sigma) for i in range(min(nh.value, maxhits))]
+        if hits_array:
+            if nh.value > maxhits:
+                raise RuntimeError("oracle search: %d hits, above maxhits %d" % (nh.value, maxhits))
+            dt = np.dtype([(name, np.dtype(ct)) for name, ct in Hit._fields_], align=True)
+            assert dt.itemsize == C.sizeof(Hit)
+            hlist = np.frombuffer(hits, dtype=dt, count=nh.value).copy()
+        else:
+            hlist = [(hits[i].rstep, hits[i].stage, hits[i].wi, hits[i].zi, hits[i].ri, hits[i].power,
+                      hits[i].sigma) for i in range(min(nh.value, maxhits))]
         if sums is not None:
             sums = sums.reshape(h.numw, h.numz, obs.corr_uselen)
         return clist, hlist, sums

@@ -12,6 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "lib", "libaccelsearch_lib.so")
 
 PZL_ERR_OVERFLOW = 1
+PZL_ERR_TIMEOUT = -5
 
 
 class Config(C.Structure):
@@ -54,12 +55,14 @@ class Stats(C.Structure):
                 ("nrsteps", C.c_longlong), ("cells", C.c_double), ("bytes_alg", C.c_double),
                 ("bytes_alg_corr", C.c_double), ("nclass", C.c_int), ("class_fftlen", C.c_int * 16),
                 ("class_ms", C.c_double * 16), ("class_launches", C.c_longlong * 16),
-                ("class_bytes", C.c_double * 16), ("ms_graph", C.c_double), ("graph_launches", C.c_longlong)]
+                ("class_bytes", C.c_double * 16), ("ms_graph", C.c_double), ("graph_launches", C.c_longlong),
+                ("class_bank_bytes", C.c_double * 16), ("class_flops", C.c_double * 16)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_ if not k.startswith("class_")}
         d["classes"] = [dict(fftlen=self.class_fftlen[i], ms=self.class_ms[i], launches=self.class_launches[i],
-                             bytes=self.class_bytes[i]) for i in range(self.nclass)]
+                             bytes=self.class_bytes[i], bank_bytes=self.class_bank_bytes[i],
+                             flops=self.class_flops[i]) for i in range(self.nclass)]
         return d
 
 
@@ -89,7 +92,8 @@ class RequestStats(C.Structure):
 class ServerStats(C.Structure):
     _fields_ = [("budget", C.c_longlong), ("admitted_bytes", C.c_longlong), ("peak_admitted_bytes", C.c_longlong),
                 ("resident_bytes", C.c_longlong), ("queued", C.c_longlong), ("completed", C.c_longlong),
-                ("pooled_plans", C.c_longlong), ("inflight", C.c_int), ("peak_inflight", C.c_int)]
+                ("pooled_plans", C.c_longlong), ("inflight", C.c_int), ("peak_inflight", C.c_int),
+                ("peak_resident_bytes", C.c_longlong), ("timed_out", C.c_longlong)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -146,6 +150,10 @@ def lib():
                                       P(RequestStats)]
         L.pzl_server_poll.argtypes = [C.c_void_p, P(C.c_longlong), C.c_int, P(C.c_int)]
         L.pzl_server_stats.argtypes = [C.c_void_p, P(ServerStats)]
+        L.pzl_server_submit_deadline.argtypes = [C.c_void_p, P(Config), C.c_void_p, C.c_int, C.c_longlong,
+                                                 C.c_double, P(C.c_longlong)]
+        L.pzl_plan_set_deadline.argtypes = [C.c_void_p, C.c_double]
+        L.pzl_plan_set_keep_hits.argtypes = [C.c_void_p, C.c_longlong]
         L.pzl_server_destroy.argtypes = [C.c_void_p]
         L.pzl_server_destroy.restype = None
         L.pzl_plan_variant.argtypes = [C.c_void_p, C.c_int, P(C.c_int)]
@@ -351,6 +359,11 @@ class Plan:
         check(lib().pzl_autotune(self.h, int(reps), buf, len(buf)))
         return buf.value.decode().splitlines()
 
+    def set_deadline(self, ms):
+        """Searches stop launching batches 'ms' milliseconds from now and fail
+        with PZL_ERR_TIMEOUT (ms <= 0: no deadline)."""
+        check(lib().pzl_plan_set_deadline(self.h, float(ms)))
+
     def set_graph(self, on=True):
         check(lib().pzl_plan_set_graph(self.h, int(bool(on))))
 
@@ -408,6 +421,10 @@ class Plan:
         out = [(cands[i].power, cands[i].sigma, cands[i].numharm, cands[i].r, cands[i].z, cands[i].w)
                for i in range(min(n.value, maxcands))]
         return out, (st.as_dict() if st is not None else None)
+
+    def set_keep_hits(self, limit):
+        """Hits kept by a search for hits() (default 2^22)."""
+        check(lib().pzl_plan_set_keep_hits(self.h, int(limit)))
 
     def hits(self):
         n = C.c_longlong(0)
@@ -520,21 +537,32 @@ class Server:
         check(lib().pzl_server_create(int(device), int(budget), int(max_inflight), C.byref(self.h)))
         self._keep = {}
 
-    def submit(self, cfg, fft):
+    def submit(self, cfg, fft, deadline_ms=0.0):
+        """Queue a search of a host spectrum; deadline_ms > 0: the request
+        fails with PZL_ERR_TIMEOUT once that long has passed since now."""
         a = np.ascontiguousarray(fft)
         if a.dtype == np.complex64:
             a = a.view(np.float32)
         assert a.dtype == np.float32
         rid = C.c_longlong(0)
-        check(lib().pzl_server_submit(self.h, C.byref(cfg), _fptr(a), len(a) // 2, C.byref(rid)))
+        if deadline_ms > 0:
+            check(lib().pzl_server_submit_deadline(self.h, C.byref(cfg), _fptr(a), 0, len(a) // 2,
+                                                   float(deadline_ms), C.byref(rid)))
+        else:
+            check(lib().pzl_server_submit(self.h, C.byref(cfg), _fptr(a), len(a) // 2, C.byref(rid)))
         self._keep[rid.value] = a  # the spectrum stays valid until the request is done
         return rid.value
 
-    def submit_device(self, cfg, ptr, numbins):
+    def submit_device(self, cfg, ptr, numbins, deadline_ms=0.0):
         """A spectrum of numbins complex bins in device memory at 'ptr' (kept
         valid by the caller until the request is done)."""
         rid = C.c_longlong(0)
-        check(lib().pzl_server_submit_device(self.h, C.byref(cfg), C.c_void_p(ptr), int(numbins), C.byref(rid)))
+        if deadline_ms > 0:
+            check(lib().pzl_server_submit_deadline(self.h, C.byref(cfg), C.c_void_p(ptr), 1, int(numbins),
+                                                   float(deadline_ms), C.byref(rid)))
+        else:
+            check(lib().pzl_server_submit_device(self.h, C.byref(cfg), C.c_void_p(ptr), int(numbins),
+                                                 C.byref(rid)))
         return rid.value
 
     def wait(self, rid, maxcands=1 << 16):

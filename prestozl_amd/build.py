@@ -9,6 +9,7 @@ prestozl_amd/lib/.  Packed FP32 instructions are disabled for the device code
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
@@ -36,18 +37,26 @@ def _run(cmd, cwd=None):
     return r
 
 
-def build_library(out=None):
+def build_library(out=None, jobs=None):
+    """Compiles every source to an object (in parallel), then links the
+    shared library.  Objects go to _build/ at the repository root."""
     os.makedirs(LIBDIR, exist_ok=True)
+    objdir = os.path.join(ROOT, "_build")
+    os.makedirs(objdir, exist_ok=True)
     out = out or os.path.join(LIBDIR, "libaccelsearch_lib.so")
     srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
     have_glib = all(os.path.isdir(d) for d in GLIB_INC)
-    cmd = [HIPCC, "--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-shared",
-           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-           "-Wno-unused-result", "-Wno-unused-value",
-           "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
+    flags = [HIPCC, "--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-Wno-unused-result", "-Wno-unused-value",
+             "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
     if have_glib:
-        cmd += ["-DPZL_HAVE_GLIB=1"] + ["-I" + d for d in GLIB_INC]
-    cmd += srcs + ["-o", out + ".tmp"]
+        flags += ["-DPZL_HAVE_GLIB=1"] + ["-I" + d for d in GLIB_INC]
+    objs = [os.path.join(objdir, os.path.basename(s) + ".o") for s in srcs]
+    jobs = jobs or min(16, os.cpu_count() or 1)
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        list(ex.map(lambda so: _run(flags + ["-c", so[0], "-o", so[1]]), zip(srcs, objs)))
+    cmd = [HIPCC, "--offload-arch=" + ARCH, "-fPIC", "-shared"] + objs + ["-o", out + ".tmp"]
     if have_glib:
         cmd += GLIB_LINK
     _run(cmd)
@@ -67,6 +76,22 @@ def build_driver(lib=None):
             "-L" + LIBDIR, "-l:" + os.path.basename(lib), "-Wl,-rpath,$ORIGIN",
             # the HIP runtime needs a newer libstdc++ than gcc 11 links against;
             # the system's, found at load time, has it
+            "-Wl,--allow-shlib-undefined"] + GLIB_LINK
+    _run(cmd)
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_pipeline(lib=None):
+    """The three-thread CPU1 -> GPU -> CPU2 pipeline driver over the accel.h API."""
+    lib = lib or os.path.join(LIBDIR, "libaccelsearch_lib.so")
+    out = os.path.join(LIBDIR, "accelsearch_pipeline")
+    if not all(os.path.isdir(d) for d in GLIB_INC):
+        print("glib-2.0 headers not found: accelsearch_pipeline not built", flush=True)
+        return None
+    cmd = ["gcc", "-O2", "-pthread", "-I" + os.path.join(ROOT, "include")] + ["-I" + d for d in GLIB_INC]
+    cmd += [os.path.join(CSRC, "accelsearch_pipeline_main.c"), "-o", out + ".tmp",
+            "-L" + LIBDIR, "-l:" + os.path.basename(lib), "-Wl,-rpath,$ORIGIN",
             "-Wl,--allow-shlib-undefined"] + GLIB_LINK
     _run(cmd)
     os.replace(out + ".tmp", out)
@@ -94,6 +119,7 @@ def build_oracle():
 def build_all():
     build_library()
     build_driver()
+    build_pipeline()
     build_realfft()
     build_oracle()
 

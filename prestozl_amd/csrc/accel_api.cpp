@@ -535,14 +535,9 @@ void create_accelobs(accelobs *obs, infodata *idata, Cmdline *cmd, int usemmap)
 
 void free_accelobs(accelobs *obs)
 {
-    if (obs->mmap_file) {
-        munmap(obs->fft, sizeof(fcomplex) * obs->numbins);
-        close(obs->mmap_file);
-    } else {
-        free(obs->fft);
-        if (obs->fftfile) fclose(obs->fftfile);
-    }
-    if (obs->workfile) fclose(obs->workfile);
+    // The plane API's plans are keyed by the spectrum's address: drop this
+    // spectrum's plans before its memory is released, so that another file
+    // mapped or allocated at the same address later cannot find them.
     {
         std::lock_guard<std::mutex> lk(g_api_mu);
         for (auto it = g_api_plans.begin(); it != g_api_plans.end();) {
@@ -554,6 +549,14 @@ void free_accelobs(accelobs *obs)
             }
         }
     }
+    if (obs->mmap_file) {
+        munmap(obs->fft, sizeof(fcomplex) * obs->numbins);
+        close(obs->mmap_file);
+    } else {
+        free(obs->fft);
+        if (obs->fftfile) fclose(obs->fftfile);
+    }
+    if (obs->workfile) fclose(obs->workfile);
     free(obs->powcut);
     free(obs->numindep);
     free(obs->rootfilenm);

@@ -269,10 +269,12 @@ void calc_props(rderivs data, double r, double z, double w, fourierprops *result
 // (accel_utils.c:584-608).  With -noharmpolish: every harmonic of every
 // candidate on its own, max_rzw_arr or max_rz_arr at (r, z, w) * (ii + 1)
 // (accel_utils.c:610-652).  Returns 0, or -1 when the spectrum is not in
-// memory (the reference then reads the file, max_rzw_file).
+// memory (the reference then reads the file, max_rzw_file).  create_accelobs
+// holds the whole spectrum in obs->fft when it maps the file, reads a .dat,
+// and also when it is asked not to map (it then reads the file whole).
 int pzl_optimize_accelcands(GSList *list, accelobs *obs)
 {
-    if (!(obs->mmap_file || obs->dat_input)) return -1;
+    if (!(obs->mmap_file || obs->dat_input || obs->fft)) return -1;
     const bool polish = obs->use_harmonic_polishing != 0;
     std::vector<PzlOptCand> oc;
     std::vector<accelcand *> cs;

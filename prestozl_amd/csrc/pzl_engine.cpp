@@ -233,6 +233,49 @@ int tuned_variant(const Plan &p, const SizeClass &c)
     return it == g_tuned.end() ? 0 : it->second;
 }
 
+// Every pinned host allocation of the engine, so that each asynchronous copy
+// into or out of pinned memory can be checked to lie inside a live
+// allocation before it is enqueued (a copy into freed pinned memory would
+// otherwise be turned by the runtime into a host write to unmapped pages).
+std::mutex g_pinreg_mu;
+std::map<uintptr_t, size_t> g_pinreg;
+
+hipError_t pinned_alloc(void **ptr, size_t bytes)
+{
+    const hipError_t e = hipHostMalloc(ptr, bytes, hipHostMallocDefault);
+    if (e == hipSuccess) {
+        std::lock_guard<std::mutex> lk(g_pinreg_mu);
+        g_pinreg[(uintptr_t)*ptr] = bytes;
+    }
+    return e;
+}
+
+void pinned_free(void *ptr)
+{
+    if (!ptr) return;
+    {
+        std::lock_guard<std::mutex> lk(g_pinreg_mu);
+        g_pinreg.erase((uintptr_t)ptr);
+    }
+    (void)hipHostFree(ptr);
+}
+
+bool pinned_range_ok(const void *ptr, size_t bytes)
+{
+    std::lock_guard<std::mutex> lk(g_pinreg_mu);
+    auto it = g_pinreg.upper_bound((uintptr_t)ptr);
+    if (it == g_pinreg.begin()) return false;
+    --it;
+    return (uintptr_t)ptr + bytes <= it->first + it->second;
+}
+
+#define PZL_PINNED_CHECK(ptr, bytes)                                                                        \
+    do {                                                                                                   \
+        if (!pinned_range_ok((ptr), (bytes)))                                                              \
+            return fail(PZL_ERR_HIP, "copy of %zu bytes at %p is outside every live pinned allocation (%s:%d)", \
+                        (size_t)(bytes), (const void *)(ptr), __FILE__, __LINE__);                         \
+    } while (0)
+
 // Pinned hit-record buffers outlive their plans: pinning host memory is slow
 // (on the order of 0.15 s per GiB) and waits for the device, so a plan takes
 // buffers from this cache before pinning new memory, and a destroyed plan
@@ -268,7 +311,7 @@ void pin_put(PzlHostHit *ptr, long long cap)
             return;
         }
     }
-    (void)hipHostFree(ptr);
+    pinned_free(ptr);
 }
 
 void free_plan(Plan *p)
@@ -292,14 +335,14 @@ void free_plan(Plan *p)
         if (p->ev_count[s]) (void)hipEventDestroy(p->ev_count[s]);
         if (p->ev_free[s]) (void)hipEventDestroy(p->ev_free[s]);
     }
-    if (p->h_counts) (void)hipHostFree(p->h_counts);
+    pinned_free(p->h_counts);
     for (auto &hb : p->hpool) {
         pin_put(hb.ptr, hb.cap);
         if (hb.ev) (void)hipEventDestroy(hb.ev);
     }
     for (int k = 0; k < Plan::kSets; k++)
         if (p->gexec[k]) (void)hipGraphExecDestroy(p->gexec[k]);
-    if (p->h_r0) (void)hipHostFree(p->h_r0);
+    pinned_free(p->h_r0);
     (void)hipFree(p->d_sums);
     if (p->st) (void)hipStreamDestroy(p->st);
     if (p->st2) (void)hipStreamDestroy(p->st2);
@@ -510,6 +553,7 @@ int launch_corr_graph(Plan &p, long long r0, int set, Timers *tm)
     // (it finalizes batch i - nslots before launching batch i)
     long long *h = p.h_r0 + (p.r0_ring++ % Plan::kR0Ring);
     *h = r0;
+    PZL_PINNED_CHECK(h, sizeof(long long));
     PZL_HIPCHK(hipMemcpyAsync(p.d_r0 + set, h, sizeof(long long), hipMemcpyHostToDevice, p.st));
     Scope s(tm, 3, 0, p.st);
     PZL_HIPCHK(hipGraphLaunch(p.gexec[set], p.st));
@@ -630,8 +674,7 @@ struct SearchState {
             auto &hb = p->hpool[k];
             long long cap = want;
             hb.ptr = pin_take(want, &cap);
-            if (!hb.ptr &&
-                hipHostMalloc((void **)&hb.ptr, sizeof(PzlHostHit) * want, hipHostMallocDefault) != hipSuccess) {
+            if (!hb.ptr && pinned_alloc((void **)&hb.ptr, sizeof(PzlHostHit) * want) != hipSuccess) {
                 hb.ptr = nullptr;
                 return -1;
             }
@@ -730,6 +773,8 @@ int apply_slot(Plan &p, int slot, long long n, SearchState &ss)
     if (buf < 0)
         return fail(ss.err ? ss.err : PZL_ERR_HIP, "no pinned buffer for %lld hit records%s%s", n,
                     ss.err_msg.empty() ? "" : ": ", ss.err_msg.c_str());
+    if (p.hpool[buf].cap < n) return fail(PZL_ERR_HIP, "pinned buffer of %lld records for %lld hits", p.hpool[buf].cap, n);
+    PZL_PINNED_CHECK(p.hpool[buf].ptr, sizeof(PzlHostHit) * n);
     PZL_HIPCHK(hipMemcpyAsync(p.hpool[buf].ptr, p.d_recs, sizeof(PzlHostHit) * n, hipMemcpyDeviceToHost, p.st2));
     PZL_HIPCHK(hipEventRecord(p.hpool[buf].ev, p.st2));
     ss.nhits += n;
@@ -761,6 +806,7 @@ int finalize(Plan &p, int slot, long long r0, int nb, SearchState &ss, Timers *t
         if (rc) return rc;
         rc = launch_batch(p, r0 + b, 1, 0, slot, nullptr, tm);
         if (rc) return rc;
+        PZL_PINNED_CHECK(p.h_counts + 2 * slot, 2 * sizeof(unsigned long long));
         PZL_HIPCHK(hipMemcpyAsync(p.h_counts + 2 * slot, p.d_counts + 2 * slot, 2 * sizeof(unsigned long long),
                                   hipMemcpyDeviceToHost, p.sum_stream()));
         rc = sync_all(p);
@@ -1024,9 +1070,9 @@ int pzl_plan_create(const PzlConfig *cfg, int device, void **plan_out)
         PCHK(hipEventRecord(p->ev_corr[k], p->st));
         PCHK(hipEventRecord(p->ev_sum[k], p->st));
     }
-    PCHK(hipHostMalloc((void **)&p->h_r0, Plan::kR0Ring * sizeof(long long), hipHostMallocDefault));
+    PCHK(pinned_alloc((void **)&p->h_r0, Plan::kR0Ring * sizeof(long long)));
     PCHK(hipMemset(p->d_counts, 0, 2 * Plan::kSlots * sizeof(unsigned long long)));
-    PCHK(hipHostMalloc((void **)&p->h_counts, 2 * Plan::kSlots * sizeof(unsigned long long), hipHostMallocDefault));
+    PCHK(pinned_alloc((void **)&p->h_counts, 2 * Plan::kSlots * sizeof(unsigned long long)));
     // The kernel bank: built once per shape and kept resident.
     {
         bool built = false;
@@ -1136,7 +1182,12 @@ int pzl_search(void *plan, long long rstep_lo, long long rstep_hi, PzlCand *cand
     // representative and the event packets do not perturb the timed search
     constexpr int kSampleEvery = 8;
     long long sampled = 0;
+    bool timed_out = false;
     for (long long r0 = rstep_lo; r0 < rstep_hi; r0 += p->B, i++) {
+        if (p->has_deadline && std::chrono::steady_clock::now() >= p->deadline) {
+            timed_out = true;  // launch nothing more; the batches in flight drain below
+            break;
+        }
         const int nb = (int)std::min<long long>(p->B, rstep_hi - r0);
         const int slot = i % p->nslots;
         tm.active = (i % kSampleEvery) == 0;
@@ -1152,6 +1203,7 @@ int pzl_search(void *plan, long long rstep_lo, long long rstep_hi, PzlCand *cand
         // the search: the spectrum and the startr table)
         int rc = launch_batch(*p, r0, nb, i % Plan::kSets, slot, nullptr, &tm, true, p->use_graph, p->overlap_sum);
         if (rc) return rc;
+        PZL_PINNED_CHECK(p->h_counts + 2 * slot, 2 * sizeof(unsigned long long));
         PZL_HIPCHK(hipMemcpyAsync(p->h_counts + 2 * slot, p->d_counts + 2 * slot, 2 * sizeof(unsigned long long),
                                   hipMemcpyDeviceToHost, p->sum_stream()));
         PZL_HIPCHK(hipEventRecord(p->ev_count[slot], p->sum_stream()));
@@ -1179,6 +1231,12 @@ int pzl_search(void *plan, long long rstep_lo, long long rstep_hi, PzlCand *cand
     }
     ss.finish();
     if (ss.err) return fail(ss.err, "hit records did not arrive: %s", ss.err_msg.c_str());
+    if (timed_out) {
+        p->last_cands.clear();
+        p->last_hits_complete = false;
+        return fail(PZL_ERR_TIMEOUT, "search deadline passed after %d of %lld batches", i,
+                    (rstep_hi - rstep_lo + p->B - 1) / p->B);
+    }
     p->last_cands = ss.cl.items();
     const auto &items = p->last_cands;
     if (ncands) *ncands = (long long)items.size();
@@ -1217,8 +1275,7 @@ int pzl_search(void *plan, long long rstep_lo, long long rstep_hi, PzlCand *cand
         for (auto &H : p->harms) seg_bytes += H.fftlen / 2 * 8.0;
         const double planes = plane_alg_bytes(*p);
         stats->bytes_alg = stats->nrsteps * (planes + seg_bytes) + (double)p->g.bank_bytes;
-        stats->bytes_alg_corr = stats->nrsteps * (planes / 2 + seg_bytes) +
-                                (double)p->g.bank_bytes * std::ceil((double)stats->nrsteps / p->B);
+        stats->bytes_alg_corr = stats->nrsteps * (planes / 2 + seg_bytes) + (double)p->g.bank_bytes;
         // sampled batches -> estimates for the whole search (launch counts exact)
         const double scale = sampled > 0 ? (double)i / (double)sampled : 0.0;
         stats->ms_segments *= scale;
@@ -1238,14 +1295,20 @@ int pzl_search(void *plan, long long rstep_lo, long long rstep_hi, PzlCand *cand
             stats->class_ms[k] = per_class.count(c.fftlen) ? per_class[c.fftlen] : 0.0;
             const long long launches = per_class_n.count(c.fftlen) ? per_class_n[c.fftlen] : 0;
             stats->class_launches[k] = launches;
-            // per rstep: its plane rows written + segments read; per launch: the kernels read once
-            double per_rstep = 0, per_launch = 0;
+            // per rstep: its plane rows written + segments read; the kernel rows
+            // once per search (they are re-read by every launch: class_bank_bytes)
+            double per_rstep = 0, bank = 0, flops = 0;
             for (int h : c.harms) {
                 const PzlHarm &H = p->harms[h];
+                const double n = H.fftlen;
                 per_rstep += (double)H.nrows * H.max_numrs * 4.0 + H.fftlen / 2 * 8.0;
-                per_launch += (double)H.nrows * H.fftlen * 8.0;
+                bank += (double)H.nrows * H.fftlen * 8.0;
+                flops += (double)H.nrows * (5.0 * n * std::log2(n) + 6.0 * n + 3.0 * H.max_numrs);
             }
-            stats->class_bytes[k] = per_rstep * (double)(rstep_hi - rstep_lo) + per_launch * (double)launches;
+            const double nrs = (double)(rstep_hi - rstep_lo);
+            stats->class_bytes[k] = per_rstep * nrs + bank;
+            stats->class_bank_bytes[k] = bank;
+            stats->class_flops[k] = flops * nrs;
         }
     }
     return 0;
@@ -1425,6 +1488,18 @@ int pzl_autotune(void *plan, int reps, char *report, int report_len)
     return 0;
 }
 
+int pzl_plan_set_deadline(void *plan, double ms)
+{
+    Plan *p = (Plan *)plan;
+    if (!p) return fail(PZL_ERR_ARG, "null plan");
+    p->has_deadline = ms > 0;
+    if (p->has_deadline)
+        p->deadline = std::chrono::steady_clock::now() +
+                      std::chrono::duration_cast<std::chrono::steady_clock::duration>(
+                          std::chrono::duration<double, std::milli>(ms));
+    return 0;
+}
+
 int pzl_plan_set_overlap(void *plan, int on)
 {
     Plan *p = (Plan *)plan;
@@ -1538,6 +1613,14 @@ int pzl_get_cands(void *plan, PzlCand *cands, long long maxcands, long long *nca
     if (!p) return fail(PZL_ERR_ARG, "null plan");
     if (ncands) *ncands = (long long)p->last_cands.size();
     for (long long k = 0; k < (long long)p->last_cands.size() && k < maxcands; k++) cands[k] = p->last_cands[k];
+    return 0;
+}
+
+int pzl_plan_set_keep_hits(void *plan, long long limit)
+{
+    Plan *p = (Plan *)plan;
+    if (!p || limit < 0) return fail(PZL_ERR_ARG, "bad arguments");
+    p->keep_hits_limit = limit;
     return 0;
 }
 

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <chrono>
 #include <vector>
 
 #include "pzl_accel.h"
@@ -115,6 +116,9 @@ struct Plan {
     long long *d_r0 = nullptr;  // [kSets] first rstep of the batch in each set
     static constexpr int kR0Ring = 2 * kSlots;
     long long *h_r0 = nullptr;  // pinned staging ring of those values
+    // pzl_plan_set_deadline: searches stop launching batches after this time
+    bool has_deadline = false;
+    std::chrono::steady_clock::time_point deadline{};
     unsigned long long r0_ring = 0;
     hipStream_t sum_stream() const { return overlap_sum ? st3 : st; }
     int sum_order = 0;  // harmonic-sum workgroup order (autotuned)

@@ -71,6 +71,8 @@ struct Request {
     long long footprint = 0;
     double cells = 0;
     Clock::time_point t_submit, t_admit, t_end;
+    bool has_deadline = false;
+    Clock::time_point deadline{};
     bool done = false;
     int status = 0;
     std::string error;
@@ -97,27 +99,35 @@ struct Server {
     std::map<long long, std::shared_ptr<Request>> reqs;
     std::vector<PooledPlan> pool;
     long long admitted_bytes = 0;  // footprints of the searches in flight
-    long long resident_bytes = 0;  // all pooled plans (busy or idle)
-    long long peak_admitted = 0;
+    long long resident_bytes = 0;  // all pooled plans (busy or idle) and plans being created
+    long long peak_admitted = 0, peak_resident = 0, timed_out = 0;
     int inflight = 0, peak_inflight = 0;
     long long next_id = 1, completed = 0;
     bool stop = false;
     std::vector<std::thread> workers;
 
     // Free idle pooled plans, least recently pooled first, until 'need' more
-    // bytes fit beside what is resident.  Caller holds mu.
+    // bytes fit beside what is resident, then drop the kernel banks that no
+    // plan references any more (each plan's footprint counted its bank; a
+    // bank left in the process-wide cache would be HBM the budget no longer
+    // sees).  Caller holds mu.
     void evict_idle_locked(long long need)
     {
+        bool evicted = false;
         for (size_t i = 0; i < pool.size() && resident_bytes + need > budget;) {
             if (!pool[i].busy) {
                 pzl_plan_destroy(pool[i].plan);
                 resident_bytes -= pool[i].bytes;
                 pool.erase(pool.begin() + i);
+                evicted = true;
             } else {
                 i++;
             }
         }
+        if (evicted) (void)pzl_bank_cache_clear();
     }
+
+    static bool expired(const Request &r, Clock::time_point now) { return r.has_deadline && now >= r.deadline; }
 
     // The head of the queue may start: a slot is free and its footprint fits
     // beside the searches in flight (idle pooled plans are evictable).
@@ -127,6 +137,30 @@ struct Server {
         return admitted_bytes + queue.front()->footprint <= budget;
     }
 
+    // Requests whose deadline passed while queued fail without running.
+    // Caller holds mu.
+    bool drop_expired_locked()
+    {
+        const auto now = Clock::now();
+        bool any = false;
+        for (auto it = queue.begin(); it != queue.end();) {
+            if (expired(**it, now)) {
+                Request &r = **it;
+                r.status = PZL_ERR_TIMEOUT;
+                r.error = "deadline passed before the request was admitted";
+                r.t_admit = r.t_end = now;
+                r.done = true;
+                completed++;
+                timed_out++;
+                it = queue.erase(it);
+                any = true;
+            } else {
+                ++it;
+            }
+        }
+        return any;
+    }
+
     void worker()
     {
         for (;;) {
@@ -134,7 +168,12 @@ struct Server {
             void *plan = nullptr;
             {
                 std::unique_lock<std::mutex> lk(mu);
-                cv.wait(lk, [&] { return (stop && queue.empty()) || head_admissible_locked(); });
+                for (;;) {
+                    if (drop_expired_locked()) done_cv.notify_all();
+                    if (head_admissible_locked() || (stop && queue.empty())) break;
+                    // wake up now and then to fail queued requests past their deadline
+                    cv.wait_for(lk, std::chrono::milliseconds(20));
+                }
                 if (!head_admissible_locked()) return;  // stopping, nothing left to admit
                 r = queue.front();
                 queue.pop_front();
@@ -151,21 +190,35 @@ struct Server {
                         r->plan_reused = true;
                         break;
                     }
-                if (!plan) evict_idle_locked(r->footprint);
+                if (!plan) {
+                    // reserve the new plan's bytes in the same critical section as
+                    // the eviction, so that two workers creating plans at once
+                    // both see each other's share of the budget
+                    evict_idle_locked(r->footprint);
+                    resident_bytes += r->footprint;
+                    peak_resident = std::max(peak_resident, resident_bytes);
+                }
             }
             int rc = 0;
             if (!plan) {
                 rc = pzl_plan_create(&r->cfg, device, &plan);
+                std::lock_guard<std::mutex> lk(mu);
                 if (rc == 0) {
-                    std::lock_guard<std::mutex> lk(mu);
                     PooledPlan p;
                     p.key = shape_key(r->cfg);
                     p.plan = plan;
-                    p.bytes = r->footprint;
+                    p.bytes = r->footprint;  // counted at the reservation
                     p.busy = true;
                     pool.push_back(p);
-                    resident_bytes += p.bytes;
+                } else {
+                    resident_bytes -= r->footprint;
+                    plan = nullptr;
                 }
+            }
+            if (rc == 0 && r->has_deadline) {
+                const double left = std::chrono::duration<double, std::milli>(r->deadline - Clock::now()).count();
+                rc = left > 0 ? pzl_plan_set_deadline(plan, left)
+                              : fail(PZL_ERR_TIMEOUT, "deadline passed before the search started");
             }
             if (rc == 0)
                 rc = r->on_device ? pzl_set_spectrum_device(plan, r->fft, r->numbins)
@@ -177,10 +230,12 @@ struct Server {
                 r->cands.resize((size_t)n);
                 rc = pzl_get_cands(plan, r->cands.data(), n, &n);
             }
+            if (plan) (void)pzl_plan_set_deadline(plan, 0);
             {
                 std::lock_guard<std::mutex> lk(mu);
                 r->status = rc;
                 if (rc) r->error = pzl_last_error();
+                if (rc == PZL_ERR_TIMEOUT) timed_out++;
                 r->nhits = st.nhits;
                 r->cells = st.cells;
                 r->t_end = Clock::now();
@@ -188,8 +243,8 @@ struct Server {
                 completed++;
                 for (auto &p : pool)
                     if (p.plan == plan) p.busy = false;
-                if (rc && plan) {
-                    // a failed plan is not reused
+                if (rc && rc != PZL_ERR_TIMEOUT && plan) {
+                    // a failed plan is not reused (a timed-out search has drained: its plan is)
                     for (size_t i = 0; i < pool.size(); i++)
                         if (pool[i].plan == plan) {
                             pzl_plan_destroy(plan);
@@ -237,7 +292,7 @@ int pzl_server_create(int device, long long hbm_budget, int max_inflight, void *
 }
 
 static int submit(void *srv, const PzlConfig *cfg, const float *fft, bool on_device, long long numbins,
-                  long long *req_id)
+                  long long *req_id, double deadline_ms = 0)
 {
     Server *s = (Server *)srv;
     if (!s || !cfg || !fft || numbins <= 0) return fail(PZL_ERR_ARG, "bad request");
@@ -254,6 +309,11 @@ static int submit(void *srv, const PzlConfig *cfg, const float *fft, bool on_dev
     // memory behind the plan's streams
     r->footprint = g.device_bytes + PZL_RUNTIME_BYTES_PER_SEARCH;
     r->t_submit = Clock::now();
+    if (deadline_ms > 0) {
+        r->has_deadline = true;
+        r->deadline = r->t_submit + std::chrono::duration_cast<Clock::duration>(
+                                        std::chrono::duration<double, std::milli>(deadline_ms));
+    }
     {
         std::lock_guard<std::mutex> lk(s->mu);
         if (s->stop) return fail(PZL_ERR_ARG, "server is shutting down");
@@ -278,6 +338,12 @@ int pzl_server_submit_device(void *srv, const PzlConfig *cfg, const void *dev_ff
                              long long *req_id)
 {
     return submit(srv, cfg, (const float *)dev_fft, true, numbins, req_id);
+}
+
+int pzl_server_submit_deadline(void *srv, const PzlConfig *cfg, const void *fft, int on_device, long long numbins,
+                               double deadline_ms, long long *req_id)
+{
+    return submit(srv, cfg, (const float *)fft, on_device != 0, numbins, req_id, deadline_ms);
 }
 
 int pzl_server_wait(void *srv, long long req_id, PzlCand *cands, long long maxcands, long long *ncands,
@@ -343,6 +409,8 @@ int pzl_server_stats(void *srv, PzlServerStats *st)
     st->peak_inflight = s->peak_inflight;
     st->completed = s->completed;
     st->pooled_plans = (long long)s->pool.size();
+    st->peak_resident_bytes = s->peak_resident;
+    st->timed_out = s->timed_out;
     return 0;
 }
 

@@ -418,3 +418,84 @@ def test_driver_noharmpolish_fdot_only(tmp_path, config1_data):
         ro, zo, d, pw = O.max_rz_arr(fft, p["r"], p["z"])
         q = O.calc_props(d, p["r"], p["z"], 0.0)
         assert p["pow"] == pytest.approx(q.pow, rel=5e-3) and p["rawpow"] == pytest.approx(q.rawpow, rel=5e-3)
+
+
+PIPELINE = os.path.join(os.path.dirname(binding.__file__), "lib", "accelsearch_pipeline")
+
+
+def _same_shape_files(dirpath, n=5):
+    """n spectra of one shape (N = 2^20, dt = 1e-4 s), each with its own
+    jerked sinusoid, written as obs<k>.fft/.inf into dirpath."""
+    os.makedirs(dirpath, exist_ok=True)
+    names = []
+    for k in range(n):
+        N, dt = 1 << 20, 1e-4
+        x = synth.timeseries(N, dt, [(0.05 + 0.01 * k, 90.0 + 37.0 * k, 6.0 - 3.0 * k, 20.0 * (k - 2))], seed=500 + k)
+        D.setup_files(dirpath, synth.realfft_packed(x), N, dt, name="obs%d" % k)
+        names.append("obs%d.fft" % k)
+    return names
+
+
+@pytest.mark.gpu
+def test_pipeline_threads_equal_serial(tmp_path):
+    """The reference pipeline's thread overlap (bin/accelsearch_pipeline_cu.py:
+    296-407): CPU1, GPU and CPU2 of different files of the same shape run at
+    once on three threads of one process.  Every file's .cand records and
+    ACCEL text equal those of the command-line driver run on it alone."""
+    opts = ["-zmax", "20", "-wmax", "40", "-numharm", "4", "-sigma", "3.0"]
+    serial, piped = str(tmp_path / "serial"), str(tmp_path / "piped")
+    names = _same_shape_files(serial)
+    _same_shape_files(piped)
+    for nm in names:
+        r = subprocess.run([DRIVER] + opts + [nm], capture_output=True, text=True, timeout=300, cwd=serial)
+        assert r.returncode == 0, r.stdout + r.stderr
+    r = subprocess.run([PIPELINE] + opts + ["--"] + names, capture_output=True, text=True, timeout=300, cwd=piped)
+    print(r.stdout[-1500:])
+    assert r.returncode == 0, r.stdout + r.stderr
+    ncands = 0
+    for nm in names:
+        base = nm[:-4] + "_ACCEL_20_JERK_40"
+        for suffix in ("", ".cand"):
+            a = open(os.path.join(serial, base + suffix), "rb").read()
+            b = open(os.path.join(piped, base + suffix), "rb").read()
+            assert a == b, (nm, suffix)
+        ncands += len(np.fromfile(os.path.join(piped, base + ".cand"), dtype=np.uint8)) // 88
+    print("%d files, %d candidates, outputs identical to the serial runs" % (len(names), ncands))
+    assert ncands > len(names)
+
+
+@pytest.mark.gpu
+def test_spectrum_read_whole_usemmap0(tmp_path):
+    """create_accelobs(..., usemmap = 0) reads the spectrum whole instead of
+    mapping it; the GPU stage and CPU2's candidate optimisation work from that
+    copy, and the outputs equal the command-line driver's (which maps)."""
+    opts = ["-zmax", "20", "-wmax", "40", "-numharm", "4", "-sigma", "3.0"]
+    a_dir, b_dir = str(tmp_path / "mapped"), str(tmp_path / "read")
+    nm = _same_shape_files(a_dir, 1)[0]
+    _same_shape_files(b_dir, 1)
+    r = subprocess.run([DRIVER] + opts + [nm], capture_output=True, text=True, timeout=300, cwd=a_dir)
+    assert r.returncode == 0, r.stdout + r.stderr
+    L = D.api()
+    L.accelsearch_GPU.restype = C.c_int
+    L.accelsearch_GPU.argtypes = [D.AccelObs, C.POINTER(C.POINTER(D.SubharmInfo)), C.POINTER(C.POINTER(D.GSList)),
+                                  C.POINTER(D.Cmdline)]
+    L.accelsearch_CPU2.argtypes = [C.POINTER(C.POINTER(D.GSList)), C.POINTER(D.AccelObs), C.POINTER(D.InfoData),
+                                   C.POINTER(D.Cmdline)]
+    cwd = os.getcwd()
+    os.chdir(b_dir)
+    try:
+        n, av = D.argv(["accelsearch"] + opts + [nm])
+        cmd = L.parseCmdline(n, av)
+        obs, idata = D.AccelObs(), D.InfoData()
+        L.create_accelobs(C.byref(obs), C.byref(idata), cmd, 0)
+        assert obs.mmap_file == 0 and bool(obs.fft)
+        shi = L.create_subharminfos(C.byref(obs))
+        cands = C.POINTER(D.GSList)()
+        assert L.accelsearch_GPU(obs, shi, C.byref(cands), cmd) == 0
+        L.accelsearch_CPU2(C.byref(cands), C.byref(obs), C.byref(idata), cmd)
+    finally:
+        os.chdir(cwd)
+    base = nm[:-4] + "_ACCEL_20_JERK_40"
+    for suffix in ("", ".cand"):
+        assert open(os.path.join(a_dir, base + suffix), "rb").read() == \
+            open(os.path.join(b_dir, base + suffix), "rb").read(), suffix
