@@ -12,14 +12,20 @@
  *   fcomplex, rderivs              include/ransomfft.h:14-19, include/presto.h:161-169
  * Candidate lists are glib GSLists as in the reference.
  *
- * Behavioural notes (see INTEGRATION.md):
+ * Behavioural notes:
  *   - kernel.data is NULL: the FFT'd response kernels live in GPU memory
  *     (built there once per search shape and cached across calls).
  *   - accelsearch_GPU searches the fundamental stage also when numharm == 1
  *     (the reference GPU path returns no candidates then,
  *     accelsearch_lib.c:648); results follow PRESTO C accelsearch.
- *   - accelsearch_CPU2 sorts, removes harmonics and frees; candidate
- *     optimisation (max_rzw) and the ACCEL/.cand writers are not provided.
+ *   - accelsearch_CPU2 runs the reference's sequence
+ *     (accelsearch_lib.c:903-980): sort, eliminate_harmonics, the
+ *     optimisation of every candidate on the GPU (max_rzw_arr_harmonics /
+ *     max_rz_arr_harmonics, or per harmonic with -noharmpolish), calc_props,
+ *     the ACCEL text tables and the binary .cand file, then frees.
+ *   - CPU1, GPU and CPU2 of different files may run at once on different
+ *     threads, as the reference pipeline does (no shared mutable state
+ *     between the stages of different files).
  */
 #ifndef PZL_ACCEL_API_H
 #define PZL_ACCEL_API_H

@@ -18,7 +18,7 @@ LIBDIR = os.path.join(HERE, "lib")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PZL_ARCH", "gfx950")
 
-SOURCES = ["pzl_kernels.hip", "pzl_optimize.hip", "pzl_realfft.hip", "pzl_dedisp.hip", "pzl_engine.cpp", "pzl_optimize.cpp", "pzl_geometry.cpp",
+SOURCES = ["pzl_kernels.hip", "pzl_sort.hip", "pzl_optimize.hip", "pzl_realfft.hip", "pzl_dedisp.hip", "pzl_engine.cpp", "pzl_optimize.cpp", "pzl_geometry.cpp",
            "pzl_stats.cpp", "pzl_server.cpp",
            "accel_api.cpp", "accel_output.cpp"]
 GLIB_INC = ["/opt/conda/include/glib-2.0", "/opt/conda/lib/glib-2.0/include"]

@@ -159,8 +159,8 @@ PlanSizes plan_sizes(const PzlConfig &cfg, const PzlGeometry &g, const std::vect
 // The buffers a plan carves from its one device arena, in carving order
 // (pzl_plan_create assigns them in this order): harmonic tables, two sets of
 // batch buffers (segments, r-index tables, harmonic r maps, planes), hit
-// slots and ordering buffers.  The sort scratch is its reported size, or,
-// before a plan exists, bounded by 12 bytes per hit + 16 MiB.
+// slots and ordering buffers.  The sort scratch is the size the sort
+// reports for the slot capacity (a host computation, also before a plan exists).
 constexpr long long kArenaAlign = 256;
 
 std::vector<long long> plan_arena_items(const PzlGeometry &g, const PlanSizes &z, const std::vector<PzlHarm> &harms,
@@ -168,10 +168,16 @@ std::vector<long long> plan_arena_items(const PzlGeometry &g, const PlanSizes &z
 {
     const long long nh = (long long)harms.size();
     const PzlHarm &F = harms[0];
-    if (sort_temp < 0) sort_temp = 12 * z.hitcap + (16LL << 20);
+    if (sort_temp < 0) {
+        // the sort reports its scratch without touching the device
+        size_t b = 0;
+        (void)pzl_sort_hits(nullptr, &b, nullptr, nullptr, nullptr, nullptr, z.hitcap, 64, nullptr);
+        sort_temp = (long long)b;
+    }
     std::vector<long long> v{(long long)sizeof(PzlHarm) * nh, (long long)sizeof(int) * nh,
                              (long long)sizeof(double) * (g.nrsteps + 1), (long long)sizeof(uint16_t) * nh * F.numz,
-                             (long long)sizeof(int) * nh * F.numw};
+                             (long long)sizeof(int) * nh * F.numw,
+                             (long long)sizeof(int) * F.numw * F.numz * std::max<long long>(nh - 1, 1)};
     for (int k = 0; k < Plan::kSets; k++) {
         v.push_back((long long)sizeof(float2) * z.seg_stride * z.B);
         v.push_back((long long)sizeof(uint16_t) * z.B * nh * g.corr_uselen);
@@ -491,6 +497,7 @@ int launch_sum_part(Plan &p, long long r0, int nb, int set, int slot, float *d_s
     sa.rinds = p.d_rinds[set];
     sa.zinds = p.d_zinds;
     sa.winds = p.d_winds;
+    sa.zoff_tab = p.d_zoff;
     for (int s = 0; s < 6; s++) sa.powcut[s] = p.g.powcut[s];
     sa.hit_keys = p.d_hkeys[slot];
     sa.hit_pows = p.d_hpows[slot];
@@ -761,7 +768,7 @@ int reset_slot(Plan &p, int slot, hipStream_t st);
 int apply_slot(Plan &p, int slot, long long n, SearchState &ss)
 {
     if (n <= 0) return reset_slot(p, slot, p.st2);
-    PZL_HIPCHK(pzl_sort_hits(p.d_sort_temp, &p.sort_temp_bytes, p.d_hkeys[slot], p.d_skeys, p.d_hpows[slot],
+    PZL_HIPCHK(pzl_sort_hits(p.d_sort_temp, &p.sort_temp_bytes, p.d_hkeys[slot], p.d_hpows[slot], p.d_skeys,
                              p.d_spows, n, p.key_bits, p.st2));
     // the sort has read the slot: a later batch's harmonic sum may refill it
     // while the records are prepared and copied
@@ -1032,6 +1039,7 @@ int pzl_plan_create(const PzlConfig *cfg, int device, void **plan_out)
     p->d_startr = (double *)carve();
     p->d_zinds = (uint16_t *)carve();
     p->d_winds = (int *)carve();
+    p->d_zoff = (int *)carve();
     for (int k = 0; k < Plan::kSets; k++) {
         p->d_segs[k] = (float2 *)carve();
         p->d_rinds[k] = (uint16_t *)carve();
@@ -1059,6 +1067,22 @@ int pzl_plan_create(const PzlConfig *cfg, int device, void **plan_out)
         PCHK(hipMemcpy(p->d_startr, p->startr.data(), sizeof(double) * p->startr.size(), hipMemcpyHostToDevice));
     PCHK(hipMemcpy(p->d_zinds, p->zinds.data(), sizeof(uint16_t) * p->zinds.size(), hipMemcpyHostToDevice));
     PCHK(hipMemcpy(p->d_winds, p->winds.data(), sizeof(int) * p->winds.size(), hipMemcpyHostToDevice));
+    {
+        // byte offset of every subharmonic row the harmonic sum reads for a
+        // fundamental (w, z): [w][z][harmonic - 1]
+        const int nq = std::max(nh - 1, 1);
+        std::vector<int> zoff((size_t)F.numw * F.numz * nq, 0);
+        for (int ii = 0; ii < F.numw; ii++)
+            for (int jj = 0; jj < F.numz; jj++)
+                for (int h = 1; h < nh; h++) {
+                    const PzlHarm &H = p->harms[h];
+                    const long long off = 4 * (H.plane_off + ((long long)p->winds[(size_t)h * F.numw + ii] * H.numz +
+                                                              p->zinds[(size_t)h * F.numz + jj]) *
+                                                                 H.rs_stride);
+                    zoff[((size_t)ii * F.numz + jj) * nq + (h - 1)] = (int)off;
+                }
+        PCHK(hipMemcpy(p->d_zoff, zoff.data(), sizeof(int) * zoff.size(), hipMemcpyHostToDevice));
+    }
     for (int k = 0; k < Plan::kSets; k++) {
         PCHK(hipEventCreateWithFlags(&p->ev_corr[k], hipEventDisableTiming));
         PCHK(hipEventCreateWithFlags(&p->ev_sum[k], hipEventDisableTiming));

@@ -70,6 +70,8 @@ struct PzlSumArgs {
     const uint16_t *rinds;   // [nb][nh][corr_uselen]
     const uint16_t *zinds;   // [nh][numz]
     const int *winds;        // [nh][numw]
+    const int *zoff_tab;     // [numw][numz][nh - 1]: byte offset of each subharmonic's
+                             // (winds, zinds) row inside one rstep's planes
     float powcut[6];
     long long *hit_keys;     // unordered hits of this batch: keys ...
     float *hit_pows;         // ... and summed powers
@@ -99,10 +101,11 @@ hipError_t pzl_launch_corr(int N, int variant, const PzlCorrArgs *a, const int *
 int pzl_corr_num_variants(int n);
 const char *pzl_corr_variant_name(int n, int v);
 hipError_t pzl_launch_harmsum(const PzlSumArgs *a, hipStream_t st);
-// Radix sort of n (key, power) pairs on 'end_bit' key bits; temp == nullptr
-// queries the scratch size into *temp_bytes.
-hipError_t pzl_sort_hits(void *temp, size_t *temp_bytes, const long long *keys_in, long long *keys_out,
-                         const float *pows_in, float *pows_out, long long n, int end_bit, hipStream_t st);
+// Radix sort of n (key, power) pairs on 'end_bit' key bits (pzl_sort.hip):
+// from (keys, pows) into (keys_alt, pows_alt), both pairs overwritten;
+// temp == nullptr queries the scratch size into *temp_bytes (host only).
+hipError_t pzl_sort_hits(void *temp, size_t *temp_bytes, long long *keys, float *pows, long long *keys_alt,
+                         float *pows_alt, long long n, int end_bit, hipStream_t st);
 // Sorted (key, power) -> host records with rr and sigma (double precision).
 hipError_t pzl_launch_hitprep(const long long *keys, const float *pows, long long n, const double *startr,
                               int corr_uselen, int numz, int numw, int nstages, const double *numindep,

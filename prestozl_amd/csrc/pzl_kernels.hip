@@ -17,7 +17,6 @@
 
 #include <mutex>
 
-#include <hipcub/hipcub.hpp>
 
 #include "pzl_fft.h"
 #include "pzl_kernels.h"
@@ -565,6 +564,59 @@ __device__ __forceinline__ void load_run(__amdgpu_buffer_rsrc_t rs, int voff, in
     }
 }
 
+// The subharmonic index of fundamental r + i is rint((m + i) j / 2^s) for
+// some real m (half-integer steps of r), so its distance from the index of r
+// lies within one of i j / 2^s: it is floor(i j / 2^s) or one more, or, when
+// i j / 2^s is an integer D, one of D - 1, D, D + 1.  SelRange gives that
+// range (clamped to the RPT loaded values) for harmonic j / 2^s.
+template <int RPT> struct SelRange {
+    static constexpr int lo(int i, int j, int s)
+    {
+        const int num = i * j, den = 1 << s;
+        const int v = num % den == 0 ? num / den - 1 : num / den;
+        return v < 0 ? 0 : (v > RPT - 1 ? RPT - 1 : v);
+    }
+    static constexpr int hi(int i, int j, int s)
+    {
+        const int num = i * j, den = 1 << s;
+        const int v = num % den == 0 ? num / den + 1 : num / den + 1;
+        const int c = v > i ? i : v;
+        return c > RPT - 1 ? RPT - 1 : c;
+    }
+    // subharmonic q (stage order: stage s = 1.., odd j ascending) is j / 2^s
+    static constexpr int stage_of(int q)
+    {
+        int s = 1;
+        while ((1 << s) - 1 <= q) s++;
+        return s;
+    }
+    static constexpr int j_of(int q) { return 2 * (q - ((1 << (stage_of(q) - 1)) - 1)) + 1; }
+    // selection bits of subharmonic q for its r + i' with i' < i
+    static constexpr int bits_before_i(int q, int i)
+    {
+        int n = 0;
+        for (int ii = 1; ii < i; ii++) n += hi(ii, j_of(q), stage_of(q)) - lo(ii, j_of(q), stage_of(q));
+        return n;
+    }
+    // selection bits needed by subharmonics q' < q
+    static constexpr int bits_before(int q)
+    {
+        int n = 0;
+        for (int qq = 0; qq < q; qq++)
+            for (int i = 1; i < RPT; i++) n += hi(i, j_of(qq), stage_of(qq)) - lo(i, j_of(qq), stage_of(qq));
+        return n;
+    }
+};
+
+// f(std::integral_constant<int, I>) for I = B .. E - 1, unrolled at compile time.
+template <int B, int E, typename F> __device__ __forceinline__ void static_for(F &&f)
+{
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
 // One workgroup: RCH consecutive r of SUM_ZB consecutive z rows of one
 // (rstep, w) of the fundamental, RPT consecutive r per thread.  The
 // subharmonic r indices depend only on (rstep, harmonic, r), so each thread
@@ -572,25 +624,45 @@ __device__ __forceinline__ void load_run(__amdgpu_buffer_rsrc_t rs, int voff, in
 // 64-bit part and a small per-thread 32-bit part).  A subharmonic's ratio j/2^s
 // is below 1, so the RPT indices of a thread's r are non-decreasing with steps
 // of 0 or 1: they lie in the RPT floats from the first one, which one load
-// fetches; the thread keeps the first index as a byte offset and the others as
-// 2-bit distances from it.  The row offsets of the subharmonic z/w maps for the
-// workgroup's rows are staged in LDS and used as the scalar offset of buffer
-// loads over the rstep's planes.  For every cell all subharmonic powers are
-// loaded first and then added in the reference's order (stage 1..S, odd
-// harmonics ascending).
+// fetches.  The thread keeps the first index as a byte offset, and for every
+// other r one bit per index that r's map can still take beyond the lowest
+// (SelRange): the value is then chosen with bitfield selects, with no
+// per-lane condition registers held across the loop.  The byte offsets of
+// the subharmonic rows of each fundamental (w, z) come from a per-plan table
+// through scalar loads and are the scalar offset of buffer loads over the
+// rstep's planes.  For every cell all subharmonic powers are loaded first and
+// then added in the reference's order (stage 1..S, odd harmonics ascending).
 //
 // Workgroup ids are dealt round-robin over the 8 XCDs; each XCD is given a
 // contiguous range of the (rstep, r chunk, z block, w) space, w fastest
 // (order 0), or of (rstep, z block, w, r chunk), r chunk fastest (order 1):
 // neighbouring w rows share subharmonic rows, neighbouring r chunks the cache
 // lines at their edges, which then meet in one L2.
+// u where bit 'pos' of 'bits' is clear, the bits of x where it is set: a
+// sign-extending bitfield extract of the bit, then a bitfield select, in
+// registers, so that the compiler keeps the selection in the loop instead of
+// turning it into per-lane conditions held in scalar registers across it.
+template <int POS> __device__ __forceinline__ unsigned pick_bit_t(unsigned u, float x, unsigned bits)
+{
+    unsigned m;
+    asm("v_bfe_i32 %0, %1, %2, 1\n\tv_bfi_b32 %0, %0, %3, %4"
+        : "=&v"(m)
+        : "v"(bits), "n"(POS), "v"(__float_as_uint(x)), "v"(u));
+    return m;
+}
+
+// The row-offset table is written once per plan, before any search: reading
+// it through the constant address space lets the compiler use scalar loads.
+typedef const __attribute__((address_space(4))) int pzl_const_int;
+
 template <int NSTAGES, int RPT>
 __global__ __launch_bounds__(SumShape<RPT>::NT) void k_harmsum(PzlSumArgs a)
 {
     constexpr int NT = SumShape<RPT>::NT, RCH = SumShape<RPT>::RCH;
     constexpr int NSUB = (1 << (NSTAGES - 1)) - 1;
     constexpr int NQ = NSUB > 0 ? NSUB : 1;
-    __shared__ int zoff[NQ][SUM_ZB];
+    constexpr int NBITS = SelRange<RPT>::bits_before(NSUB);
+    constexpr int NW = NBITS > 0 ? (NBITS + 31) / 32 : 1;
     const int tid = threadIdx.x;
     const int chunks = (a.corr_uselen + RCH - 1) / RCH;
     const int zblocks = (a.numz + SUM_ZB - 1) / SUM_ZB;
@@ -631,37 +703,29 @@ __global__ __launch_bounds__(SumShape<RPT>::NT) void k_harmsum(PzlSumArgs a)
     const __amdgpu_buffer_rsrc_t planes = uniform_rsrc(a.planes + (long long)b * a.plane_stride,
                                                        (int)(4 * (a.plane_stride + PZL_PLANE_SLACK)));
 
-    if constexpr (NSUB > 0) {
-        for (int k = tid; k < NSUB * SUM_ZB; k += NT) {
-            const int q = k / SUM_ZB, zi = k % SUM_ZB;
-            const int h = q + 1;
-            const PzlHarm &H = a.harms[h];
-            const int z = min(z0 + zi, a.numz - 1);
-            zoff[q][zi] = (int)(4 * (H.plane_off + ((long long)a.winds[h * a.numw + w] * H.numz +
-                                                    a.zinds[h * a.numz + z]) * H.rs_stride));
-        }
-    }
-    int rind[NQ];      // byte offset of the first subharmonic index within its row
-    unsigned dsel[NQ];  // 2-bit distances of the others from it
+    int rind[NQ];        // byte offset of the first subharmonic index within its row
+    unsigned sel[NW];    // selection bits (SelRange), harmonic by harmonic
+#pragma unroll
+    for (int k = 0; k < NW; k++) sel[k] = 0;
     if constexpr (NSUB > 0) {
         const long long m0 = a.int_rlo ? (long long)(2.0 * a.startr[rstep]) : 0;
-#pragma unroll
-        for (int q = 0; q < NSUB; q++) {
+        static_for<0, NSUB>([&](auto QC) {
+            constexpr int q = decltype(QC)::value;
+            constexpr int s = SelRange<RPT>::stage_of(q), j = SelRange<RPT>::j_of(q);
             const int h = q + 1;
             int sidx[RPT];
             if (a.int_rlo) {
                 // rint((m0 + r) j / 2^s) - 2 rlo_h with m0 j split as q0 2^s + rem0
-                const int j = a.harms[h].harmnum, sh = a.harms[h].stage;
                 const long long p0 = m0 * j;
-                const long long q0 = p0 >> sh;
-                const int rem0 = (int)(p0 - (q0 << sh));
+                const long long q0 = p0 >> s;
+                const int rem0 = (int)(p0 - (q0 << s));
                 const int base_h = (int)(q0 - 2 * a.hr_tab[b * a.nh + h].rlo);
-                const int half = (1 << sh) >> 1;  // sh >= 1 for every subharmonic
+                constexpr int half = (1 << s) >> 1;
 #pragma unroll
                 for (int i = 0; i < RPT; i++) {
                     const int tt = rem0 + rr[i] * j;
-                    int qq = tt >> sh;
-                    const int rem = tt - (qq << sh);
+                    int qq = tt >> s;
+                    const int rem = tt - (qq << s);
                     // round half to even on the full quotient q0 + qq
                     if (rem > half || (rem == half && ((q0 + qq) & 1))) qq++;
                     sidx[i] = base_h + qq;
@@ -672,52 +736,51 @@ __global__ __launch_bounds__(SumShape<RPT>::NT) void k_harmsum(PzlSumArgs a)
                     sidx[i] = a.rinds[((long long)b * a.nh + h) * a.corr_uselen + rr[i]];
             }
             rind[q] = 4 * sidx[0];
-            unsigned d = 0;
-#pragma unroll
-            for (int i = 1; i < RPT; i++) d |= (unsigned)min(max(sidx[i] - sidx[0], 0), RPT - 1) << (2 * i);
-            dsel[q] = d;
-        }
+            static_for<1, RPT>([&](auto IC) {
+                constexpr int i = decltype(IC)::value;
+                constexpr int lo = SelRange<RPT>::lo(i, j, s), hi = SelRange<RPT>::hi(i, j, s);
+                constexpr int b0 = SelRange<RPT>::bits_before(q) + SelRange<RPT>::bits_before_i(q, i);
+                const int d = sidx[i] - sidx[0];
+                static_for<lo + 1, hi + 1>([&](auto CC) {
+                    constexpr int c = decltype(CC)::value;
+                    constexpr int bit = b0 + (c - lo - 1);
+                    sel[bit >> 5] |= (unsigned)(d >= c) << (bit & 31);
+                });
+            });
+        });
     }
-    __syncthreads();
 
     const unsigned long long lanemask_lt = (1ull << (__lane_id())) - 1ull;
     const int frow = (int)(4 * (F.plane_off + ((long long)w * a.numz + z0) * F.rs_stride));
+    const pzl_const_int *ztab = (const pzl_const_int *)(a.zoff_tab + ((long long)w * a.numz + z0) * NQ);
     for (int zi = 0; zi < nz; zi++) {
         const int z = z0 + zi;
         float acc[RPT];
-        load_run<RPT>(planes, 4 * rbase, __builtin_amdgcn_readfirstlane(frow + zi * 4 * (int)F.rs_stride), acc);
+        load_run<RPT>(planes, 4 * rbase, frow + zi * 4 * (int)F.rs_stride, acc);
         float x[NQ][RPT];
         if constexpr (NSUB > 0) {
+            const pzl_const_int *zt = ztab + zi * NQ;
 #pragma unroll
-            for (int q = 0; q < NSUB; q++)
-                load_run<RPT>(planes, rind[q], __builtin_amdgcn_readfirstlane(zoff[q][zi]), x[q]);
+            for (int q = 0; q < NSUB; q++) load_run<RPT>(planes, rind[q], zt[q], x[q]);
         }
-        int q = 0;
-#pragma unroll
-        for (int s = 0; s < NSTAGES; s++) {
-            if (s > 0) {
-#pragma unroll
-                for (int k = 0; k < (1 << (s - 1)); k++, q++) {
-#pragma unroll
-                    for (int i = 0; i < RPT; i++) {
-                        float v = x[q][0];
-                        const int d = (int)((dsel[q] >> (2 * i)) & 3u);
-#pragma unroll
-                        for (int e = 1; e <= i; e++) v = d >= e ? x[q][e] : v;
-                        acc[i] += v;
-                    }
-                }
-            }
+        // threshold of stage s after its harmonics are in: compaction of the
+        // wave's hits in scan order (one atomic per wave)
+        auto stage_check = [&](int s) {
             bool hit[RPT];
-            unsigned long long mask[RPT];
-            int cnt = 0;
+            bool any = false;
 #pragma unroll
             for (int i = 0; i < RPT; i++) {
                 hit[i] = active[i] && (acc[i] > a.powcut[s]);
-                mask[i] = __ballot(hit[i]);
-                cnt += __popcll(mask[i]);
+                any = any || hit[i];
             }
-            if (cnt) {
+            if (__ballot(any)) {
+                unsigned long long mask[RPT];
+                int cnt = 0;
+#pragma unroll
+                for (int i = 0; i < RPT; i++) {
+                    mask[i] = __ballot(hit[i]);
+                    cnt += __popcll(mask[i]);
+                }
                 unsigned long long slot0 = 0;
                 if (__lane_id() == 0) slot0 = atomicAdd(a.nhits, (unsigned long long)cnt);
                 slot0 = __shfl(slot0, 0);
@@ -735,7 +798,27 @@ __global__ __launch_bounds__(SumShape<RPT>::NT) void k_harmsum(PzlSumArgs a)
                     slot0 += __popcll(mask[i]);
                 }
             }
-        }
+        };
+        stage_check(0);
+        static_for<0, NSUB>([&](auto QC) {
+            constexpr int q = decltype(QC)::value;
+            constexpr int s = SelRange<RPT>::stage_of(q), j = SelRange<RPT>::j_of(q);
+            acc[0] += x[q][0];
+            static_for<1, RPT>([&](auto IC) {
+                constexpr int i = decltype(IC)::value;
+                constexpr int lo = SelRange<RPT>::lo(i, j, s), hi = SelRange<RPT>::hi(i, j, s);
+                constexpr int b0 = SelRange<RPT>::bits_before(q) + SelRange<RPT>::bits_before_i(q, i);
+                unsigned u = __float_as_uint(x[q][lo]);
+                static_for<lo + 1, hi + 1>([&](auto CC) {
+                    constexpr int c = decltype(CC)::value;
+                    constexpr int bit = b0 + (c - lo - 1);
+                    u = pick_bit_t<bit & 31>(u, x[q][c], sel[bit >> 5]);
+                });
+                acc[i] += __uint_as_float(u);
+            });
+            // the last harmonic of stage s (q = 2^s - 2): its threshold
+            if constexpr (((q + 2) & (q + 1)) == 0) stage_check(s);
+        });
         if (a.sums_out) {
 #pragma unroll
             for (int i = 0; i < RPT; i++)
@@ -849,14 +932,6 @@ extern "C" hipError_t pzl_launch_threshold(const float *plane, long long n, floa
 // ---------------------------------------------------------------------------
 // Launchers (C linkage for the host engine)
 // ---------------------------------------------------------------------------
-
-extern "C" hipError_t pzl_sort_hits(void *temp, size_t *temp_bytes, const long long *keys_in, long long *keys_out,
-                                    const float *pows_in, float *pows_out, long long n, int end_bit,
-                                    hipStream_t st)
-{
-    return hipcub::DeviceRadixSort::SortPairs(temp, *temp_bytes, keys_in, keys_out, pows_in, pows_out, (int)n, 0,
-                                              end_bit, st);
-}
 
 extern "C" hipError_t pzl_launch_hitprep(const long long *keys, const float *pows, long long n,
                                          const double *startr, int corr_uselen, int numz, int numw, int nstages,

@@ -63,6 +63,7 @@ struct Plan {
     double numindep_d[6] = {0, 0, 0, 0, 0, 0};
     uint16_t *d_zinds = nullptr;
     int *d_winds = nullptr;
+    int *d_zoff = nullptr;  // [numw][numz][nh - 1] subharmonic row byte offsets (harmonic sum)
     const float2 *d_fft = nullptr;
     float2 *d_fft_own = nullptr;
     long long dev_bytes = 0;  // device memory this plan has allocated and holds (the bank aside)

@@ -405,8 +405,8 @@ hipError_t pzl_launch_harmsum(const PzlSumArgs *a, hipStream_t st)
     return hipSuccess;
 }
 
-hipError_t pzl_sort_hits(void *temp, size_t *temp_bytes, const long long *keys_in, long long *keys_out,
-                         const float *pows_in, float *pows_out, long long n, int, hipStream_t st)
+hipError_t pzl_sort_hits(void *temp, size_t *temp_bytes, long long *keys_in, float *pows_in, long long *keys_out,
+                         float *pows_out, long long n, int, hipStream_t st)
 {
     if (!temp) {
         *temp_bytes = 4096;

@@ -4,8 +4,8 @@ context and a 1.1 factor to estimates of its own buffers).
 
 For each search shape: the model of a created plan equals, byte for byte, what
 the plan allocated (counted at every allocation) plus its kernel bank; the
-model before the plan exists bounds it from above (sort scratch bounded by 12
-bytes per hit + 16 MiB); and the device's free memory drops by that amount
+model before the plan exists is not below it (the hit sort reports its scratch
+without the device, so the two agree); and the device's free memory drops by that amount
 plus at most the runtime allowance that admission adds per search
 (PZL_RUNTIME_BYTES_PER_SEARCH, 32 MiB: the HIP runtime's memory behind the
 plan's streams; measured up to 27 MiB)."""
