@@ -323,6 +323,17 @@ int pzl_realfft_device(int device, void *d_data, long long n, int isign);
  * launches: the DM trials of a dedispersion run (realfft per .dat file,
  * realfft.c:66-251). */
 int pzl_realfft_device_batch(int device, void *d_data, long long n, int nseries, int isign);
+/* The same transforms without allocating anything: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under
+ * stream capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * workspace is a fixed device array of 16.5 MiB that each GPU holds until the
+ * process ends, so n * nseries is at most 2^22 (one series of 4M samples, 16
+ * DM trials of 2^18); larger requests fail with PZL_ERR_ARG.  The launches are
+ * enqueued on 'stream' (a hipStream_t; NULL: the null stream) and the call
+ * returns without waiting for them.  Every call on a GPU shares that GPU's
+ * workspace: calls whose transforms overlap on one GPU (other streams, other
+ * threads) race and must be ordered by the caller. */
+int pzl_realfft_device_noalloc(int device, void *d_data, long long n, int nseries, int isign, void *stream);
 
 /* ---------------- dedispersion: the DM-trial producer ---------------------
  * prepsubband_cu's GPU kernels (prepsubband_kernels.cu:23-24) on device

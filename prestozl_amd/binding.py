@@ -121,6 +121,7 @@ def lib():
         L.pzl_realfft.argtypes = [P(C.c_float), C.c_longlong, C.c_int]
         L.pzl_realfft_device.argtypes = [C.c_int, C.c_void_p, C.c_longlong, C.c_int]
         L.pzl_realfft_device_batch.argtypes = [C.c_int, C.c_void_p, C.c_longlong, C.c_int, C.c_int]
+        L.pzl_realfft_device_noalloc.argtypes = [C.c_int, C.c_void_p, C.c_longlong, C.c_int, C.c_int, C.c_void_p]
         L.pzl_downsamp_data_array.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int]
         L.pzl_get_dedisp.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p]
         L.pzl_get_dedisp_strided.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
@@ -273,6 +274,14 @@ def realfft_device(ptr, n, isign=-1, device=0, nseries=1):
     """The same in place on device memory at 'ptr': nseries series of n floats
     each, back to back."""
     check(lib().pzl_realfft_device_batch(device, C.c_void_p(ptr), int(n), int(nseries), int(isign)))
+
+
+def realfft_device_noalloc(ptr, n, isign=-1, device=0, nseries=1, stream=None):
+    """pzl_realfft_device_noalloc: the transforms of realfft_device with the
+    library's fixed workspace (n * nseries <= 2^22), enqueued on 'stream' (a
+    hipStream_t as an int, None: the null stream) without waiting."""
+    check(lib().pzl_realfft_device_noalloc(device, C.c_void_p(ptr), int(n), int(nseries), int(isign),
+                                           C.c_void_p(stream)))
 
 
 def device_memory(device=0):

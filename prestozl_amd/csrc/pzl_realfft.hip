@@ -492,32 +492,42 @@ static hipError_t launch_rows(int L, dim3 grid, size_t lds, hipStream_t st, A...
     return hipGetLastError();
 }
 
-// The transforms of nb series of n floats each, back to back at d (device
-// memory of the current device).
-static int realfft_on_device(float *d_data, long long n, int nb, int isign, hipStream_t st, RfWork *w)
+// The workspace (float2 elements) of nb transforms of n floats: the series
+// workspace, the two FFT twiddle tables and the four-step tables (double2 = 2
+// float2).  Checks the arguments; fills M1, M2 and 'single'.
+static int realfft_workspace(long long n, int nb, int isign, int *M1, int *M2, bool *single, long long *need)
 {
     if (n < 2 || n % 2) return fail(PZL_ERR_ARG, "realfft: n = %lld must be even and >= 2", n);
     if (isign != -1 && isign != 1) return fail(PZL_ERR_ARG, "realfft: isign must be -1 or +1");
     if (nb < 1 || nb > 65535) return fail(PZL_ERR_ARG, "realfft: %d series (1 to 65535)", nb);
     const long long M = n / 2;
-    float2 *d = (float2 *)d_data;
-    int M1 = (int)M, M2 = 1;
-    const bool single = M <= RF_MAXE;
-    if (!smooth(M) || (!single && !split(M, &M1, &M2)))
+    *M1 = (int)M;
+    *M2 = 1;
+    *single = M <= RF_MAXE;
+    if (!smooth(M) || (!*single && !split(M, M1, M2)))
         return fail(PZL_ERR_UNSUPPORTED, "realfft: n/2 = %lld is not a product of 2, 3, 5, 7, 11 up to 2^26", M);
-    // series workspace, the two FFT twiddle tables, the four-step tables (double2 = 2 float2)
+    const long long nhi = *single ? 0 : (M + RF_TB - 1) / RF_TB;
+    *need = (*single ? 0 : M * nb) + *M1 + *M2 + (*single ? 0 : 2 * (RF_TB + nhi) + 1);
+    return 0;
+}
+
+// The transforms of nb series of n floats each, back to back at d (device
+// memory of the current device), enqueued on st with the workspace 'buf' of
+// at least realfft_workspace's 'need' elements.
+static int realfft_enqueue(float *d_data, long long n, int nb, int isign, hipStream_t st, float2 *buf)
+{
+    int M1, M2;
+    bool single;
+    long long need;
+    int rc = realfft_workspace(n, nb, isign, &M1, &M2, &single, &need);
+    if (rc) return rc;
+    (void)need;
+    const long long M = n / 2;
+    float2 *d = (float2 *)d_data;
     const long long nhi = single ? 0 : (M + RF_TB - 1) / RF_TB;
-    const long long need = (single ? 0 : M * nb) + M1 + M2 + (single ? 0 : 2 * (RF_TB + nhi) + 1);
-    if (need > w->cap) {
-        (void)hipFree(w->buf);
-        w->buf = nullptr;
-        w->cap = 0;
-        PZL_HIPCHK(hipMalloc(&w->buf, sizeof(float2) * need));
-        w->cap = need;
-    }
-    float2 *ws = w->buf, *tw1 = w->buf + (single ? 0 : M * nb), *tw2 = tw1 + M1;
-    const long long toff = (tw2 + M2 - w->buf + 1) & ~1LL;  // 16-byte aligned (float2 units)
-    double2 *tlo = (double2 *)(w->buf + toff), *thi = tlo + RF_TB;
+    float2 *ws = buf, *tw1 = buf + (single ? 0 : M * nb), *tw2 = tw1 + M1;
+    const long long toff = (tw2 + M2 - buf + 1) & ~1LL;  // 16-byte aligned (float2 units)
+    double2 *tlo = (double2 *)(buf + toff), *thi = tlo + RF_TB;
     hipLaunchKernelGGL(k_rf_twiddles, dim3((M1 + 255) / 256), dim3(256), 0, st, tw1, M1);
     hipLaunchKernelGGL(k_rf_twiddles, dim3((M2 + 255) / 256), dim3(256), 0, st, tw2, M2);
     if (!single) {
@@ -557,7 +567,76 @@ static int realfft_on_device(float *d_data, long long n, int nb, int isign, hipS
     return 0;
 }
 
+// The same with the device's cached workspace, grown with hipMalloc as needed.
+static int realfft_on_device(float *d_data, long long n, int nb, int isign, hipStream_t st, RfWork *w)
+{
+    int M1, M2;
+    bool single;
+    long long need;
+    int rc = realfft_workspace(n, nb, isign, &M1, &M2, &single, &need);
+    if (rc) return rc;
+    if (need > w->cap) {
+        (void)hipFree(w->buf);
+        w->buf = nullptr;
+        w->cap = 0;
+        PZL_HIPCHK(hipMalloc(&w->buf, sizeof(float2) * need));
+        w->cap = need;
+    }
+    return realfft_enqueue(d_data, n, nb, isign, st, w->buf);
+}
+
+// Workspace of pzl_realfft_device_noalloc: a file-scope device array (each GPU
+// the library runs on holds its own copy until the process ends), reached
+// from the host only through the address hipGetSymbolAddress gives on that
+// device.  2^21 + 2^16 float2 (16.5 MiB): one series of up to 2^22 samples,
+// or n * nseries up to 2^22 in general.
+#define PZL_RF_SCRATCH_ELEMS ((1LL << 21) + (1LL << 16))
+static __device__ __align__(256) float2 scratch[PZL_RF_SCRATCH_ELEMS];
+
+namespace {
+std::mutex g_rf_scratch_mu;
+std::vector<float2 *> g_rf_scratch;  // per device: the address of 'scratch' there
+
+// The device address of 'scratch' on 'device' (which must be current), looked
+// up on the first call for that device.
+int scratch_on(int device, float2 **out)
+{
+    std::lock_guard<std::mutex> lk(g_rf_scratch_mu);
+    if ((int)g_rf_scratch.size() <= device) g_rf_scratch.resize(device + 1, nullptr);
+    if (!g_rf_scratch[device]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(PZL_ERR_HIP, "realfft: the scratch array of device %d has no address: %s", device,
+                        hipGetErrorString(e));
+        g_rf_scratch[device] = (float2 *)ptr;
+    }
+    *out = g_rf_scratch[device];
+    return 0;
+}
+}  // namespace
+
 extern "C" {
+
+int pzl_realfft_device_noalloc(int device, void *d_data, long long n, int nseries, int isign, void *stream)
+{
+    if (!d_data) return fail(PZL_ERR_ARG, "realfft: null data");
+    int M1, M2;
+    bool single;
+    long long need;
+    int rc = realfft_workspace(n, nseries, isign, &M1, &M2, &single, &need);
+    if (rc) return rc;
+    if (need > PZL_RF_SCRATCH_ELEMS)
+        return fail(PZL_ERR_ARG,
+                    "realfft: %d series of %lld samples need %lld bytes of workspace; the allocation-free entry "
+                    "point has %lld (n * nseries up to 2^22)",
+                    nseries, n, (long long)sizeof(float2) * need, (long long)sizeof(float2) * PZL_RF_SCRATCH_ELEMS);
+    PZL_HIPCHK(hipSetDevice(device));
+    float2 *ws = nullptr;
+    rc = scratch_on(device, &ws);
+    if (rc) return rc;
+    return realfft_enqueue((float *)d_data, n, nseries, isign, (hipStream_t)stream, ws);
+}
 
 int pzl_realfft_device(int device, void *d_data, long long n, int isign)
 {
