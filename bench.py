@@ -425,8 +425,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the one-rstep-per-launch (plain GPU form) search")
-    ap.add_argument("--serve", type=int, default=24, help="requests in the served-stream leg (0: skip)")
-    ap.add_argument("--serve-rate", type=float, default=1.5, help="Poisson arrival rate of the served stream (1/s)")
+    ap.add_argument("--serve", type=int, default=256,
+                    help="requests in the served-stream leg (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=1.2, help="Poisson arrival rate of the served stream (1/s)")
     ap.add_argument("--serve-inflight", type=int, default=3)
     ap.add_argument("--dm-trials", type=int, default=564, help="files in the DM-trial leg (0: skip)")
     ap.add_argument("--no-tune", action="store_true", help="use the default correlation variants")
@@ -442,6 +443,8 @@ def main():
                     help="run only the dedispersion -> realfft -> search leg and print its line")
     ap.add_argument("--dedisp-no-serve", action="store_true",
                     help="with --dedisp-only: leave out the request-loop search of every trial")
+    ap.add_argument("--sigma", type=float, default=5.0,
+                    help="profiling runs: the search threshold (the headline search uses 5; a high one leaves almost no hits)")
     ap.add_argument("--overlap", type=int, default=-1,
                     help="profiling runs: 0 = one stream, 1 = harmonic sums on a side stream, 2 = also the "
                          "shorter FFT lengths (the default)")
@@ -471,7 +474,7 @@ def main():
         return
     fft, N, dt, opts, sigs = synth.config2(N=1 << args.log2n)
     kw = dict(N=N, dt=dt, numbins=len(fft), zmax=opts["zmax"], wmax=opts["wmax"], numharm=opts["numharm"],
-              sigma=opts["sigma"], wmaxP=True)
+              sigma=args.sigma, wmaxP=True)
     plan = B.Plan(B.make_config(batchsize=args.batchsize, **kw))
     plan.set_spectrum(fft)
     g = plan.geom
