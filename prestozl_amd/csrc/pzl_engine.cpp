@@ -809,6 +809,9 @@ int finalize(Plan &p, int slot, long long r0, int nb, SearchState &ss, Timers *t
     int rc = sync_all(p);
     if (rc) return rc;
     for (int b = 0; b < nb; b++) {
+        // the previous rstep's sort (stream st2) reads and rewrites the slot's
+        // buffers: the harmonic sum refills them only after it
+        PZL_HIPCHK(hipStreamWaitEvent(p.sum_stream(), p.ev_free[slot], 0));
         rc = reset_slot(p, slot, p.sum_stream());
         if (rc) return rc;
         rc = launch_batch(p, r0 + b, 1, 0, slot, nullptr, tm);

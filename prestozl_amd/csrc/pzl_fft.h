@@ -287,7 +287,7 @@ template <int R, int SIGN, typename C> PZL_FFT_HD void dftR(C *v)
 // in place.  R and the butterflies per thread are compile-time, so the
 // register arrays need no scratch.
 template <int SIGN, int R, int NT>
-__device__ void lds_fft_pass_f64(double2 *buf, int n, int Ns, int tid)
+__device__ __forceinline__ void lds_fft_pass_f64(double2 *buf, int n, int Ns, int tid)
 {
     constexpr int MAXB = (4096 / R + NT - 1) / NT;  // butterflies per thread, n <= 4096
     const int nbf = n / R;
@@ -332,7 +332,7 @@ __device__ void lds_fft_pass_f64(double2 *buf, int n, int Ns, int tid)
 // while 4 divides what is left, then 2, 3, 5.  Used by the kernel-bank build
 // and the segment transforms.
 template <int SIGN, int NT>
-__device__ void lds_fft_f64(double2 *buf, int n, int tid)
+__device__ __forceinline__ void lds_fft_f64(double2 *buf, int n, int tid)
 {
     int Ns = 1;
     while (Ns < n) {

@@ -49,7 +49,10 @@ struct PzlCorrArgs {
     int nh, nb;
     const int *class_harms;  // harmonics of one FFT length
     int nclass;
-    long long total_items;   // work units: per harmonic of the class, ceil(nrows / rows per workgroup)
+    long long total_items;   // work units: per harmonic of the class, ceil(nrows / rows per workgroup),
+                             // times nblk
+    int rb, nblk;            // a unit's rows take rsteps [k rb, min(nb, (k + 1) rb)) of the batch,
+                             // k < nblk (set by the launcher)
     const float2 *segs;
     long long seg_stride;
     const float2 *bank;

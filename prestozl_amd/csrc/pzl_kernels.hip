@@ -408,21 +408,25 @@ __global__ __launch_bounds__(NT) void k_corr(PzlCorrArgs a)
     }
 
     for (long long unit = blockIdx.x; unit < a.total_items; unit += gridDim.x) {
-        // the unit's harmonic and first row (units: per harmonic of the class, ceil(nrows / G))
+        // the unit's rows (per harmonic of the class, ceil(nrows / G) row groups) and
+        // its block of rsteps: the blocks of one row group are adjacent units
+        const int ru = (int)(unit / a.nblk), blk = (int)(unit - (long long)ru * a.nblk);
         int h = -1, row0 = 0;
         {
-            long long acc = 0;
+            int acc = 0;
             for (int c = 0; c < a.nclass; c++) {
                 const int hh = a.class_harms[c];
-                const long long cnt = (a.harms[hh].nrows + G - 1) / G;
-                if (unit < acc + cnt) {
+                const int cnt = (a.harms[hh].nrows + G - 1) / G;
+                if (ru < acc + cnt) {
                     h = hh;
-                    row0 = (int)(unit - acc) * G;
+                    row0 = (ru - acc) * G;
                     break;
                 }
                 acc += cnt;
             }
         }
+        const int b_lo = __builtin_amdgcn_readfirstlane(blk * a.rb);
+        const int b_hi = __builtin_amdgcn_readfirstlane(min(a.nb, b_lo + a.rb));
         h = __builtin_amdgcn_readfirstlane(h);
         row0 = __builtin_amdgcn_readfirstlane(row0);
         const PzlHarm &H = a.harms[h];
@@ -493,8 +497,8 @@ __global__ __launch_bounds__(NT) void k_corr(PzlCorrArgs a)
                 }
             }
         };
-        load_seg(0);
-        for (int b = 0; b < a.nb; b++) {
+        load_seg(b_lo);
+        for (int b = b_lo; b < b_hi; b++) {
             const int numrs = __builtin_amdgcn_readfirstlane(a.hr_tab[b * a.nh + h].numrs);
             float *pbase = a.planes + (long long)b * a.plane_stride + H.plane_off + (long long)row0 * H.rs_stride;
             float2 cur[CP::per(0)][NSV];
@@ -502,7 +506,7 @@ __global__ __launch_bounds__(NT) void k_corr(PzlCorrArgs a)
             for (int i = 0; i < CP::per(0); i++)
 #pragma unroll
                 for (int r = 0; r < NSV; r++) cur[i][r] = sv[i][r];
-            if (b + 1 < a.nb) load_seg(b + 1);
+            if (b + 1 < b_hi) load_seg(b + 1);
             // pass 0: P[k] = seg[k mod N/2] * conj(kern[k])
 #pragma unroll
             for (int i = 0; i < CP::per(0); i++) {
@@ -753,15 +757,29 @@ __global__ __launch_bounds__(SumShape<RPT>::NT) void k_harmsum(PzlSumArgs a)
     const unsigned long long lanemask_lt = (1ull << (__lane_id())) - 1ull;
     const int frow = (int)(4 * (F.plane_off + ((long long)w * a.numz + z0) * F.rs_stride));
     const pzl_const_int *ztab = (const pzl_const_int *)(a.zoff_tab + ((long long)w * a.numz + z0) * NQ);
+    // A subharmonic's z map repeats rows (z h rounds to the same row for about
+    // 1 - h of consecutive z): its values are then still in registers and are
+    // not loaded again.
+    float x[NQ][RPT];
+    int zprev[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) zprev[q] = -1;
     for (int zi = 0; zi < nz; zi++) {
         const int z = z0 + zi;
         float acc[RPT];
         load_run<RPT>(planes, 4 * rbase, frow + zi * 4 * (int)F.rs_stride, acc);
-        float x[NQ][RPT];
         if constexpr (NSUB > 0) {
             const pzl_const_int *zt = ztab + zi * NQ;
+            int off[NQ];
 #pragma unroll
-            for (int q = 0; q < NSUB; q++) load_run<RPT>(planes, rind[q], zt[q], x[q]);
+            for (int q = 0; q < NSUB; q++) off[q] = zt[q];  // all scalar loads first, then the branches
+#pragma unroll
+            for (int q = 0; q < NSUB; q++) {
+                if (off[q] != zprev[q]) {
+                    load_run<RPT>(planes, rind[q], off[q], x[q]);
+                    zprev[q] = off[q];
+                }
+            }
         }
         // threshold of stage s after its harmonics are in: compaction of the
         // wave's hits in scan order (one atomic per wave)
@@ -978,6 +996,9 @@ extern "C" hipError_t pzl_launch_segments(const PzlBatchArgs *a, int max_fftlen,
 // per G rows of a harmonic (the grid is not persistent, so that kernels of
 // concurrent searches interleave); its twiddles come from a table built once
 // per device.
+// Waves a correlation launch aims for at least (two rounds of 4 waves per SIMD).
+constexpr long long kCorrWaves = 2 * 4 * 1024;
+
 template <int N, int NT, int G, bool PING, int SH, int PADSH, int... Rs>
 static hipError_t launch_corr_t(const PzlCorrArgs *a, const int *class_nrows, hipStream_t st)
 {
@@ -1006,12 +1027,23 @@ static hipError_t launch_corr_t(const PzlCorrArgs *a, const int *class_nrows, hi
             tables[dev] = t;
         }
     }
-    long long units = 0;
-    for (int c = 0; c < a->nclass; c++) units += (class_nrows[c] + G - 1) / G;
-    if (units < 1) return hipSuccess;
+    long long rowunits = 0;
+    for (int c = 0; c < a->nclass; c++) rowunits += (class_nrows[c] + G - 1) / G;
+    if (rowunits < 1 || a->nb < 1) return hipSuccess;
+    // A class with few row groups (the short FFT lengths) splits the batch's
+    // rsteps over several units, so that the launch still has about
+    // kCorrWaves waves: its rows are then read once per block of rsteps
+    // instead of once per batch.
+    const long long waves = rowunits * (NT / 64);
+    const int nblk = (int)std::min<long long>(a->nb, std::max<long long>(1, (kCorrWaves + waves - 1) / waves));
+    const int rb = (a->nb + nblk - 1) / nblk;
+    const int nblk_used = (a->nb + rb - 1) / rb;
+    const long long units = rowunits * nblk_used;
     if (units > 0x7fffffffLL) return hipErrorInvalidValue;
     PzlCorrArgs a2 = *a;
     a2.total_items = units;
+    a2.rb = rb;
+    a2.nblk = nblk_used;
     a2.twtab = tables[dev];
     hipLaunchKernelGGL((k_corr<N, NT, G, PING, SH, PADSH, Rs...>), dim3((unsigned)units), dim3(NT), CP::smem(), st,
                        a2);
@@ -1038,7 +1070,6 @@ struct CorrVariant {
 #define PZL_CVP(N, NT, G, PING, PADSH, ...)                                                                   \
     {N, launch_corr_t<N, NT, G, PING, 0, PADSH, __VA_ARGS__>,                                                 \
      #N ":" #NT "t:" #G "r:" #PING ":pad" #PADSH ":" #__VA_ARGS__}
-
 // Candidates per FFT length; the first of each length is the default, the
 // autotuner may pick another on the device.
 static const CorrVariant g_corr_variants[] = {

@@ -69,9 +69,11 @@ struct Plan {
     long long dev_bytes = 0;  // device memory this plan has allocated and holds (the bank aside)
     char *d_arena = nullptr;  // one allocation holding every buffer below (spectrum and d_sums aside)
     long long fft_bins = 0;
-    // per batch, two sets: the correlation of one batch fills one set while
-    // the harmonic sum of the previous batch reads the other
-    static constexpr int kSets = 2;
+    // per batch, three sets: the correlation of one batch fills one set while
+    // the harmonic sum of the previous batch reads another, and the segments
+    // of the next batch are written into the third beside both, so that no
+    // batch's correlation waits for its segments
+    static constexpr int kSets = 3;
     float2 *d_segs[kSets] = {};
     long long seg_stride = 0;
     uint16_t *d_rinds[kSets] = {};
