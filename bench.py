@@ -383,9 +383,11 @@ def config3_bench(steps):
         if variants is None:
             plan.autotune(reps=2)
             variants = {f: plan.variant(f) for f in plan.fftlens()}
+            group = plan.corr_group()
         else:  # the tuned plan's kernel variants: the same arithmetic, so the same candidates
             for f, v in variants.items():
                 plan.set_variant(f, v)
+            plan.set_corr_group(*group)
         plan.set_graph(graph)
         plan.search(stats=False)
         t0 = time.perf_counter()
@@ -412,6 +414,16 @@ def measured_traffic(kernel):
         return t.get(kernel, {}).get("bytes_per_launch")
     except (OSError, ValueError):
         return None
+
+
+def corr_group_desc(plan):
+    """How a plan launches its correlations per batch."""
+    from prestozl_amd import binding as B
+    g, every = plan.corr_group()
+    if g < 0:
+        return "one launch per FFT length"
+    return ("every FFT length in one launch: " if every else "the shorter FFT lengths in one launch: ") + \
+        B.corr_groups()[g]
 
 
 def main():
@@ -445,6 +457,9 @@ def main():
                     help="with --dedisp-only: leave out the request-loop search of every trial")
     ap.add_argument("--sigma", type=float, default=5.0,
                     help="profiling runs: the search threshold (the headline search uses 5; a high one leaves almost no hits)")
+    ap.add_argument("--graph", type=int, default=-1,
+                    help="1: replay each full batch's segments and correlations from a captured graph, 0: launch "
+                         "them one by one, -1: the plan's default")
     ap.add_argument("--overlap", type=int, default=-1,
                     help="profiling runs: 0 = one stream, 1 = harmonic sums on a side stream, 2 = also the "
                          "shorter FFT lengths (the default)")
@@ -486,6 +501,8 @@ def main():
     tune = [] if args.no_tune else plan.autotune(reps=3)
     if args.overlap >= 0:
         plan.set_overlap(args.overlap)
+    if args.graph >= 0:
+        plan.set_graph(bool(args.graph))
     for _ in range(args.warmup):
         plan.search(lo, hi, stats=False)
     hip_sync()
@@ -499,9 +516,18 @@ def main():
     total = time.perf_counter() - t0
     cells = stats_all[0]["cells"]
     value = cells * args.steps / total
-    # dominant kernel: the correlation class with the most time
+    # dominant kernel: the correlation class with the most time.  Replayed
+    # graphs are timed as a whole: the per-kernel durations then come from one
+    # search with the same kernels launched one by one, right after
+    roof_stats, roof_note = stats_all, "kernel durations from the timed searches"
+    if any(st["graph_launches"] > 0 for st in stats_all):
+        plan.set_graph(False)
+        roof_stats = [plan.search(lo, hi, stats=True)[1]]
+        plan.set_graph(True)
+        roof_note = "kernel durations from one search launching the same kernels one by one after the timed " \
+                    "graph-replay searches"
     cls = {}
-    for st in stats_all:
+    for st in roof_stats:
         for c in st["classes"]:
             d = cls.setdefault(c["fftlen"], dict(ms=0.0, launches=0, bytes=0.0, flops=0.0, bank=0.0))
             d["ms"] += c["ms"]
@@ -509,8 +535,19 @@ def main():
             d["bytes"] += c["bytes"]
             d["flops"] += c["flops"]
             d["bank"] = c["bank_bytes"]
-    dom_len = max(cls, key=lambda k: cls[k]["ms"])
-    dom = cls[dom_len]
+    # a grouped launch holds the shorter lengths (or every length): one kernel
+    gid, every = plan.corr_group()
+    kern = {"k_corr<%d>" % k: v for k, v in cls.items() if v["launches"] > 0}
+    if gid >= 0:
+        longest = max(cls)
+        members = [k for k in cls if every or k != longest]
+        kern["k_corr_group"] = dict(ms=sum(st["ms_corr_group"] for st in roof_stats),
+                                    launches=sum(st["group_launches"] for st in roof_stats),
+                                    bytes=sum(cls[k]["bytes"] for k in members),
+                                    flops=sum(cls[k]["flops"] for k in members),
+                                    bank=sum(cls[k]["bank"] for k in members))
+    dom_name = max(kern, key=lambda k: kern[k]["ms"])
+    dom = kern[dom_name]
     avg_ms = dom["ms"] / dom["launches"]
     # algorithmic bytes per launch: the plane rows written and the segments read
     # by one launch, plus the class's kernel rows amortised over the search's
@@ -519,7 +556,7 @@ def main():
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     flops_per_launch = dom["flops"] / dom["launches"]
     tflops = flops_per_launch / (avg_ms * 1e-3) / 1e12
-    traffic = measured_traffic("k_corr<%d>" % dom_len)
+    traffic = measured_traffic(dom_name)
     out = {
         "metric": "f-fdot-fdotdot cells/s at zmax200/wmax300/16 harm; p50/p99 latency per search",
         "value": value,
@@ -540,7 +577,7 @@ def main():
                    "planes_MiB_per_rstep": g.plane_bytes_per_rstep / 2 ** 20},
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                        "samples": len(lat)},
-        "roofline": {"bound": "hbm", "kernel": "k_corr<%d>" % dom_len, "achieved": achieved,
+        "roofline": {"bound": "hbm", "kernel": dom_name, "achieved": achieved, "timing": roof_note,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic,
                      "avg_launch_ms": avg_ms, "alg_bytes_per_launch": bytes_per_launch,
@@ -557,14 +594,17 @@ def main():
         "search_roofline": {"bytes_alg_per_search": stats_all[0]["bytes_alg"],
                             "achieved_GBs": stats_all[0]["bytes_alg"] * args.steps / total / 1e9,
                             "frac": stats_all[0]["bytes_alg"] * args.steps / total / 1e9 / HBM_PEAK_GBS},
-        "kernel_ms_per_search": {"segments": np.mean([s["ms_segments"] for s in stats_all]),
-                                 "corr": np.mean([s["ms_corr"] for s in stats_all]),
-                                 "harmsum": np.mean([s["ms_sum"] for s in stats_all]),
+        "kernel_ms_per_search": {"segments": np.mean([s["ms_segments"] for s in roof_stats]),
+                                 "corr": np.mean([s["ms_corr"] for s in roof_stats]),
+                                 "harmsum": np.mean([s["ms_sum"] for s in roof_stats]),
                                  "host": np.mean([s["ms_host"] for s in stats_all]),
-                                 "by_fftlen": {str(k): v["ms"] / args.steps for k, v in sorted(cls.items())}},
+                                 "by_fftlen": {str(k): v["ms"] / len(roof_stats) for k, v in sorted(cls.items())},
+                                 "grouped": np.mean([s["ms_corr_group"] for s in roof_stats]),
+                                 "graph_replay": np.mean([s["ms_graph"] for s in stats_all])},
         "autotune": tune,
         "variants": {str(c["fftlen"]): B.lib().pzl_corr_variant_name(c["fftlen"], plan.variant(c["fftlen"])).decode()
                      for c in stats_all[0]["classes"]},
+        "corr_launches": corr_group_desc(plan),
         "candidates": len(cands),
         "hits": stats_all[0]["nhits"],
     }
@@ -590,18 +630,20 @@ def main():
         out["cpu2_noharmpolish"] = {"points": len(nhp_pts), "ms": tmax * 1e3, "points_per_s": len(nhp_pts) / tmax,
                                     "note": "optimize_accelcand -noharmpolish (max_rzw_arr per harmonic) of every "
                                             "search candidate's harmonics"}
-    # variants of the execution: graph replay of every full batch (on top of the
-    # default overlap), and everything in one stream without overlap
+    # variants of the execution: the other launch form of full batches (one
+    # by one when the timed searches replayed graphs, and the reverse), and
+    # everything in one stream without overlap
     if args.no_legs:
         args.no_plain, args.serve, args.dm_trials = True, 0, 0
-    plan.set_graph(not args.no_legs)
     if not args.no_legs:
-        plan.search(lo, hi, stats=False)  # captures
+        timed_graph = args.graph != 0  # the plans replay graphs by default
+        plan.set_graph(not timed_graph)
+        plan.search(lo, hi, stats=False)  # captures, when replaying
         ts = time.perf_counter()
         cg, _ = plan.search(lo, hi, stats=False)
         tg = time.perf_counter() - ts
-        out["graph_replay"] = {"value": cells / tg, "unit": "cells/s", "ms_per_search": tg * 1e3,
-                               "same_candidates": cg == cands}
+        out["eager_launches" if timed_graph else "graph_replay"] = {
+            "value": cells / tg, "unit": "cells/s", "ms_per_search": tg * 1e3, "same_candidates": cg == cands}
         plan.set_graph(False)
         plan.set_overlap(False)
         ts = time.perf_counter()
@@ -628,6 +670,7 @@ def main():
         plan1.set_spectrum(fft)
         for c in stats_all[0]["classes"]:  # the same kernel variants: identical arithmetic
             plan1.set_variant(c["fftlen"], plan.variant(c["fftlen"]))
+        plan1.set_corr_group(*plan.corr_group())
         ts = time.perf_counter()
         c1, _ = plan1.search(stats=False)
         out["plain_gpu"] = {"value": cells / (time.perf_counter() - ts), "unit": "cells/s", "batchsize": 1,

@@ -96,6 +96,9 @@ typedef struct PzlStats {
     double class_flops[16];       /* nominal FP32 operations of the class's launches:
                                      5 N log2 N per inverse FFT, 6 N per product with
                                      conj(kernel), 3 per stored power */
+    double ms_corr_group;         /* summed duration of the grouped correlation launches (the
+                                     class_ms of the lengths they hold are then 0) */
+    long long group_launches;
 } PzlStats;
 
 const char *pzl_last_error(void);
@@ -150,9 +153,11 @@ int pzl_plan_set_deadline(void *plan, double ms);
  * same shape on the device.  report (optional) receives one line per variant.
  * Replaces the reference's per-shape cuFFT plan choice (accel_kernels.cu:96-158). */
 int pzl_autotune(void *plan, int reps, char *report, int report_len);
-/* Graph replay: every full batch of rsteps (segments, all correlation classes,
- * harmonic sum) is launched as one captured hipGraph per hit slot, the batch
- * index read from device memory.  Off by default.  Per-kernel timing is then
+/* Graph replay: the segments and correlations of every full batch of rsteps
+ * are launched as one captured hipGraph (one per buffer set, captured on the
+ * set's first full batch), the batch's first rstep read from device memory;
+ * the harmonic sum stays a launch of its own, beside the next batch.  On by
+ * default; on = 0 launches every kernel one by one.  Per-kernel timing is then
  * reported per batch (ms_graph) instead of per kernel. */
 int pzl_plan_set_graph(void *plan, int on);
 /* Overlap: the harmonic sum of a batch runs on its own stream, concurrently
@@ -173,6 +178,16 @@ int pzl_tune_batchsize(const PzlConfig *cfg, int device, const float *host_fft, 
  * it. */
 int pzl_plan_sum_form(void *plan, int *order, int *rpt);
 int pzl_plan_set_sum_form(void *plan, int order, int rpt);
+/* Correlation launches per batch (chosen by pzl_autotune): one per FFT length
+ * (group -1), or one grouped launch of correlation group 'group' for the
+ * shorter lengths (all = 0; the longest keeps its own launch) or for every
+ * length (all = 1: one correlation launch per batch).  A group holds one
+ * kernel variant per length; it must hold every length it is given.  Results
+ * do not depend on it beyond the variants' rounding. */
+int pzl_plan_corr_group(void *plan, int *group, int *all);
+int pzl_plan_set_corr_group(void *plan, int group, int all);
+int pzl_corr_num_groups(void);
+const char *pzl_corr_group_name(int group);
 /* The variant in use for an FFT length of the plan; set one explicitly. */
 int pzl_plan_variant(void *plan, int fftlen, int *variant);
 int pzl_plan_set_variant(void *plan, int fftlen, int variant);

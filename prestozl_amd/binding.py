@@ -56,7 +56,8 @@ class Stats(C.Structure):
                 ("bytes_alg_corr", C.c_double), ("nclass", C.c_int), ("class_fftlen", C.c_int * 16),
                 ("class_ms", C.c_double * 16), ("class_launches", C.c_longlong * 16),
                 ("class_bytes", C.c_double * 16), ("ms_graph", C.c_double), ("graph_launches", C.c_longlong),
-                ("class_bank_bytes", C.c_double * 16), ("class_flops", C.c_double * 16)]
+                ("class_bank_bytes", C.c_double * 16), ("class_flops", C.c_double * 16),
+                ("ms_corr_group", C.c_double), ("group_launches", C.c_longlong)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_ if not k.startswith("class_")}
@@ -163,6 +164,10 @@ def lib():
         L.pzl_tune_batchsize.argtypes = [P(Config), C.c_int, P(C.c_float), C.c_longlong, P(C.c_int), C.c_int,
                                          C.c_longlong, P(C.c_int), P(C.c_double)]
         L.pzl_plan_set_variant.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.pzl_plan_corr_group.argtypes = [C.c_void_p, P(C.c_int), P(C.c_int)]
+        L.pzl_plan_set_corr_group.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.pzl_corr_group_name.argtypes = [C.c_int]
+        L.pzl_corr_group_name.restype = C.c_char_p
         L.pzl_plan_sum_form.argtypes = [C.c_void_p, P(C.c_int), P(C.c_int)]
         L.pzl_optimize.argtypes = [C.c_void_p, P(OptCand), C.c_longlong]
         L.pzl_optimize_spectrum.argtypes = [C.c_int, P(C.c_float), C.c_longlong, P(OptCand), C.c_longlong]
@@ -254,6 +259,12 @@ class DeviceArray:
 
 def downsamp_data_array(d_out, d_in, worklen, bs, nsub):
     check(lib().pzl_downsamp_data_array(C.c_void_p(d_out), C.c_void_p(d_in), worklen, bs, nsub))
+
+
+def corr_groups():
+    """Names of the correlation groups (grouped launches of several FFT lengths)."""
+    L = lib()
+    return [L.pzl_corr_group_name(g).decode() for g in range(L.pzl_corr_num_groups())]
 
 
 def get_dedisp(d_cur, d_last, d_out, worklen, nsub, numdms, d_offsets, out_stride=None):
@@ -417,6 +428,16 @@ class Plan:
 
     def set_variant(self, fftlen, v):
         check(lib().pzl_plan_set_variant(self.h, int(fftlen), int(v)))
+
+    def corr_group(self):
+        """(group, all): the correlation group of the shorter FFT lengths (all
+        False) or of every length (all True); group -1: one launch per length."""
+        g, a = C.c_int(0), C.c_int(0)
+        check(lib().pzl_plan_corr_group(self.h, C.byref(g), C.byref(a)))
+        return g.value, bool(a.value)
+
+    def set_corr_group(self, group, all=False):
+        check(lib().pzl_plan_set_corr_group(self.h, int(group), int(bool(all))))
 
     def set_spectrum_device(self, ptr, numbins):
         check(lib().pzl_set_spectrum_device(self.h, C.c_void_p(ptr), numbins))

@@ -38,6 +38,16 @@
 
 using namespace pzl;
 
+namespace pzl {
+void relax_capture_mode()
+{
+    static thread_local bool done = false;
+    if (done) return;
+    hipStreamCaptureMode m = hipStreamCaptureModeRelaxed;
+    done = hipThreadExchangeStreamCaptureMode(&m) == hipSuccess;
+}
+}  // namespace pzl
+
 namespace {
 
 const long long kDefaultHitCap = 1LL << 26;  // hits per batch slot (12 bytes each)
@@ -237,6 +247,40 @@ int tuned_variant(const Plan &p, const SizeClass &c)
     std::lock_guard<std::mutex> lk(g_tune_mu);
     auto it = g_tuned.find(tune_key(p, c));
     return it == g_tuned.end() ? 0 : it->second;
+}
+
+// ... and the grouped launch (-1: one launch per length; g: the shorter
+// lengths in group g; g + ngroups: every length in group g), per (device,
+// batch size, every class's length and row counts).
+std::map<std::vector<long long>, int> g_tuned_group;
+
+std::vector<long long> group_key(const Plan &p)
+{
+    std::vector<long long> k{p.device, p.B};
+    for (auto &c : p.classes) {
+        k.push_back(-c.fftlen);
+        for (int n : c.nrows) k.push_back(n);
+    }
+    return k;
+}
+
+int tuned_group(const Plan &p)
+{
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tuned_group.find(group_key(p));
+    return it == g_tuned_group.end() ? -1 : it->second;
+}
+
+// Whether group g holds every FFT length of the plan but the longest (all:
+// the longest too).
+bool group_covers(const Plan &p, int g, bool all)
+{
+    if (g < 0 || g >= pzl_corr_num_groups() || p.classes.size() < 2) return false;
+    int longest = 0;
+    for (auto &c : p.classes) longest = std::max(longest, c.fftlen);
+    for (auto &c : p.classes)
+        if ((all || c.fftlen != longest) && !pzl_corr_group_has(g, c.fftlen)) return false;
+    return true;
 }
 
 // Every pinned host allocation of the engine, so that each asynchronous copy
@@ -464,7 +508,29 @@ int launch_corr_part(Plan &p, long long r0, int nb, int set, Timers *tm, const l
     size_t main_k = 0;
     for (size_t k = 1; k < p.classes.size(); k++)
         if (p.classes[k].fftlen > p.classes[main_k].fftlen) main_k = k;
+    const bool grouped = p.side_group >= 0 && p.classes.size() > 1;
+    if (grouped) {
+        // the shorter lengths (with group_all: every length, on the main
+        // stream) in one grouped launch, longest first
+        hipStream_t cs = side && !p.group_all ? ss : p.st;
+        std::vector<PzlCorrArgs> args;
+        std::vector<const PzlCorrArgs *> argp;
+        std::vector<int> lens;
+        std::vector<const int *> rows;
+        args.reserve(p.classes.size());
+        for (size_t k = 0; k < p.classes.size(); k++) {
+            if (k == main_k && !p.group_all) continue;
+            const SizeClass &c = p.classes[k];
+            args.push_back(corr_args(p, c, nb, set));
+            lens.push_back(c.fftlen);
+            rows.push_back(c.nrows.data());
+        }
+        for (auto &a : args) argp.push_back(&a);
+        Scope s(tm, 1, 0, cs);
+        PZL_HIPCHK(pzl_launch_corr_group(p.side_group, (int)args.size(), lens.data(), argp.data(), rows.data(), cs));
+    }
     for (size_t k = 0; k < p.classes.size(); k++) {
+        if (grouped && (k != main_k || p.group_all)) continue;
         const SizeClass &c = p.classes[k];
         hipStream_t cs = (side && k != main_k) ? ss : p.st;
         const PzlCorrArgs ca = corr_args(p, c, nb, set);
@@ -533,26 +599,34 @@ int drop_graphs(Plan &p)
 // The correlation part of a full batch as one graph launch: the graph of a
 // buffer set (segments and every correlation class) is captured once, with the
 // batch's first rstep read from device memory, and replayed for every later
-// full batch in that set; the first batch of each set runs eagerly.
+// full batch in that set; the first batch of each set runs eagerly.  A capture
+// that fails (the runtime can invalidate one when another thread makes a call
+// it counts as unsafe meanwhile) is dropped: the batch runs eagerly and a
+// later batch of the set captures again; after kMaxCaptureFailures the plan
+// launches eagerly for good.
 int launch_corr_graph(Plan &p, long long r0, int set, Timers *tm)
 {
     if (!p.gexec[set]) {
         int rc = launch_corr_part(p, r0, p.B, set, tm, nullptr);
         if (rc) return rc;
         hipGraph_t g = nullptr;
-        PZL_HIPCHK(hipStreamBeginCapture(p.st, hipStreamCaptureModeThreadLocal));
+        const hipError_t eb = hipStreamBeginCapture(p.st, hipStreamCaptureModeThreadLocal);
+        if (eb != hipSuccess) {
+            (void)hipGetLastError();
+            if (++p.capture_failures >= Plan::kMaxCaptureFailures) p.use_graph = false;
+            return 0;
+        }
         rc = launch_corr_part(p, 0, p.B, set, nullptr, p.d_r0 + set);
         const hipError_t e = hipStreamEndCapture(p.st, &g);
-        if (rc) {
-            if (g) (void)hipGraphDestroy(g);
-            return rc;
-        }
-        if (e != hipSuccess) return fail(PZL_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
-        const hipError_t ei = hipGraphInstantiate(&p.gexec[set], g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
+        hipError_t ei = hipErrorUnknown;
+        if (rc == 0 && e == hipSuccess && g) ei = hipGraphInstantiate(&p.gexec[set], g, nullptr, nullptr, 0);
+        if (g) (void)hipGraphDestroy(g);
         if (ei != hipSuccess) {
+            // nothing of the capture ran: the batch is already done eagerly above
             p.gexec[set] = nullptr;
-            return fail(PZL_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(ei));
+            (void)hipGetLastError();
+            pzl::g_err.clear();
+            if (++p.capture_failures >= Plan::kMaxCaptureFailures) p.use_graph = false;
         }
         return 0;
     }
@@ -701,6 +775,7 @@ struct SearchState {
     }
     void run()
     {
+        relax_capture_mode();
         for (;;) {
             std::pair<int, long long> j;
             Plan::HostBuf hb;
@@ -897,6 +972,7 @@ int pzl_geometry_from_config(const PzlConfig *cfg, PzlGeometry *g)
 
 int pzl_plan_create(const PzlConfig *cfg, int device, void **plan_out)
 {
+    const CaptureRelaxed relaxed;
     if (!cfg || !plan_out) return fail(PZL_ERR_ARG, "null argument");
     *plan_out = nullptr;
     Plan *p = new Plan();
@@ -991,6 +1067,11 @@ int pzl_plan_create(const PzlConfig *cfg, int device, void **plan_out)
         for (int h : c.harms) c.nrows.push_back(p->harms[h].nrows);
         c.variant = tuned_variant(*p, c);
         p->classes.push_back(c);
+    }
+    {
+        const int t = tuned_group(*p), ng = pzl_corr_num_groups();
+        p->side_group = t < 0 ? -1 : t % ng;
+        p->group_all = t >= ng;
     }
     p->hitcap = sizes.hitcap;
     const long long maxkey = std::max<long long>(p->g.nrsteps, 1) * p->g.numharmstages * F.numw * F.numz *
@@ -1113,7 +1194,11 @@ int pzl_plan_create(const PzlConfig *cfg, int device, void **plan_out)
     return 0;
 }
 
-void pzl_plan_destroy(void *plan) { free_plan((Plan *)plan); }
+void pzl_plan_destroy(void *plan)
+{
+    const CaptureRelaxed relaxed;
+    free_plan((Plan *)plan);
+}
 
 int pzl_plan_device_bytes(void *plan, long long *held)
 {
@@ -1150,6 +1235,7 @@ int pzl_plan_harm(void *plan, int idx, PzlHarmInfo *h)
 
 int pzl_set_spectrum(void *plan, const float *host_fft, long long numbins)
 {
+    const CaptureRelaxed relaxed;
     Plan *p = (Plan *)plan;
     if (!p || !host_fft || numbins < 1) return fail(PZL_ERR_ARG, "bad spectrum");
     PZL_HIPCHK(hipSetDevice(p->device));
@@ -1167,6 +1253,7 @@ int pzl_set_spectrum(void *plan, const float *host_fft, long long numbins)
 
 int pzl_set_spectrum_device(void *plan, const void *dev_fft, long long numbins)
 {
+    const CaptureRelaxed relaxed;
     Plan *p = (Plan *)plan;
     if (!p || !dev_fft || numbins < 1) return fail(PZL_ERR_ARG, "bad spectrum");
     p->d_fft = (const float2 *)dev_fft;
@@ -1177,6 +1264,7 @@ int pzl_set_spectrum_device(void *plan, const void *dev_fft, long long numbins)
 int pzl_search(void *plan, long long rstep_lo, long long rstep_hi, PzlCand *cands, long long maxcands,
                long long *ncands, PzlStats *stats)
 {
+    const CaptureRelaxed relaxed;
     Plan *p = (Plan *)plan;
     if (!p) return fail(PZL_ERR_ARG, "null plan");
     if (!p->d_fft) return fail(PZL_ERR_NOSPECTRUM, "no spectrum set");
@@ -1281,8 +1369,12 @@ int pzl_search(void *plan, long long rstep_lo, long long rstep_hi, PzlCand *cand
                 stats->ms_segments += ms;
             } else if (r.kind == 1) {
                 stats->ms_corr += ms;
-                per_class[r.fftlen] += ms;
-                per_class_n[r.fftlen]++;
+                if (r.fftlen == 0) {
+                    stats->ms_corr_group += ms;  // grouped launch of the shorter lengths
+                } else {
+                    per_class[r.fftlen] += ms;
+                    per_class_n[r.fftlen]++;
+                }
                 stats->corr_launches++;
             } else if (r.kind == 2) {
                 stats->ms_sum += ms;
@@ -1309,9 +1401,12 @@ int pzl_search(void *plan, long long rstep_lo, long long rstep_hi, PzlCand *cand
         stats->ms_corr *= scale;
         stats->ms_sum *= scale;
         stats->ms_graph *= scale;
+        stats->ms_corr_group *= scale;
         for (auto &kv : per_class) kv.second *= scale;
         for (auto &kv : per_class_n) kv.second = i;
-        stats->corr_launches = (long long)i * (long long)p->classes.size();
+        const bool grouped = p->side_group >= 0 && p->classes.size() > 1;
+        stats->group_launches = grouped ? i : 0;
+        stats->corr_launches = (long long)i * (grouped ? (p->group_all ? 1LL : 2LL) : (long long)p->classes.size());
         stats->graph_launches = (long long)std::llround(stats->graph_launches * scale);
         for (auto &kv : per_class) stats->ms_corr_max_class = std::max(stats->ms_corr_max_class, kv.second);
         stats->nclass = 0;
@@ -1457,6 +1552,8 @@ int pzl_autotune(void *plan, int reps, char *report, int report_len)
         for (size_t i = 0; i < lead_times.size() && i < 5; i++) {
             if (lead_times[i].first > 1.08f * lead_times[0].first) break;
             c.variant = lead_times[i].second;
+            rc = drop_graphs(*p);  // captured graphs hold the variants of their capture
+            if (rc) return rc;
             float ms = 1e30f;
             for (int k = 0; k < 2; k++) {
                 long long n = 0;
@@ -1475,6 +1572,8 @@ int pzl_autotune(void *plan, int reps, char *report, int report_len)
             }
         }
         c.variant = best;
+        rc = drop_graphs(*p);
+        if (rc) return rc;
         std::lock_guard<std::mutex> lk(g_tune_mu);
         g_tuned[tune_key(*p, c)] = best;
     }
@@ -1508,6 +1607,56 @@ int pzl_autotune(void *plan, int reps, char *report, int report_len)
         }
         p->sum_order = best_order;
         p->sum_rpt = best_rpt;
+    }
+    // one launch per length, the shorter lengths in one grouped launch, or
+    // every length in one: each form a group allows is timed on the same band
+    // (best of three searches), and a form replaces the one kept so far only
+    // when it is faster by more than the timing noise (0.5%)
+    if (p->classes.size() >= 2 && nr >= band) {
+        const long long lo = (nr - band) / 2;
+        const int ng = pzl_corr_num_groups();
+        std::vector<int> forms{-1};  // as g_tuned_group
+        for (int all = 0; all < 2; all++)
+            for (int g = 0; g < ng; g++)
+                if (group_covers(*p, g, all != 0)) forms.push_back(g + all * ng);
+        if (forms.size() > 1) {
+            float best_ms = 1e30f;
+            int best = -1;
+            for (int f : forms) {
+                p->side_group = f < 0 ? -1 : f % ng;
+                p->group_all = f >= ng;
+                rc = drop_graphs(*p);
+                if (rc) return rc;
+                float ms = 1e30f;
+                for (int k = 0; k < 3; k++) {
+                    long long n = 0;
+                    const auto t0 = std::chrono::steady_clock::now();
+                    rc = pzl_search(plan, lo, lo + band, nullptr, 0, &n, nullptr);
+                    if (rc) {
+                        p->side_group = -1;
+                        p->group_all = false;
+                        return rc;
+                    }
+                    ms = std::min(ms, (float)std::chrono::duration<double, std::milli>(
+                                          std::chrono::steady_clock::now() - t0)
+                                          .count());
+                }
+                snprintf(line, sizeof line, "%s%s in a search band of %lld rsteps: %.3f ms\n",
+                         f < 0 ? "one launch per FFT length" : (f >= ng ? "every length in " : "shorter lengths in "),
+                         f < 0 ? "" : pzl_corr_group_name(f % ng), band, ms);
+                rep += line;
+                if (ms < 0.995f * best_ms) {
+                    best_ms = ms;
+                    best = f;
+                }
+            }
+            p->side_group = best < 0 ? -1 : best % ng;
+            p->group_all = best >= ng;
+            rc = drop_graphs(*p);
+            if (rc) return rc;
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            g_tuned_group[group_key(*p)] = best;
+        }
     }
     if (report && report_len > 0) {
         snprintf(report, (size_t)report_len, "%s", rep.c_str());
@@ -1579,6 +1728,35 @@ int pzl_tune_batchsize(const PzlConfig *cfg, int device, const float *host_fft, 
             *best = candidates[k];
         }
     }
+    return 0;
+}
+
+int pzl_plan_corr_group(void *plan, int *group, int *all)
+{
+    Plan *p = (Plan *)plan;
+    if (!p) return fail(PZL_ERR_ARG, "null plan");
+    if (group) *group = p->side_group;
+    if (all) *all = p->side_group >= 0 && p->group_all;
+    return 0;
+}
+
+int pzl_plan_set_corr_group(void *plan, int group, int all)
+{
+    Plan *p = (Plan *)plan;
+    if (!p) return fail(PZL_ERR_ARG, "null plan");
+    if (group >= 0 && !group_covers(*p, group, all != 0))
+        return fail(PZL_ERR_ARG, "correlation group %d does not hold every %sFFT length of this plan", group,
+                    all ? "" : "shorter ");
+    if (group < 0) {
+        group = -1;
+        all = 0;
+    }
+    if (group != p->side_group || (all != 0) != p->group_all) {
+        int rc = drop_graphs(*p);
+        if (rc) return rc;
+    }
+    p->side_group = group;
+    p->group_all = all != 0;
     return 0;
 }
 

@@ -103,6 +103,15 @@ hipError_t pzl_launch_segments(const PzlBatchArgs *a, int max_fftlen, hipStream_
 hipError_t pzl_launch_corr(int N, int variant, const PzlCorrArgs *a, const int *class_nrows, hipStream_t st);
 int pzl_corr_num_variants(int n);
 const char *pzl_corr_variant_name(int n, int v);
+// Correlation of several FFT-length classes in one launch of group 'group':
+// class c has length fftlens[c], arguments *args[c] and host row counts
+// class_nrows[c]; a length the group does not hold, or one given twice, is an
+// error.
+hipError_t pzl_launch_corr_group(int group, int ncls, const int *fftlens, const PzlCorrArgs *const *args,
+                                 const int *const *class_nrows, hipStream_t st);
+int pzl_corr_num_groups(void);
+const char *pzl_corr_group_name(int group);
+int pzl_corr_group_has(int group, int n);
 hipError_t pzl_launch_harmsum(const PzlSumArgs *a, hipStream_t st);
 // Radix sort of n (key, power) pairs on 'end_bit' key bits (pzl_sort.hip):
 // from (keys, pows) into (keys_alt, pows_alt), both pairs overwritten;

@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
 #include <mutex>
 
 
@@ -374,15 +375,16 @@ template <int R> struct ExpTab {
 // with P_0 = pdata (K*[k] + K*[k+N/2]) and P_1 = pdata (K*[k] - K*[k+N/2])
 // exp(2 pi i k/N): the first radix-2 stage of the N-point transform done on the
 // kernel row once per batch instead of per rstep.
+// Work units u0, u0 + ustep, ... (< a.total_items) of one correlation
+// variant, by the NT threads of a workgroup; smem holds CP::smem() bytes.
 template <int N, int NT, int G, bool PING, int SH, int PADSH, int... Rs>
-__global__ __launch_bounds__(NT) void k_corr(PzlCorrArgs a)
+__device__ __forceinline__ void corr_units(const PzlCorrArgs &a, long long u0, long long ustep, float2 *smem)
 {
     constexpr int NF = N >> SH;  // FFT length
     constexpr int GF = G << SH;  // FFTs per workgroup
     using F = CorrFFT<NF, NT, GF, PING, SH, PADSH, Rs...>;
     using CP = typename F::CP;
     constexpr int R0 = CP::RAD[0], NB0 = NF / R0;
-    extern __shared__ float2 smem[];
     float2 *b0 = smem, *b1 = PING ? smem + CP::BUF : smem;
     const int tid = threadIdx.x;
     // power normalisation 1 / N^2 (accel_utils.c:1234, 1278-1282); for power-of-two N
@@ -407,7 +409,7 @@ __global__ __launch_bounds__(NT) void k_corr(PzlCorrArgs a)
         }
     }
 
-    for (long long unit = blockIdx.x; unit < a.total_items; unit += gridDim.x) {
+    for (long long unit = u0; unit < a.total_items; unit += ustep) {
         // the unit's rows (per harmonic of the class, ceil(nrows / G) row groups) and
         // its block of rsteps: the blocks of one row group are adjacent units
         const int ru = (int)(unit / a.nblk), blk = (int)(unit - (long long)ru * a.nblk);
@@ -533,6 +535,77 @@ __global__ __launch_bounds__(NT) void k_corr(PzlCorrArgs a)
             if constexpr (!PING || (CP::NP % 2 == 0)) __syncthreads();
         }
     }
+}
+
+template <int N, int NT, int G, bool PING, int SH, int PADSH, int... Rs>
+__global__ __launch_bounds__(NT) void k_corr(PzlCorrArgs a)
+{
+    extern __shared__ float2 smem[];
+    corr_units<N, NT, G, PING, SH, PADSH, Rs...>(a, blockIdx.x, gridDim.x, smem);
+}
+
+// One correlation variant as a type: its launch geometry, twiddle table and
+// work units, for launches of one variant and for grouped launches.
+template <int N_, int NT_, int G_, bool PING, int SH, int PADSH, int... Rs> struct CorrV {
+    static constexpr int N = N_, NT = NT_, G = G_;
+    using CP = CorrPlan<(N >> SH), NT, (G << SH), PING, PADSH, Rs...>;
+    static constexpr size_t smem() { return CP::smem(); }
+    static __device__ __forceinline__ void unit(const PzlCorrArgs &a, long long u, float2 *sm)
+    {
+        corr_units<N, NT, G, PING, SH, PADSH, Rs...>(a, u, a.total_items, sm);
+    }
+    // The variant's twiddles ([TWN][NT], see k_corr_twiddles), built on the
+    // current device on first use (never inside a stream capture: the first
+    // batch of every plan runs eagerly) and kept until the process ends.
+    static hipError_t table(hipStream_t st, const float2 **out)
+    {
+        static std::mutex mu;
+        static float2 *tables[64] = {};
+        int dev = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e != hipSuccess) return e;
+        if (dev < 0 || dev >= 64) return hipErrorInvalidDevice;
+        std::lock_guard<std::mutex> lk(mu);
+        if (!tables[dev]) {
+            float2 *t = nullptr;
+            e = hipMalloc(&t, sizeof(float2) * (size_t)(CP::TWN > 0 ? CP::TWN : 1) * NT);
+            if (e != hipSuccess) return e;
+            hipLaunchKernelGGL((k_corr_twiddles<N, NT, G, PING, SH, PADSH, Rs...>), dim3(1), dim3(NT), 0, st, t);
+            e = hipGetLastError();
+            if (e == hipSuccess) e = hipStreamSynchronize(st);
+            if (e != hipSuccess) {
+                (void)hipFree(t);
+                return e;
+            }
+            tables[dev] = t;
+        }
+        *out = tables[dev];
+        return hipSuccess;
+    }
+};
+
+// Waves a correlation launch aims for at least (two rounds of 4 waves per SIMD).
+constexpr long long kCorrWaves = 2 * 4 * 1024;
+
+// Work units of one class for a variant of NT threads and G rows per
+// workgroup: a class with few row groups (the short FFT lengths) splits the
+// batch's rsteps over several units, so that the launch still has about
+// kCorrWaves waves; its rows are then read once per block of rsteps instead
+// of once per batch.  Returns false when the units exceed 2^31 - 1.
+static bool corr_units_of(const PzlCorrArgs &a, const int *class_nrows, int NT, int G, PzlCorrArgs &out)
+{
+    long long rowunits = 0;
+    for (int c = 0; c < a.nclass; c++) rowunits += (class_nrows[c] + G - 1) / G;
+    out = a;
+    out.total_items = 0;
+    out.rb = out.nblk = 1;
+    if (rowunits < 1 || a.nb < 1) return true;
+    const long long waves = rowunits * (NT / 64);
+    const int nblk = (int)std::min<long long>(a.nb, std::max<long long>(1, (kCorrWaves + waves - 1) / waves));
+    out.rb = (a.nb + nblk - 1) / nblk;
+    out.nblk = (a.nb + out.rb - 1) / out.rb;
+    out.total_items = rowunits * out.nblk;
+    return out.total_items <= 0x7fffffffLL;
 }
 
 // ---------------------------------------------------------------------------
@@ -993,61 +1066,161 @@ extern "C" hipError_t pzl_launch_segments(const PzlBatchArgs *a, int max_fftlen,
 
 // One correlation variant: kernel-row length N, NT threads, G rows per
 // workgroup, ping-pong LDS or not, split rows or not, radices.  One workgroup
-// per G rows of a harmonic (the grid is not persistent, so that kernels of
-// concurrent searches interleave); its twiddles come from a table built once
-// per device.
-// Waves a correlation launch aims for at least (two rounds of 4 waves per SIMD).
-constexpr long long kCorrWaves = 2 * 4 * 1024;
-
+// per work unit (the grid is not persistent, so that kernels of concurrent
+// searches interleave); its twiddles come from a table built once per device.
 template <int N, int NT, int G, bool PING, int SH, int PADSH, int... Rs>
 static hipError_t launch_corr_t(const PzlCorrArgs *a, const int *class_nrows, hipStream_t st)
 {
-    using CP = CorrPlan<(N >> SH), NT, (G << SH), PING, PADSH, Rs...>;
-    static std::mutex mu;
-    static float2 *tables[64] = {};
-    int dev = 0;
-    hipError_t e = hipGetDevice(&dev);
+    using V = CorrV<N, NT, G, PING, SH, PADSH, Rs...>;
+    PzlCorrArgs a2;
+    if (!corr_units_of(*a, class_nrows, NT, G, a2)) return hipErrorInvalidValue;
+    if (a2.total_items < 1) return hipSuccess;
+    hipError_t e = V::table(st, &a2.twtab);
     if (e != hipSuccess) return e;
-    if (dev < 0 || dev >= 64) return hipErrorInvalidDevice;
-    {
-        std::lock_guard<std::mutex> lk(mu);
-        if (!tables[dev]) {
-            // built on first use (never inside a stream capture: the first
-            // batch of every plan runs eagerly), kept until the process ends
-            float2 *t = nullptr;
-            e = hipMalloc(&t, sizeof(float2) * (size_t)(CP::TWN > 0 ? CP::TWN : 1) * NT);
-            if (e != hipSuccess) return e;
-            hipLaunchKernelGGL((k_corr_twiddles<N, NT, G, PING, SH, PADSH, Rs...>), dim3(1), dim3(NT), 0, st, t);
-            e = hipGetLastError();
-            if (e == hipSuccess) e = hipStreamSynchronize(st);
-            if (e != hipSuccess) {
-                (void)hipFree(t);
-                return e;
-            }
-            tables[dev] = t;
-        }
-    }
-    long long rowunits = 0;
-    for (int c = 0; c < a->nclass; c++) rowunits += (class_nrows[c] + G - 1) / G;
-    if (rowunits < 1 || a->nb < 1) return hipSuccess;
-    // A class with few row groups (the short FFT lengths) splits the batch's
-    // rsteps over several units, so that the launch still has about
-    // kCorrWaves waves: its rows are then read once per block of rsteps
-    // instead of once per batch.
-    const long long waves = rowunits * (NT / 64);
-    const int nblk = (int)std::min<long long>(a->nb, std::max<long long>(1, (kCorrWaves + waves - 1) / waves));
-    const int rb = (a->nb + nblk - 1) / nblk;
-    const int nblk_used = (a->nb + rb - 1) / rb;
-    const long long units = rowunits * nblk_used;
-    if (units > 0x7fffffffLL) return hipErrorInvalidValue;
-    PzlCorrArgs a2 = *a;
-    a2.total_items = units;
-    a2.rb = rb;
-    a2.nblk = nblk_used;
-    a2.twtab = tables[dev];
-    hipLaunchKernelGGL((k_corr<N, NT, G, PING, SH, PADSH, Rs...>), dim3((unsigned)units), dim3(NT), CP::smem(), st,
-                       a2);
+    hipLaunchKernelGGL((k_corr<N, NT, G, PING, SH, PADSH, Rs...>), dim3((unsigned)a2.total_items), dim3(NT),
+                       V::smem(), st, a2);
     return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Grouped correlation: the classes of several FFT lengths in one launch
+// ---------------------------------------------------------------------------
+//
+// A group is a list of variants with one thread count, one per FFT length,
+// longest first.  The launch holds the work units of every class the caller
+// passes, class after class in the group's order: each workgroup finds its
+// class from its index (uniform, so the whole workgroup takes one variant's
+// code) and runs that unit.  The short lengths, which alone fill a small part
+// of the chip, then share one launch and its tail with the longer ones.
+
+#define PZL_CORR_GROUP_MAX 10
+
+struct PzlCorrGroupArgs {
+    PzlCorrArgs c[PZL_CORR_GROUP_MAX];    // per member: its class (total_items 0: absent)
+    long long begin[PZL_CORR_GROUP_MAX + 1];  // first workgroup of each member
+};
+
+template <int K, typename V, typename... Vs>
+__device__ __forceinline__ void group_unit(const PzlCorrGroupArgs &g, long long wg, float2 *sm)
+{
+    if (wg < g.begin[K + 1]) {
+        V::unit(g.c[K], wg - g.begin[K], sm);
+        return;
+    }
+    if constexpr (sizeof...(Vs) > 0) group_unit<K + 1, Vs...>(g, wg, sm);
+}
+
+// WMIN: waves per SIMD the registers must allow (1: the compiler's choice)
+template <int NT, int WMIN, typename... Vs>
+__global__ __launch_bounds__(NT, WMIN) void k_corr_group(PzlCorrGroupArgs g)
+{
+    extern __shared__ float2 smem[];
+    group_unit<0, Vs...>(g, blockIdx.x, smem);
+}
+
+template <int NT, int WMIN, typename... Vs> struct CorrGroup {
+    static constexpr int M = sizeof...(Vs);
+    static_assert(M <= PZL_CORR_GROUP_MAX, "too many members");
+    static_assert(((Vs::NT == NT) && ...), "members share the thread count");
+    static constexpr int lens[M] = {Vs::N...};
+    static int member(int n)
+    {
+        for (int k = 0; k < M; k++)
+            if (lens[k] == n) return k;
+        return -1;
+    }
+    // members K..: their classes' units (cls[k] < 0: absent), in order
+    template <int K, typename V, typename... Rest>
+    static hipError_t fill(PzlCorrGroupArgs &g, long long &acc, const int *cls, const PzlCorrArgs *const *args,
+                           const int *const *class_nrows, hipStream_t st)
+    {
+        g.begin[K] = acc;
+        g.c[K] = PzlCorrArgs{};
+        if (cls[K] >= 0) {
+            if (!corr_units_of(*args[cls[K]], class_nrows[cls[K]], NT, V::G, g.c[K])) return hipErrorInvalidValue;
+            if (g.c[K].total_items > 0) {
+                hipError_t e = V::table(st, &g.c[K].twtab);
+                if (e != hipSuccess) return e;
+            }
+            acc += g.c[K].total_items;
+        }
+        if constexpr (sizeof...(Rest) > 0) return fill<K + 1, Rest...>(g, acc, cls, args, class_nrows, st);
+        return hipSuccess;
+    }
+    static hipError_t launch(int ncls, const int *fftlens, const PzlCorrArgs *const *args,
+                             const int *const *class_nrows, hipStream_t st)
+    {
+        int cls[M];
+        for (int k = 0; k < M; k++) cls[k] = -1;
+        for (int c = 0; c < ncls; c++) {
+            const int k = member(fftlens[c]);
+            if (k < 0 || cls[k] >= 0) return hipErrorInvalidValue;  // not a member, or a length twice
+            cls[k] = c;
+        }
+        PzlCorrGroupArgs g{};
+        long long acc = 0;
+        hipError_t e = fill<0, Vs...>(g, acc, cls, args, class_nrows, st);
+        if (e != hipSuccess) return e;
+        for (int k = M; k <= PZL_CORR_GROUP_MAX; k++) g.begin[k] = acc;
+        if (acc < 1) return hipSuccess;
+        if (acc > 0x7fffffffLL) return hipErrorInvalidValue;
+        const size_t sm = std::max({Vs::smem()...});
+        hipLaunchKernelGGL((k_corr_group<NT, WMIN, Vs...>), dim3((unsigned)acc), dim3(NT), sm, st, g);
+        return hipGetLastError();
+    }
+    static int has(int n) { return member(n) >= 0; }
+};
+
+// The groups, each with a member for every FFT length from 128 to 2048 (the
+// lengths shorter than the longest of a search).
+using CorrGroup256 =
+    CorrGroup<256, 1, CorrV<2048, 256, 1, true, 0, 5, 8, 8, 4, 8>, CorrV<1280, 256, 2, true, 0, 4, 16, 16, 5>,
+              CorrV<1024, 256, 2, true, 0, 5, 8, 8, 4, 4>, CorrV<768, 256, 4, true, 0, 4, 12, 4, 4, 4>,
+              CorrV<512, 256, 4, true, 0, 4, 8, 8, 8>, CorrV<384, 256, 8, true, 0, 4, 6, 4, 4, 4>,
+              CorrV<256, 256, 8, true, 0, 4, 8, 8, 4>, CorrV<192, 256, 16, true, 0, 4, 12, 4, 4>,
+              CorrV<128, 256, 16, true, 0, 4, 8, 4, 4>>;
+
+// The same with fewer rows per workgroup for the lengths whose variants above
+// hold many values per thread: every member stays within 128 VGPRs, so the
+// group keeps 4 waves per SIMD.
+using CorrGroup256Lean =
+    CorrGroup<256, 4, CorrV<2048, 256, 1, true, 0, 5, 8, 8, 4, 8>, CorrV<1280, 256, 1, true, 0, 4, 4, 8, 8, 5>,
+              CorrV<1024, 256, 2, true, 0, 5, 8, 8, 4, 4>, CorrV<768, 256, 1, true, 0, 4, 3, 4, 4, 4, 4>,
+              CorrV<512, 256, 4, true, 0, 4, 8, 8, 8>, CorrV<384, 256, 4, true, 0, 4, 8, 8, 6>,
+              CorrV<256, 256, 8, true, 0, 4, 8, 8, 4>, CorrV<192, 256, 8, true, 0, 4, 8, 4, 6>,
+              CorrV<128, 256, 16, true, 0, 4, 8, 4, 4>>;
+
+struct CorrGroupEntry {
+    const char *name;
+    hipError_t (*launch)(int, const int *, const PzlCorrArgs *const *, const int *const *, hipStream_t);
+    int (*has)(int);
+};
+
+static const CorrGroupEntry g_corr_groups[] = {
+    {"256t: 2048:1r 1280:2r 1024:2r 768:4r 512:4r 384:8r 256:8r 192:16r 128:16r", CorrGroup256::launch,
+     CorrGroup256::has},
+    {"256t lean: 2048:1r 1280:1r 1024:2r 768:1r 512:4r 384:4r 256:8r 192:8r 128:16r", CorrGroup256Lean::launch,
+     CorrGroup256Lean::has},
+};
+static const int g_ncorr_groups = (int)(sizeof(g_corr_groups) / sizeof(g_corr_groups[0]));
+
+extern "C" int pzl_corr_num_groups(void) { return g_ncorr_groups; }
+
+extern "C" const char *pzl_corr_group_name(int group)
+{
+    return group >= 0 && group < g_ncorr_groups ? g_corr_groups[group].name : nullptr;
+}
+
+extern "C" int pzl_corr_group_has(int group, int n)
+{
+    return group >= 0 && group < g_ncorr_groups && g_corr_groups[group].has(n);
+}
+
+extern "C" hipError_t pzl_launch_corr_group(int group, int ncls, const int *fftlens, const PzlCorrArgs *const *args,
+                                            const int *const *class_nrows, hipStream_t st)
+{
+    if (group < 0 || group >= g_ncorr_groups || ncls < 0) return hipErrorInvalidValue;
+    return g_corr_groups[group].launch(ncls, fftlens, args, class_nrows, st);
 }
 
 typedef hipError_t (*CorrLaunch)(const PzlCorrArgs *, const int *, hipStream_t);
@@ -1077,15 +1250,18 @@ static const CorrVariant g_corr_variants[] = {
     PZL_CV(128, 256, 16, true, 8, 4, 4),
     PZL_CV(192, 256, 16, true, 12, 4, 4),
     PZL_CV(192, 64, 4, true, 12, 4, 4),
+    PZL_CV(192, 256, 8, true, 8, 4, 6),
     PZL_CV(256, 128, 8, true, 16, 16),
     PZL_CV(256, 256, 8, true, 8, 8, 4),
     PZL_CV(384, 256, 8, true, 6, 4, 4, 4),
     PZL_CV(384, 64, 2, true, 12, 4, 4, 2),
+    PZL_CV(384, 256, 4, true, 8, 8, 6),
     PZL_CV(512, 128, 4, true, 8, 8, 8),
     PZL_CV(512, 256, 4, true, 8, 8, 8),
     PZL_CV(512, 128, 4, true, 16, 16, 2),
     PZL_CV(768, 256, 4, true, 12, 4, 4, 4),
     PZL_CV(768, 64, 1, true, 12, 4, 4, 4),
+    PZL_CV(768, 256, 1, true, 3, 4, 4, 4, 4),
     PZL_CV(1024, 128, 2, true, 16, 16, 4),
     PZL_CV(1024, 256, 2, true, 8, 8, 4, 4),
     PZL_CV(1024, 64, 1, true, 16, 16, 4),
@@ -1095,6 +1271,7 @@ static const CorrVariant g_corr_variants[] = {
     PZL_CVP(1024, 256, 2, true, 5, 8, 4, 4, 8),
     PZL_CV(1280, 128, 1, true, 16, 16, 5),
     PZL_CV(1280, 256, 2, true, 16, 16, 5),
+    PZL_CV(1280, 256, 1, true, 4, 8, 8, 5),
     PZL_CVP(1280, 128, 1, true, 3, 16, 16, 5),
     PZL_CVP(1280, 128, 1, true, 5, 16, 16, 5),
     PZL_CVP(1280, 320, 2, true, 5, 8, 8, 4, 5),

@@ -15,6 +15,23 @@ namespace pzl {
 int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
 extern thread_local std::string g_err;
 
+// While a thread of the library captures a plan's graph, calls that the
+// runtime counts as unsafe during a capture (allocations, frees,
+// synchronisations) made by other threads must not invalidate it: the
+// library's own threads take the relaxed interaction mode for good
+// (relax_capture_mode), and calls into the library take it for their duration
+// (CaptureRelaxed), restoring the caller's mode on return.
+void relax_capture_mode();
+struct CaptureRelaxed {
+    hipStreamCaptureMode prev = hipStreamCaptureModeRelaxed;
+    bool ok = false;
+    CaptureRelaxed() { ok = hipThreadExchangeStreamCaptureMode(&prev) == hipSuccess; }
+    ~CaptureRelaxed()
+    {
+        if (ok) (void)hipThreadExchangeStreamCaptureMode(&prev);
+    }
+};
+
 #define PZL_HIPCHK(call)                                                                                \
     do {                                                                                                \
         hipError_t e_ = (call);                                                                         \
@@ -113,8 +130,10 @@ struct Plan {
     hipEvent_t ev_count[kSlots] = {};
     hipEvent_t ev_free[kSlots] = {};
     float *d_sums = nullptr;
-    // graph replay of full batches (one executable graph per hit slot)
-    bool use_graph = false;
+    // graph replay of full batches (one executable graph per buffer set)
+    bool use_graph = true;
+    static constexpr int kMaxCaptureFailures = 8;
+    int capture_failures = 0;  // captures the runtime refused (the batches ran eagerly)
     hipGraphExec_t gexec[kSets] = {};  // correlation part, per buffer set
     long long *d_r0 = nullptr;  // [kSets] first rstep of the batch in each set
     static constexpr int kR0Ring = 2 * kSlots;
@@ -124,6 +143,8 @@ struct Plan {
     std::chrono::steady_clock::time_point deadline{};
     unsigned long long r0_ring = 0;
     hipStream_t sum_stream() const { return overlap_sum ? st3 : st; }
+    int side_group = -1;     // >= 0: the shorter FFT lengths in one grouped launch (autotuned) ...
+    bool group_all = false;  // ... and the longest length in it too: one correlation launch per batch
     int sum_order = 0;  // harmonic-sum workgroup order (autotuned)
     int sum_rpt = 1;    // harmonic-sum fundamental r per thread (autotuned)
     // results of the last search

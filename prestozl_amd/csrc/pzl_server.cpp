@@ -163,6 +163,7 @@ struct Server {
 
     void worker()
     {
+        relax_capture_mode();
         for (;;) {
             std::shared_ptr<Request> r;
             void *plan = nullptr;

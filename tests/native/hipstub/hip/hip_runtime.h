@@ -77,6 +77,7 @@ hipError_t hipEventQuery(hipEvent_t ev);
 hipError_t hipEventSynchronize(hipEvent_t ev);
 hipError_t hipEventElapsedTime(float *ms, hipEvent_t a, hipEvent_t b);
 hipError_t hipStreamBeginCapture(hipStream_t st, hipStreamCaptureMode m);
+hipError_t hipThreadExchangeStreamCaptureMode(hipStreamCaptureMode *m);
 hipError_t hipStreamEndCapture(hipStream_t st, hipGraph_t *g);
 hipError_t hipGraphInstantiate(hipGraphExec_t *e, hipGraph_t g, void *, char *, size_t);
 hipError_t hipGraphLaunch(hipGraphExec_t e, hipStream_t st);

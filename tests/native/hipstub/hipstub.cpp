@@ -34,6 +34,7 @@ struct StubStream {
     std::deque<std::function<void()>> q;
     bool busy = false, stop = false;
     std::thread th;
+    std::vector<std::function<void()>> *capture = nullptr;  // being captured (by the capturing thread)
     StubStream()
     {
         th = std::thread([this] {
@@ -95,11 +96,16 @@ std::mutex g_mem_mu;
 std::map<void *, size_t> g_dev;
 long long g_dev_bytes = 0;
 
-// Run f on the stream, or now when the stream is the null stream.
+// Run f on the stream, or now when the stream is the null stream; a stream
+// being captured records f into its graph instead.
 void enqueue(hipStream_t st, std::function<void()> f)
 {
     if (!st) {
         f();
+        return;
+    }
+    if (st->capture) {
+        st->capture->push_back(std::move(f));
         return;
     }
     st->push(std::move(f));
@@ -314,12 +320,58 @@ hipError_t hipEventElapsedTime(float *ms, hipEvent_t a, hipEvent_t b)
     *ms = std::chrono::duration<float, std::milli>(b->t - a->t).count();
     return hipSuccess;
 }
-hipError_t hipStreamBeginCapture(hipStream_t, hipStreamCaptureMode) { return hipErrorUnknown; }
-hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t *) { return hipErrorUnknown; }
-hipError_t hipGraphInstantiate(hipGraphExec_t *, hipGraph_t, void *, char *, size_t) { return hipErrorUnknown; }
-hipError_t hipGraphLaunch(hipGraphExec_t, hipStream_t) { return hipErrorUnknown; }
-hipError_t hipGraphExecDestroy(hipGraphExec_t) { return hipSuccess; }
-hipError_t hipGraphDestroy(hipGraph_t) { return hipSuccess; }
+// Graphs: the operations enqueued on a stream between begin and end of a
+// capture, replayed in order on the stream a launch names.
+struct StubGraph {
+    std::vector<std::function<void()>> ops;
+};
+struct StubGraphExec {
+    std::vector<std::function<void()>> ops;
+};
+
+hipError_t hipThreadExchangeStreamCaptureMode(hipStreamCaptureMode *m)
+{
+    static thread_local hipStreamCaptureMode mode = hipStreamCaptureModeGlobal;
+    std::swap(mode, *m);
+    return hipSuccess;
+}
+
+hipError_t hipStreamBeginCapture(hipStream_t st, hipStreamCaptureMode)
+{
+    if (!st || st->capture) return hipErrorInvalidValue;
+    st->capture = new std::vector<std::function<void()>>();
+    return hipSuccess;
+}
+hipError_t hipStreamEndCapture(hipStream_t st, hipGraph_t *g)
+{
+    if (!st || !st->capture || !g) return hipErrorInvalidValue;
+    *g = new StubGraph{std::move(*st->capture)};
+    delete st->capture;
+    st->capture = nullptr;
+    return hipSuccess;
+}
+hipError_t hipGraphInstantiate(hipGraphExec_t *e, hipGraph_t g, void *, char *, size_t)
+{
+    if (!e || !g) return hipErrorInvalidValue;
+    *e = new StubGraphExec{g->ops};
+    return hipSuccess;
+}
+hipError_t hipGraphLaunch(hipGraphExec_t e, hipStream_t st)
+{
+    if (!e) return hipErrorInvalidValue;
+    for (auto &f : e->ops) enqueue(st, f);
+    return hipSuccess;
+}
+hipError_t hipGraphExecDestroy(hipGraphExec_t e)
+{
+    delete e;
+    return hipSuccess;
+}
+hipError_t hipGraphDestroy(hipGraph_t g)
+{
+    delete g;
+    return hipSuccess;
+}
 
 // ------------------------------------------------------------- fake kernels
 namespace {
@@ -374,6 +426,21 @@ hipError_t pzl_launch_corr(int, int, const PzlCorrArgs *a, const int *, hipStrea
 
 int pzl_corr_num_variants(int) { return 1; }
 const char *pzl_corr_variant_name(int, int) { return "stub"; }
+
+// One grouped launch: each class's plane rows, as pzl_launch_corr.
+hipError_t pzl_launch_corr_group(int group, int ncls, const int *, const PzlCorrArgs *const *args, const int *const *,
+                                 hipStream_t st)
+{
+    if (group != 0) return hipErrorInvalidValue;
+    for (int c = 0; c < ncls; c++) {
+        hipError_t e = pzl_launch_corr(0, 0, args[c], nullptr, st);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+int pzl_corr_num_groups(void) { return 1; }
+const char *pzl_corr_group_name(int group) { return group == 0 ? "stub group" : nullptr; }
+int pzl_corr_group_has(int group, int n) { return group == 0 && n >= 8 && n <= 4096; }
 int pzl_corr_supported(int n) { return n >= 8 && n <= 4096; }
 
 // Hits of rstep r: a count and cells that depend on r alone.

@@ -463,6 +463,102 @@ def test_corr_variants(config1_data, shape):
     plan.close()
 
 
+def _groups_for(plan):
+    """(group, all) for every correlation group that holds the plan's shorter
+    FFT lengths (all False) or every length (all True)."""
+    from prestozl_amd import binding as B
+    ok = []
+    for every in (False, True):
+        for g in range(len(B.corr_groups())):
+            try:
+                plan.set_corr_group(g, every)
+            except RuntimeError:
+                continue
+            ok.append((g, every))
+    plan.set_corr_group(-1)
+    return ok
+
+
+def _group_name(form):
+    from prestozl_amd import binding as B
+    g, every = form
+    return "none" if g < 0 else ("every length: " if every else "shorter lengths: ") + B.corr_groups()[g]
+
+
+@pytest.mark.parametrize("which", ["c2", "c3"])
+def test_corr_groups(which):
+    """Every correlation group, holding the shorter FFT lengths or every
+    length in one launch, gives the oracle's planes (FP64 correlation,
+    tolerance TOL_POW) for the largest and smallest harmonic of each length it
+    holds, at the config-2 shape (2048-point fundamental: 192 .. 2048) and the
+    config-3 shape (4096-point fundamental: the group holds 2048 and shorter,
+    so only the shorter lengths are grouped)."""
+    from prestozl_amd import binding as B
+    fft, kw = _c2_small() if which == "c2" else _c3_small()[:2]
+    plan = _plan(kw, fft)
+    obs, ctx64 = _oracle(kw, O.FP64)
+    hl = O.harmonics(obs)
+    rstep = plan.geom.nrsteps // 2
+    rstep_bins = plan.geom.corr_uselen // 2
+    startr = obs.rlo + rstep * rstep_bins
+    by_len = {}
+    for idx in range(len(hl)):
+        by_len.setdefault(plan.harms[idx].fftlen, []).append(idx)
+    longest = max(by_len)
+    forms = _groups_for(plan)
+    assert any(not every for _, every in forms), "no group holds the shorter lengths %s" % sorted(by_len)
+    if which == "c2":
+        assert any(every for _, every in forms), "no group holds every length %s" % sorted(by_len)
+    refs = {}
+    tested = 0
+    for form in forms:
+        plan.set_corr_group(*form)
+        for n, idxs in sorted(by_len.items()):
+            if n == longest and not form[1]:
+                continue
+            for idx in sorted({idxs[0], idxs[-1]}):
+                if idx not in refs:
+                    nh, hn = hl[idx]
+                    refs[idx] = ctx64.subharm(fft, nh, hn, startr, startr + rstep_bins - 0.5)[0]
+                pg, _ = plan.plane(rstep, idx)
+                e, _ = rel_err(pg, refs[idx], floor=1.0)
+                print("%s fftlen %d harmonic %s: max rel err %.3g" % (_group_name(form), n, hl[idx], e))
+                assert e < TOL_POW
+                tested += 1
+    assert tested >= 2 * len(forms)
+    plan.close()
+
+
+@pytest.mark.parametrize("which", ["c2", "c3"])
+def test_search_band_every_group(which):
+    """A band of the config-2 (3 rsteps at sigma 3, with the 210.7 Hz
+    injection) and config-3 (4 rsteps) searches with one launch per FFT length
+    and in every grouped form (the shorter lengths, or every length, in one
+    launch of each group): hits and candidates as the oracle's."""
+    from prestozl_amd import binding as B
+    if which == "c2":
+        fft, kw = _c2_small()
+        kw["sigma"] = 3.0
+        lo, hi = 60, 63
+    else:
+        fft, kw, _ = _c3_small()
+        lo, hi = 100, 104
+    plan = _plan(kw, fft)
+    obs, ctx32 = _oracle(kw, O.FP32)
+    co, ho, _ = ctx32.search(fft, lo, hi)
+    for form in [(-1, False)] + _groups_for(plan):
+        plan.set_corr_group(*form)
+        cg, st = plan.search(lo, hi)
+        hg, pg = plan.hits()
+        near, worst, common = _compare_hits(plan, hg, pg, ho, list(obs.powcut))
+        print("%s %s: %d gpu hits, %d oracle hits, %d near, err %.3g, grouped launches %d, cands %d/%d"
+              % (which, _group_name(form), len(hg), len(ho), near, worst, st["group_launches"], len(cg), len(co)))
+        assert worst < TOL_POW and common > 0
+        assert (st["group_launches"] > 0) == (form[0] >= 0)
+        _assert_same_cands(cg, co, kw["sigma"])
+    plan.close()
+
+
 @pytest.mark.parametrize("which", ["c1", "c2"])
 def test_graph_replay_identical(config1_data, which):
     """Batches replayed from captured graphs give the eager path's hits and
@@ -470,6 +566,7 @@ def test_graph_replay_identical(config1_data, which):
     fft, kw = _c1(config1_data) if which == "c1" else _c2_small()
     plan = _plan(kw, fft)
     lo, hi = (0, -1) if which == "c1" else (40, 80)  # more batches than hit slots: replays happen
+    plan.set_graph(False)
     c_eager, _ = plan.search(lo, hi, stats=False)
     h_eager = plan.hits()
     plan.set_graph(True)
@@ -499,6 +596,7 @@ def test_tune_batchsize(config1_data):
     pb = _plan(kw2, fft)
     for c in st["classes"]:  # the same kernel variants: identical arithmetic
         pb.set_variant(c["fftlen"], p8.variant(c["fftlen"]))
+    pb.set_corr_group(*p8.corr_group())
     assert pb.search(stats=False)[0] == c8
     p8.close()
     pb.close()
