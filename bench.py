@@ -461,8 +461,8 @@ def main():
                     help="1: replay each full batch's segments and correlations from a captured graph, 0: launch "
                          "them one by one, -1: the plan's default")
     ap.add_argument("--overlap", type=int, default=-1,
-                    help="profiling runs: 0 = one stream, 1 = harmonic sums on a side stream, 2 = also the "
-                         "shorter FFT lengths (the default)")
+                    help="profiling runs: 0 = one stream, 1 = harmonic sums on a side stream, 2 = also the next "
+                         "batch's segments ahead and the shorter FFT lengths beside (the default)")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
                     help="run a plain device copy of MIB MiB five times and exit (HBM counter calibration)")
     args = ap.parse_args()
@@ -521,11 +521,16 @@ def main():
     # search with the same kernels launched one by one, right after
     roof_stats, roof_note = stats_all, "kernel durations from the timed searches"
     if any(st["graph_launches"] > 0 for st in stats_all):
+        # the same kernels launched one by one in the replay's arrangement:
+        # segments and correlations in one stream, harmonic sums beside them
         plan.set_graph(False)
+        plan.set_overlap(1)
         roof_stats = [plan.search(lo, hi, stats=True)[1]]
+        plan.set_overlap(args.overlap if args.overlap >= 0 else 2)
         plan.set_graph(True)
-        roof_note = "kernel durations from one search launching the same kernels one by one after the timed " \
-                    "graph-replay searches"
+        roof_note = "kernel durations from one search after the timed graph-replay searches, launching the same " \
+                    "kernels one by one in the same arrangement (segments and correlations in one stream, " \
+                    "harmonic sums beside them)"
     cls = {}
     for st in roof_stats:
         for c in st["classes"]:
@@ -657,8 +662,8 @@ def main():
         tso = time.perf_counter() - ts
         out["overlap_sum_only"] = {"value": cells / tso, "unit": "cells/s", "ms_per_search": tso * 1e3,
                                    "same_candidates": co == cands,
-                                   "note": "harmonic sums and segments on their own streams, every FFT length "
-                                           "correlated on the main stream"}
+                                   "note": "harmonic sums on their own stream, segments and every correlation "
+                                           "in the main stream"}
         plan.set_overlap(True)
     if not args.no_legs and not args.rsteps:
         # the autotuner's rsteps-per-launch choice, timed on a 64-rstep band

@@ -160,11 +160,13 @@ int pzl_autotune(void *plan, int reps, char *report, int report_len);
  * default; on = 0 launches every kernel one by one.  Per-kernel timing is then
  * reported per batch (ms_graph) instead of per kernel. */
 int pzl_plan_set_graph(void *plan, int on);
-/* Overlap: the harmonic sum of a batch runs on its own stream, concurrently
- * with the correlation of the next batch (two plane sets), and the segments of
- * the next batch on a third.  on = 1: that; on = 2 (the default): also the
- * correlations of the shorter FFT lengths on the segments' stream, beside the
- * longest length's; on = 0: every kernel of the search in one stream. */
+/* Overlap: on = 1: the harmonic sum of a batch runs on its own stream,
+ * concurrently with the segments and correlations of the next batch (three
+ * buffer sets), which stay in one stream (the arrangement of a replayed
+ * graph); on = 2 (the default): also the segments of the next batch ahead on a
+ * third stream, and the correlations of the shorter FFT lengths there beside
+ * the longest length's (when they are not grouped); on = 0: every kernel of
+ * the search in one stream. */
 int pzl_plan_set_overlap(void *plan, int on);
 /* Rsteps per launch: for each candidate batch size, plan, autotune and time a
  * band of band_rsteps rsteps from the middle of the search on this spectrum;

@@ -388,9 +388,10 @@ class Plan:
         check(lib().pzl_plan_set_graph(self.h, int(bool(on))))
 
     def set_overlap(self, on=2):
-        """0 / False: one stream; 1: harmonic sums and segments on their own
-        streams; 2 / True (the default): also the shorter FFT lengths'
-        correlations beside the longest's."""
+        """0 / False: one stream; 1: harmonic sums on their own stream,
+        segments and correlations in one; 2 / True (the default): also the
+        next batch's segments ahead on a third stream, with the shorter FFT
+        lengths' correlations beside the longest's."""
         mode = (2 if on else 0) if isinstance(on, bool) else int(on)
         check(lib().pzl_plan_set_overlap(self.h, mode))
 

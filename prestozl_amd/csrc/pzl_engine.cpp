@@ -32,21 +32,12 @@
 #include "pzl_accel.h"
 #include "pzl_candlist.h"
 #include "pzl_kernels.h"
+#include "pzl_launch.h"
 #include "pzl_math.h"
 #include "pzl_plan.h"
 #include "pzl_stats.h"
 
 using namespace pzl;
-
-namespace pzl {
-void relax_capture_mode()
-{
-    static thread_local bool done = false;
-    if (done) return;
-    hipStreamCaptureMode m = hipStreamCaptureModeRelaxed;
-    done = hipThreadExchangeStreamCaptureMode(&m) == hipSuccess;
-}
-}  // namespace pzl
 
 namespace {
 
@@ -597,36 +588,57 @@ int drop_graphs(Plan &p)
 }
 
 // The correlation part of a full batch as one graph launch: the graph of a
-// buffer set (segments and every correlation class) is captured once, with the
+// buffer set (segments and every correlation launch) is built once, with the
 // batch's first rstep read from device memory, and replayed for every later
-// full batch in that set; the first batch of each set runs eagerly.  A capture
-// that fails (the runtime can invalidate one when another thread makes a call
-// it counts as unsafe meanwhile) is dropped: the batch runs eagerly and a
-// later batch of the set captures again; after kMaxCaptureFailures the plan
-// launches eagerly for good.
+// full batch in that set; the first batch of each set runs eagerly (it also
+// builds what the kernels read once per device).  The graph is built from the
+// recorded launches with hipGraphAddKernelNode, in launch order, each node
+// depending on the one before: no stream is captured, so HIP calls made
+// meanwhile by other threads are unaffected.  A graph the runtime refuses is
+// dropped (the batch has run eagerly) and a later batch of the set builds
+// again; after kMaxGraphFailures the plan launches eagerly for good.
+int build_batch_graph(Plan &p, int set, hipGraphExec_t *out)
+{
+    std::vector<PzlKernelRec> recs;
+    pzl_recorder = &recs;
+    const int rc = launch_corr_part(p, 0, p.B, set, nullptr, p.d_r0 + set);
+    pzl_recorder = nullptr;
+    if (rc) return rc;
+    hipGraph_t g = nullptr;
+    PZL_HIPCHK(hipGraphCreate(&g, 0));
+    hipGraphNode_t prev = nullptr;
+    hipError_t e = hipSuccess;
+    for (auto &r : recs) {
+        void *args[1] = {r.arg.data()};
+        hipKernelNodeParams np{};
+        np.func = const_cast<void *>(r.func);
+        np.gridDim = r.grid;
+        np.blockDim = r.block;
+        np.sharedMemBytes = (unsigned)r.smem;
+        np.kernelParams = args;
+        hipGraphNode_t node = nullptr;
+        e = hipGraphAddKernelNode(&node, g, prev ? &prev : nullptr, prev ? 1 : 0, &np);
+        if (e != hipSuccess) break;
+        prev = node;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        *out = nullptr;
+        return fail(PZL_ERR_HIP, "batch graph: %s", hipGetErrorString(e));
+    }
+    return 0;
+}
+
 int launch_corr_graph(Plan &p, long long r0, int set, Timers *tm)
 {
     if (!p.gexec[set]) {
         int rc = launch_corr_part(p, r0, p.B, set, tm, nullptr);
         if (rc) return rc;
-        hipGraph_t g = nullptr;
-        const hipError_t eb = hipStreamBeginCapture(p.st, hipStreamCaptureModeThreadLocal);
-        if (eb != hipSuccess) {
-            (void)hipGetLastError();
-            if (++p.capture_failures >= Plan::kMaxCaptureFailures) p.use_graph = false;
-            return 0;
-        }
-        rc = launch_corr_part(p, 0, p.B, set, nullptr, p.d_r0 + set);
-        const hipError_t e = hipStreamEndCapture(p.st, &g);
-        hipError_t ei = hipErrorUnknown;
-        if (rc == 0 && e == hipSuccess && g) ei = hipGraphInstantiate(&p.gexec[set], g, nullptr, nullptr, 0);
-        if (g) (void)hipGraphDestroy(g);
-        if (ei != hipSuccess) {
-            // nothing of the capture ran: the batch is already done eagerly above
-            p.gexec[set] = nullptr;
+        if (build_batch_graph(p, set, &p.gexec[set])) {
             (void)hipGetLastError();
             pzl::g_err.clear();
-            if (++p.capture_failures >= Plan::kMaxCaptureFailures) p.use_graph = false;
+            if (++p.graph_failures >= Plan::kMaxGraphFailures) p.use_graph = false;
         }
         return 0;
     }
@@ -775,7 +787,6 @@ struct SearchState {
     }
     void run()
     {
-        relax_capture_mode();
         for (;;) {
             std::pair<int, long long> j;
             Plan::HostBuf hb;
@@ -972,7 +983,6 @@ int pzl_geometry_from_config(const PzlConfig *cfg, PzlGeometry *g)
 
 int pzl_plan_create(const PzlConfig *cfg, int device, void **plan_out)
 {
-    const CaptureRelaxed relaxed;
     if (!cfg || !plan_out) return fail(PZL_ERR_ARG, "null argument");
     *plan_out = nullptr;
     Plan *p = new Plan();
@@ -1194,11 +1204,7 @@ int pzl_plan_create(const PzlConfig *cfg, int device, void **plan_out)
     return 0;
 }
 
-void pzl_plan_destroy(void *plan)
-{
-    const CaptureRelaxed relaxed;
-    free_plan((Plan *)plan);
-}
+void pzl_plan_destroy(void *plan) { free_plan((Plan *)plan); }
 
 int pzl_plan_device_bytes(void *plan, long long *held)
 {
@@ -1235,7 +1241,6 @@ int pzl_plan_harm(void *plan, int idx, PzlHarmInfo *h)
 
 int pzl_set_spectrum(void *plan, const float *host_fft, long long numbins)
 {
-    const CaptureRelaxed relaxed;
     Plan *p = (Plan *)plan;
     if (!p || !host_fft || numbins < 1) return fail(PZL_ERR_ARG, "bad spectrum");
     PZL_HIPCHK(hipSetDevice(p->device));
@@ -1253,7 +1258,6 @@ int pzl_set_spectrum(void *plan, const float *host_fft, long long numbins)
 
 int pzl_set_spectrum_device(void *plan, const void *dev_fft, long long numbins)
 {
-    const CaptureRelaxed relaxed;
     Plan *p = (Plan *)plan;
     if (!p || !dev_fft || numbins < 1) return fail(PZL_ERR_ARG, "bad spectrum");
     p->d_fft = (const float2 *)dev_fft;
@@ -1264,7 +1268,6 @@ int pzl_set_spectrum_device(void *plan, const void *dev_fft, long long numbins)
 int pzl_search(void *plan, long long rstep_lo, long long rstep_hi, PzlCand *cands, long long maxcands,
                long long *ncands, PzlStats *stats)
 {
-    const CaptureRelaxed relaxed;
     Plan *p = (Plan *)plan;
     if (!p) return fail(PZL_ERR_ARG, "null plan");
     if (!p->d_fft) return fail(PZL_ERR_NOSPECTRUM, "no spectrum set");
@@ -1316,7 +1319,8 @@ int pzl_search(void *plan, long long rstep_lo, long long rstep_hi, PzlCand *cand
         if (i >= p->nslots) PZL_HIPCHK(hipStreamWaitEvent(p->sum_stream(), p->ev_free[slot], 0));
         // (segments ahead on their own stream read only what is constant during
         // the search: the spectrum and the startr table)
-        int rc = launch_batch(*p, r0, nb, i % Plan::kSets, slot, nullptr, &tm, true, p->use_graph, p->overlap_sum);
+        int rc = launch_batch(*p, r0, nb, i % Plan::kSets, slot, nullptr, &tm, true, p->use_graph,
+                              p->overlap_sum && p->side_classes);
         if (rc) return rc;
         PZL_PINNED_CHECK(p->h_counts + 2 * slot, 2 * sizeof(unsigned long long));
         PZL_HIPCHK(hipMemcpyAsync(p->h_counts + 2 * slot, p->d_counts + 2 * slot, 2 * sizeof(unsigned long long),
@@ -1608,20 +1612,23 @@ int pzl_autotune(void *plan, int reps, char *report, int report_len)
         p->sum_order = best_order;
         p->sum_rpt = best_rpt;
     }
-    // one launch per length, the shorter lengths in one grouped launch, or
-    // every length in one: each form a group allows is timed on the same band
-    // (best of three searches), and a form replaces the one kept so far only
-    // when it is faster by more than the timing noise (0.5%)
+    // every length in one grouped launch, the shorter lengths in one, or one
+    // launch per length: each form a group allows is timed on the same band
+    // (best of three searches), in that order (the last group first: the
+    // leanest), and a form replaces the one kept so far only when it is faster
+    // by more than 2% (the forms differ by about that much, and the choice
+    // should not follow the timing noise)
     if (p->classes.size() >= 2 && nr >= band) {
         const long long lo = (nr - band) / 2;
         const int ng = pzl_corr_num_groups();
-        std::vector<int> forms{-1};  // as g_tuned_group
-        for (int all = 0; all < 2; all++)
-            for (int g = 0; g < ng; g++)
+        std::vector<int> forms;  // as g_tuned_group
+        for (int all = 1; all >= 0; all--)
+            for (int g = ng - 1; g >= 0; g--)
                 if (group_covers(*p, g, all != 0)) forms.push_back(g + all * ng);
+        forms.push_back(-1);
         if (forms.size() > 1) {
             float best_ms = 1e30f;
-            int best = -1;
+            int best = forms[0];
             for (int f : forms) {
                 p->side_group = f < 0 ? -1 : f % ng;
                 p->group_all = f >= ng;
@@ -1645,7 +1652,7 @@ int pzl_autotune(void *plan, int reps, char *report, int report_len)
                          f < 0 ? "one launch per FFT length" : (f >= ng ? "every length in " : "shorter lengths in "),
                          f < 0 ? "" : pzl_corr_group_name(f % ng), band, ms);
                 rep += line;
-                if (ms < 0.995f * best_ms) {
+                if (ms < 0.98f * best_ms) {
                     best_ms = ms;
                     best = f;
                 }

@@ -21,6 +21,7 @@
 
 #include "pzl_fft.h"
 #include "pzl_kernels.h"
+#include "pzl_launch.h"
 #include "pzl_math.h"
 #include "pzl_response.h"
 #include "pzl_stats.h"
@@ -375,6 +376,8 @@ template <int R> struct ExpTab {
 // with P_0 = pdata (K*[k] + K*[k+N/2]) and P_1 = pdata (K*[k] - K*[k+N/2])
 // exp(2 pi i k/N): the first radix-2 stage of the N-point transform done on the
 // kernel row once per batch instead of per rstep.
+thread_local std::vector<PzlKernelRec> *pzl_recorder = nullptr;
+
 // Work units u0, u0 + ustep, ... (< a.total_items) of one correlation
 // variant, by the NT threads of a workgroup; smem holds CP::smem() bytes.
 template <int N, int NT, int G, bool PING, int SH, int PADSH, int... Rs>
@@ -566,6 +569,7 @@ template <int N_, int NT_, int G_, bool PING, int SH, int PADSH, int... Rs> stru
         if (e != hipSuccess) return e;
         if (dev < 0 || dev >= 64) return hipErrorInvalidDevice;
         std::lock_guard<std::mutex> lk(mu);
+        if (!tables[dev] && pzl_recorder) return hipErrorNotReady;  // built by an enqueued launch first
         if (!tables[dev]) {
             float2 *t = nullptr;
             e = hipMalloc(&t, sizeof(float2) * (size_t)(CP::TWN > 0 ? CP::TWN : 1) * NT);
@@ -1060,8 +1064,7 @@ extern "C" hipError_t pzl_launch_segments(const PzlBatchArgs *a, int max_fftlen,
     int n2 = 1;
     while (n2 < numdata) n2 <<= 1;
     const size_t smem = (size_t)numdata * sizeof(double2) + (size_t)n2 * sizeof(float);
-    hipLaunchKernelGGL(k_segments, dim3(a->nb * a->nh), dim3(SEG_NT), smem, st, *a);
-    return hipGetLastError();
+    return pzl_launch((const void *)k_segments, dim3(a->nb * a->nh), dim3(SEG_NT), smem, st, *a);
 }
 
 // One correlation variant: kernel-row length N, NT threads, G rows per
@@ -1077,9 +1080,8 @@ static hipError_t launch_corr_t(const PzlCorrArgs *a, const int *class_nrows, hi
     if (a2.total_items < 1) return hipSuccess;
     hipError_t e = V::table(st, &a2.twtab);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_corr<N, NT, G, PING, SH, PADSH, Rs...>), dim3((unsigned)a2.total_items), dim3(NT),
-                       V::smem(), st, a2);
-    return hipGetLastError();
+    return pzl_launch((const void *)k_corr<N, NT, G, PING, SH, PADSH, Rs...>, dim3((unsigned)a2.total_items), dim3(NT),
+                      V::smem(), st, a2);
 }
 
 // ---------------------------------------------------------------------------
@@ -1165,8 +1167,7 @@ template <int NT, int WMIN, typename... Vs> struct CorrGroup {
         if (acc < 1) return hipSuccess;
         if (acc > 0x7fffffffLL) return hipErrorInvalidValue;
         const size_t sm = std::max({Vs::smem()...});
-        hipLaunchKernelGGL((k_corr_group<NT, WMIN, Vs...>), dim3((unsigned)acc), dim3(NT), sm, st, g);
-        return hipGetLastError();
+        return pzl_launch((const void *)k_corr_group<NT, WMIN, Vs...>, dim3((unsigned)acc), dim3(NT), sm, st, g);
     }
     static int has(int n) { return member(n) >= 0; }
 };

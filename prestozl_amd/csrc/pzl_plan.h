@@ -15,23 +15,6 @@ namespace pzl {
 int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
 extern thread_local std::string g_err;
 
-// While a thread of the library captures a plan's graph, calls that the
-// runtime counts as unsafe during a capture (allocations, frees,
-// synchronisations) made by other threads must not invalidate it: the
-// library's own threads take the relaxed interaction mode for good
-// (relax_capture_mode), and calls into the library take it for their duration
-// (CaptureRelaxed), restoring the caller's mode on return.
-void relax_capture_mode();
-struct CaptureRelaxed {
-    hipStreamCaptureMode prev = hipStreamCaptureModeRelaxed;
-    bool ok = false;
-    CaptureRelaxed() { ok = hipThreadExchangeStreamCaptureMode(&prev) == hipSuccess; }
-    ~CaptureRelaxed()
-    {
-        if (ok) (void)hipThreadExchangeStreamCaptureMode(&prev);
-    }
-};
-
 #define PZL_HIPCHK(call)                                                                                \
     do {                                                                                                \
         hipError_t e_ = (call);                                                                         \
@@ -71,7 +54,8 @@ struct Plan {
     hipStream_t st3 = nullptr;  // harmonic sums (when overlapped)
     hipStream_t st4 = nullptr;  // segments (when overlapped), ahead of the correlation
     bool overlap_sum = true;    // harmonic sums on st3, concurrent with the next batch's correlation
-    bool side_classes = true;   // with overlap_sum: shorter FFT lengths correlated on st4, beside the longest
+    bool side_classes = true;   // with overlap_sum: the next batch's segments ahead on st4, and the shorter
+                                // FFT lengths correlated there beside the longest (unless grouped)
     // resident
     PzlHarm *d_harms = nullptr;
     float2 *d_bank = nullptr;  // shared, owned by the bank cache
@@ -132,8 +116,8 @@ struct Plan {
     float *d_sums = nullptr;
     // graph replay of full batches (one executable graph per buffer set)
     bool use_graph = true;
-    static constexpr int kMaxCaptureFailures = 8;
-    int capture_failures = 0;  // captures the runtime refused (the batches ran eagerly)
+    static constexpr int kMaxGraphFailures = 8;
+    int graph_failures = 0;  // batch graphs the runtime refused (those batches ran eagerly)
     hipGraphExec_t gexec[kSets] = {};  // correlation part, per buffer set
     long long *d_r0 = nullptr;  // [kSets] first rstep of the batch in each set
     static constexpr int kR0Ring = 2 * kSlots;

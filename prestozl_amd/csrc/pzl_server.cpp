@@ -163,7 +163,6 @@ struct Server {
 
     void worker()
     {
-        relax_capture_mode();
         for (;;) {
             std::shared_ptr<Request> r;
             void *plan = nullptr;

@@ -18,6 +18,10 @@ struct float2 {
 struct double2 {
     double x, y;
 };
+struct dim3 {
+    unsigned x = 1, y = 1, z = 1;
+    dim3(unsigned a = 1, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {}
+};
 
 typedef enum hipError_t {
     hipSuccess = 0,
@@ -31,6 +35,15 @@ typedef struct StubStream *hipStream_t;
 typedef struct StubEvent *hipEvent_t;
 typedef struct StubGraph *hipGraph_t;
 typedef struct StubGraphExec *hipGraphExec_t;
+typedef struct StubGraphNode *hipGraphNode_t;
+struct hipKernelNodeParams {
+    dim3 blockDim;
+    void **extra;
+    void *func;
+    dim3 gridDim;
+    void **kernelParams;
+    unsigned sharedMemBytes;
+};
 
 enum hipMemcpyKind {
     hipMemcpyHostToHost = 0,
@@ -77,8 +90,13 @@ hipError_t hipEventQuery(hipEvent_t ev);
 hipError_t hipEventSynchronize(hipEvent_t ev);
 hipError_t hipEventElapsedTime(float *ms, hipEvent_t a, hipEvent_t b);
 hipError_t hipStreamBeginCapture(hipStream_t st, hipStreamCaptureMode m);
-hipError_t hipThreadExchangeStreamCaptureMode(hipStreamCaptureMode *m);
 hipError_t hipStreamEndCapture(hipStream_t st, hipGraph_t *g);
+hipError_t hipGraphCreate(hipGraph_t *g, unsigned flags);
+// Kernel nodes of the stand-in: kernelParams[0] points to the address of the
+// recorded operation (see the launchers in hipstub.cpp).
+hipError_t hipGraphAddKernelNode(hipGraphNode_t *n, hipGraph_t g, const hipGraphNode_t *deps, size_t ndeps,
+                                 const hipKernelNodeParams *p);
+hipError_t hipLaunchKernel(const void *f, dim3 g, dim3 b, void **args, size_t smem, hipStream_t st);
 hipError_t hipGraphInstantiate(hipGraphExec_t *e, hipGraph_t g, void *, char *, size_t);
 hipError_t hipGraphLaunch(hipGraphExec_t e, hipStream_t st);
 hipError_t hipGraphExecDestroy(hipGraphExec_t e);

@@ -21,10 +21,12 @@
 #include <map>
 #include <mutex>
 #include <set>
+#include <memory>
 #include <thread>
 #include <vector>
 
 #include "pzl_kernels.h"
+#include "pzl_launch.h"
 
 using Clock = std::chrono::steady_clock;
 
@@ -320,8 +322,9 @@ hipError_t hipEventElapsedTime(float *ms, hipEvent_t a, hipEvent_t b)
     *ms = std::chrono::duration<float, std::milli>(b->t - a->t).count();
     return hipSuccess;
 }
-// Graphs: the operations enqueued on a stream between begin and end of a
-// capture, replayed in order on the stream a launch names.
+// Graphs: operations captured from a stream, or added as kernel nodes (each
+// a recorded launch of the stand-in's kernels), replayed in order on the
+// stream a launch names.
 struct StubGraph {
     std::vector<std::function<void()>> ops;
 };
@@ -329,12 +332,24 @@ struct StubGraphExec {
     std::vector<std::function<void()>> ops;
 };
 
-hipError_t hipThreadExchangeStreamCaptureMode(hipStreamCaptureMode *m)
+hipError_t hipGraphCreate(hipGraph_t *g, unsigned)
 {
-    static thread_local hipStreamCaptureMode mode = hipStreamCaptureModeGlobal;
-    std::swap(mode, *m);
+    if (!g) return hipErrorInvalidValue;
+    *g = new StubGraph{};
     return hipSuccess;
 }
+
+hipError_t hipGraphAddKernelNode(hipGraphNode_t *n, hipGraph_t g, const hipGraphNode_t *, size_t,
+                                 const hipKernelNodeParams *p)
+{
+    if (!g || !p || !p->kernelParams) return hipErrorInvalidValue;
+    const std::function<void()> *op = *(std::function<void()> *const *)p->kernelParams[0];
+    g->ops.push_back(*op);
+    if (n) *n = (hipGraphNode_t)(uintptr_t)g->ops.size();
+    return hipSuccess;
+}
+
+hipError_t hipLaunchKernel(const void *, dim3, dim3, void **, size_t, hipStream_t) { return hipErrorInvalidValue; }
 
 hipError_t hipStreamBeginCapture(hipStream_t st, hipStreamCaptureMode)
 {
@@ -397,22 +412,51 @@ hipError_t pzl_launch_bank_build(const PzlHarm *, int, int total_rows, int max_f
     return hipSuccess;
 }
 
+extern "C++" {
+thread_local std::vector<PzlKernelRec> *pzl_recorder = nullptr;
+}
+
+namespace {
+// The stand-in's kernels: enqueued, or -- while the engine records a batch
+// -- recorded, the argument block holding the address of the operation (kept
+// until the process ends) for hipGraphAddKernelNode.
+std::mutex g_ops_mu;
+std::vector<std::unique_ptr<std::function<void()>>> g_ops;
+
+hipError_t launch_op(hipStream_t st, std::function<void()> f)
+{
+    if (!pzl_recorder) {
+        enqueue(st, std::move(f));
+        return hipSuccess;
+    }
+    std::function<void()> *op;
+    {
+        std::lock_guard<std::mutex> lk(g_ops_mu);
+        g_ops.push_back(std::make_unique<std::function<void()>>(std::move(f)));
+        op = g_ops.back().get();
+    }
+    PzlKernelRec r{nullptr, dim3(1), dim3(1), 0, std::vector<unsigned char>(sizeof op)};
+    std::memcpy(r.arg.data(), &op, sizeof op);
+    pzl_recorder->push_back(std::move(r));
+    return hipSuccess;
+}
+}  // namespace
+
 hipError_t pzl_launch_segments(const PzlBatchArgs *a, int, hipStream_t st)
 {
     const PzlBatchArgs b = *a;
-    enqueue(st, [=] {
+    return launch_op(st, [=] {
         touch(b.segs, b.seg_stride * b.nb);
         touch(b.hr_tab, (long long)b.nb * b.nh);
         touch_r(b.fft, b.numbins);
         if (b.rstep0_dev) touch_r(b.rstep0_dev, 1);
     });
-    return hipSuccess;
 }
 
 hipError_t pzl_launch_corr(int, int, const PzlCorrArgs *a, const int *, hipStream_t st)
 {
     const PzlCorrArgs c = *a;
-    enqueue(st, [=] {
+    return launch_op(st, [=] {
         // the class's own plane rows only: classes run concurrently on two streams
         for (int b = 0; b < c.nb; b++)
             for (int k = 0; k < c.nclass; k++) {
@@ -421,7 +465,6 @@ hipError_t pzl_launch_corr(int, int, const PzlCorrArgs *a, const int *, hipStrea
                 touch_r(c.segs + b * c.seg_stride + H.seg_off, H.fftlen / 2);
             }
     });
-    return hipSuccess;
 }
 
 int pzl_corr_num_variants(int) { return 1; }
