@@ -94,6 +94,53 @@ __global__ __launch_bounds__(BANK_NT) void k_bank_build(const PzlHarm *__restric
 
 #define SEG_NT 256
 
+// The element of rank k (0-based, ascending) of n non-negative floats in LDS,
+// by the SEG_NT threads of a workgroup: a radix select on their bit patterns
+// (which order non-negative floats as their values), 8 bits at a time from
+// the top.  Per digit: a histogram of the keys that match the digits found so
+// far, a scan of it, and the digit whose bin holds rank k.  The keys are left
+// as they are.
+__device__ __forceinline__ float seg_select(const float *keys, int n, unsigned k, int tid)
+{
+    __shared__ unsigned hist[256];
+    __shared__ unsigned wsum[SEG_NT / 64];
+    __shared__ unsigned found[2];
+    static_assert(SEG_NT == 256, "one histogram bin per thread");
+    const int lane = __lane_id(), wave = tid / 64;
+    unsigned prefix = 0, mask = 0;
+    for (int shift = 24; shift >= 0; shift -= 8) {
+        hist[tid] = 0;
+        __syncthreads();
+        for (int i = tid; i < n; i += SEG_NT) {
+            const unsigned u = __float_as_uint(keys[i]);
+            if ((u & mask) == prefix) atomicAdd(&hist[(u >> shift) & 255u], 1u);
+        }
+        __syncthreads();
+        // inclusive scan of the 256 bins, one per thread
+        const unsigned c = hist[tid];
+        unsigned x = c;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const unsigned y = __shfl_up(x, d);
+            if (lane >= d) x += y;
+        }
+        if (lane == 63) wsum[wave] = x;
+        __syncthreads();
+#pragma unroll
+        for (int w = 0; w < SEG_NT / 64; w++) x += w < wave ? wsum[w] : 0u;
+        if (x - c <= k && k < x) {  // exactly one bin holds rank k
+            found[0] = (unsigned)tid;
+            found[1] = k - (x - c);
+        }
+        __syncthreads();
+        prefix |= found[0] << shift;
+        mask |= 255u << shift;
+        k = found[1];
+        __syncthreads();
+    }
+    return __uint_as_float(prefix);
+}
+
 __global__ __launch_bounds__(SEG_NT) void k_segments(PzlBatchArgs a)
 {
     extern __shared__ double2 sbuf[];  // fftlen/2 complex doubles, then keys
@@ -131,24 +178,8 @@ __global__ __launch_bounds__(SEG_NT) void k_segments(PzlBatchArgs a)
     if (a.nph > 0.0) {
         norm = 1.0 / sqrt(a.nph);
     } else {
-        // bitonic sort of the powers; the median is sorted element (n-1)/2
-        for (int k = 2; k <= n2; k <<= 1) {
-            for (int j = k >> 1; j > 0; j >>= 1) {
-                for (int i = tid; i < n2; i += SEG_NT) {
-                    const int ixj = i ^ j;
-                    if (ixj > i) {
-                        const float ki = keys[i], kj = keys[ixj];
-                        const bool up = ((i & k) == 0);
-                        if ((ki > kj) == up) {
-                            keys[i] = kj;
-                            keys[ixj] = ki;
-                        }
-                    }
-                }
-                __syncthreads();
-            }
-        }
-        norm = 1.0 / sqrt((double)keys[(numdata - 1) / 2] / log(2.0));
+        // the lower median: element (n-1)/2 of the powers in ascending order
+        norm = 1.0 / sqrt((double)seg_select(keys, n2, (unsigned)(numdata - 1) / 2, tid) / log(2.0));
     }
     __syncthreads();
     for (int i = tid; i < numdata; i += SEG_NT) {
@@ -1175,7 +1206,7 @@ template <int NT, int WMIN, typename... Vs> struct CorrGroup {
 // The groups, each with a member for every FFT length from 128 to 2048 (the
 // lengths shorter than the longest of a search).
 using CorrGroup256 =
-    CorrGroup<256, 1, CorrV<2048, 256, 1, true, 0, 5, 8, 8, 4, 8>, CorrV<1280, 256, 2, true, 0, 4, 16, 16, 5>,
+    CorrGroup<256, 1, CorrV<2048, 256, 1, true, 0, 3, 8, 8, 8, 4>, CorrV<1280, 256, 2, true, 0, 4, 16, 16, 5>,
               CorrV<1024, 256, 2, true, 0, 5, 8, 8, 4, 4>, CorrV<768, 256, 4, true, 0, 4, 12, 4, 4, 4>,
               CorrV<512, 256, 4, true, 0, 4, 8, 8, 8>, CorrV<384, 256, 8, true, 0, 4, 6, 4, 4, 4>,
               CorrV<256, 256, 8, true, 0, 4, 8, 8, 4>, CorrV<192, 256, 16, true, 0, 4, 12, 4, 4>,
@@ -1185,7 +1216,7 @@ using CorrGroup256 =
 // hold many values per thread: every member stays within 128 VGPRs, so the
 // group keeps 4 waves per SIMD.
 using CorrGroup256Lean =
-    CorrGroup<256, 4, CorrV<2048, 256, 1, true, 0, 5, 8, 8, 4, 8>, CorrV<1280, 256, 1, true, 0, 4, 4, 8, 8, 5>,
+    CorrGroup<256, 4, CorrV<2048, 256, 1, true, 0, 3, 8, 8, 8, 4>, CorrV<1280, 256, 1, true, 0, 4, 4, 8, 8, 5>,
               CorrV<1024, 256, 2, true, 0, 5, 8, 8, 4, 4>, CorrV<768, 256, 1, true, 0, 4, 3, 4, 4, 4, 4>,
               CorrV<512, 256, 4, true, 0, 4, 8, 8, 8>, CorrV<384, 256, 4, true, 0, 4, 8, 8, 6>,
               CorrV<256, 256, 8, true, 0, 4, 8, 8, 4>, CorrV<192, 256, 8, true, 0, 4, 8, 4, 6>,
