@@ -64,9 +64,16 @@ def config2_full():
 
 @pytest.mark.parametrize("inj", [0, 1, 2])
 def test_config2_full_length_bands(config2_full, inj):
-    """N = 2^24, sigma 5, batchsize 8: 8 rsteps centred on injection 'inj'."""
+    """N = 2^24, sigma 5, batchsize 8: 8 rsteps centred on injection 'inj',
+    with every FFT length in one grouped launch (the lean group: the form the
+    autotuner keeps at this shape) for two of the bands and one launch per
+    length for the third."""
+    from prestozl_amd import binding as B
     fft, kw, sigs = config2_full
     plan = _plan(kw, fft)
+    if inj != 1:
+        lean = [g for g, name in enumerate(B.corr_groups()) if "lean" in name][0]
+        plan.set_corr_group(lean, True)
     obs, ctx = _oracle(kw)
     k = injection_rstep(obs, kw["N"], kw["dt"], sigs[inj])
     _check("config2 injection %d" % inj, plan, ctx, obs, fft, k - 4, k + 4, kw["sigma"])
