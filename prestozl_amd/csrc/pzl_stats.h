@@ -24,12 +24,58 @@ PZL_HD double pzl_as26223(double t)
     return t - (2.515517 + t * (0.802853 + t * 0.010328)) / (1.0 + t * (1.432788 + t * (0.189269 + t * 0.001308)));
 }
 
+// c[0] + c[1] x + ... + c[7] x^7
+PZL_HD double pzl_poly7(const double *c, double x)
+{
+    double s = c[7];
+    for (int i = 6; i >= 0; --i) s = s * x + c[i];
+    return s;
+}
+
+// x with upper normal tail q (0 < q < 1) to about 1e-16 relative: Wichura's
+// algorithm AS 241 (PPND16; Applied Statistics 37 (1988) 477-484), rational
+// functions of 0.180625 - (1/2 - q)^2 near the centre and of
+// sqrt(-log(smaller tail)) beyond.
+PZL_HD double pzl_as241_upper(double q)
+{
+    const double a[8] = {3.3871328727963666080e0,  1.3314166789178437745e+2, 1.9715909503065514427e+3,
+                         1.3731693765509461125e+4, 4.5921953931549871457e+4, 6.7265770927008700853e+4,
+                         3.3430575583588128105e+4, 2.5090809287301226727e+3};
+    const double b[8] = {1.0,
+                         4.2313330701600911252e+1, 6.8718700749205790830e+2, 5.3941960214247511077e+3,
+                         2.1213794301586595867e+4, 3.9307895800092710610e+4, 2.8729085735721942674e+4,
+                         5.2264952788528545610e+3};
+    const double c[8] = {1.42343711074968357734e0,  4.63033784615654529590e0,  5.76949722146069140550e0,
+                         3.64784832476320460504e0,  1.27045825245236838258e0,  2.41780725177450611770e-1,
+                         2.27238449892691845833e-2, 7.74545014278341407640e-4};
+    const double d[8] = {1.0,
+                         2.05319162663775882187e0,  1.67638483018380384940e0,  6.89767334985100004550e-1,
+                         1.48103976427480074590e-1, 1.51986665636164571966e-2, 5.47593808499534494600e-4,
+                         1.05075007164441684324e-9};
+    const double e[8] = {6.65790464350110377720e0,  5.46378491116411436990e0,  1.78482653991729133580e0,
+                         2.96560571828504891230e-1, 2.65321895265761230930e-2, 1.24266094738807843860e-3,
+                         2.71155556874348757815e-5, 2.01033439929228813265e-7};
+    const double f[8] = {1.0,
+                         5.99832206555887937690e-1, 1.36929880922735805310e-1, 1.48753612908506148525e-2,
+                         7.86869131145613259100e-4, 1.84631831751005468180e-5, 1.42151175831644588870e-7,
+                         2.04426310338993978564e-15};
+    const double h = 0.5 - q;  // the lower tail of x minus 1/2
+    if (fabs(h) <= 0.425) {
+        const double r = 0.180625 - h * h;
+        return h * pzl_poly7(a, r) / pzl_poly7(b, r);
+    }
+    const double r = sqrt(-(h > 0.0 ? log(q) : log1p(-q)));
+    const double v = r <= 5.0 ? pzl_poly7(c, r - 1.6) / pzl_poly7(d, r - 1.6) : pzl_poly7(e, r - 5.0) / pzl_poly7(f, r - 5.0);
+    return h > 0.0 ? v : -v;
+}
+
 // x with pzl_upper_normal_tail(x) == q, safeguarded Newton to full precision
+// from AS 241's value (one step, in practice, to meet the tolerance)
 PZL_HD double pzl_inverse_upper_normal_tail(double q)
 {
     if (q <= 0.0) return 40.0;
     if (q >= 1.0) return -40.0;
-    double x = (q < 0.5) ? pzl_as26223(sqrt(-2.0 * log(q))) : -pzl_as26223(sqrt(-2.0 * log1p(-q)));
+    double x = pzl_as241_upper(q);
     double lo = -40.0, hi = 40.0;
     for (int it = 0; it < 100; ++it) {
         const double f = pzl_upper_normal_tail(x) - q;
@@ -38,10 +84,14 @@ PZL_HD double pzl_inverse_upper_normal_tail(double q)
         const double dens = exp(-0.5 * x * x) * 0.39894228040143267794;
         if (!(dens > 0.0)) break;
         double xn = x + f / dens;
+        // converged (a step of zero included: x is then a bracket end, which
+        // the safeguard below would take for a step out of the bracket)
+        if (fabs(xn - x) <= 1e-15 * fabs(xn)) {
+            x = xn;
+            break;
+        }
         if (!(xn > lo && xn < hi)) xn = 0.5 * (lo + hi);
-        const bool done = fabs(xn - x) <= 1e-15 * fabs(xn);
         x = xn;
-        if (done) break;
     }
     return x;
 }

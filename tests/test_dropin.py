@@ -216,6 +216,31 @@ def test_sigma_wrappers():
         assert L.candidate_sigma(p, numsum, ntr) == pytest.approx(sigma, abs=1e-3)
 
 
+def test_candidate_sigma_dense_grid():
+    """The significance the GPU gives every hit (pzl_stats.h, here its host
+    build behind accel.h's candidate_sigma) against the oracle's restatement of
+    the reference (pinned to the reference's dcdflib) on a dense grid: from the
+    detection threshold of each harmonic count to strong signals, both the
+    Poisson-sum and the asymptotic chi-square branches, the inverse normal tail
+    started from AS 241.  The stored sigma is a float: it must come out the
+    same float."""
+    L = D.api()
+    n = same = 0
+    worst = 0.0
+    for numsum in (1, 2, 4, 8, 16, 32):
+        for ntr in (1.0, 3.3e5, 1.4e10):
+            p0 = O.power_for_sigma(2.0, numsum, ntr)
+            for power in np.concatenate([p0 * np.linspace(1.0, 4.0, 700), p0 * np.logspace(0.61, 2.5, 120)]):
+                a = L.candidate_sigma(float(power), numsum, ntr)
+                b = O.candidate_sigma(float(power), numsum, ntr)
+                worst = max(worst, abs(a - b) / max(b, 1.0))
+                same += np.float32(a) == np.float32(b)
+                n += 1
+    print("%d points: max rel diff %.3g, same float %d" % (n, worst, same))
+    assert worst < 1e-12
+    assert same >= n - 2
+
+
 # --------------------------------------------------------------------- GPU
 
 
