@@ -510,6 +510,24 @@ __device__ __forceinline__ void corr_units(const PzlCorrArgs &a, long long u0, l
                 for (int r = 0; r < R0; r++) kv[i][r] = ok ? kr[j + r * NB0] : float2{0.f, 0.f};
             }
         }
+        if constexpr (SH == 0 && R0 % 2 == 0) {
+            // The two inputs r and r + R0/2 of a pass-0 butterfly share their
+            // segment value (the spread segment has period N/2), so the first
+            // radix-2 stage of the butterfly's DFT is done on the kernel row,
+            // once per unit: K[r] + K[r+R0/2], and (K[r] - K[r+R0/2]) times
+            // exp(-2 pi i r / R0) (the conjugate of that stage's twiddle, the
+            // product below conjugates the kernel value).
+            constexpr int HR = R0 / 2;
+            constexpr ExpTab<R0> et{};
+#pragma unroll
+            for (int i = 0; i < CP::per(0); i++)
+#pragma unroll
+                for (int r = 0; r < HR; r++) {
+                    const float2 k1 = kv[i][r], k2 = kv[i][r + HR];
+                    kv[i][r] = cadd(k1, k2);
+                    kv[i][r + HR] = cmul(csub(k1, k2), float2{et.c[r], et.s[r]});
+                }
+        }
         const int offset = 2 * H.halfwidth;
         // segment values of the next rstep are loaded while this one is transformed;
         // without the split, the spread segment's period N/2 lets an even radix
@@ -551,14 +569,36 @@ __device__ __forceinline__ void corr_units(const PzlCorrArgs &a, long long u0, l
                     int g, j;
                     F::template where<0>(bf, g, j);
                     float2 v[R0];
+                    if constexpr (HALFPER) {
+                        // outputs 2m and 2m+1 of the R0-point DFT: the R0/2-point
+                        // DFTs of seg * conj(the combined kernel values)
+                        constexpr int HR = R0 / 2;
+                        float2 ev[HR], od[HR];
 #pragma unroll
-                    for (int r = 0; r < R0; r++) {
-                        const float2 pd = cur[i][HALFPER ? r % (R0 / 2) : r];
-                        const float2 kk = kv[i][r];
-                        // re = dr*kr + di*ki, im = di*kr - dr*ki (accel_utils.c:1262-1265)
-                        v[r] = float2{pd.x * kk.x + pd.y * kk.y, pd.y * kk.x - pd.x * kk.y};
+                        for (int r = 0; r < HR; r++) {
+                            const float2 pd = cur[i][r];
+                            const float2 ka = kv[i][r], kb = kv[i][r + HR];
+                            // re = dr*kr + di*ki, im = di*kr - dr*ki (accel_utils.c:1262-1265)
+                            ev[r] = float2{pd.x * ka.x + pd.y * ka.y, pd.y * ka.x - pd.x * ka.y};
+                            od[r] = float2{pd.x * kb.x + pd.y * kb.y, pd.y * kb.x - pd.x * kb.y};
+                        }
+                        dftR<HR, 1>(ev);
+                        dftR<HR, 1>(od);
+#pragma unroll
+                        for (int m = 0; m < HR; m++) {
+                            v[2 * m] = ev[m];
+                            v[2 * m + 1] = od[m];
+                        }
+                    } else {
+#pragma unroll
+                        for (int r = 0; r < R0; r++) {
+                            const float2 pd = cur[i][r];
+                            const float2 kk = kv[i][r];
+                            // re = dr*kr + di*ki, im = di*kr - dr*ki (accel_utils.c:1262-1265)
+                            v[r] = float2{pd.x * kk.x + pd.y * kk.y, pd.y * kk.x - pd.x * kk.y};
+                        }
+                        dftR<R0, 1>(v);
                     }
-                    dftR<R0, 1>(v);
 #pragma unroll
                     for (int r = 0; r < R0; r++) b0[lds_pad<PADSH>(g * NF + j * R0 + r)] = v[r];
                 }
