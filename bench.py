@@ -463,6 +463,11 @@ def main():
     ap.add_argument("--overlap", type=int, default=-1,
                     help="profiling runs: 0 = one stream, 1 = harmonic sums on a side stream, 2 = also the next "
                          "batch's segments ahead and the shorter FFT lengths beside (the default)")
+    ap.add_argument("--cu-split", type=int, nargs=2, default=None, metavar=("SUM", "SUM_ONLY"),
+                    help="harmonic sums on the first SUM compute units, segments and correlations on all but the "
+                         "first SUM_ONLY (0 0: every kernel on every CU; default: the autotuner's choice)")
+    ap.add_argument("--sum-form", type=int, nargs=2, default=None, metavar=("ORDER", "RPT"),
+                    help="harmonic-sum workgroup order and r per thread (default: the autotuner's choice)")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
                     help="run a plain device copy of MIB MiB five times and exit (HBM counter calibration)")
     args = ap.parse_args()
@@ -499,6 +504,10 @@ def main():
         lo = g.nrsteps // 2
         hi = lo + args.rsteps
     tune = [] if args.no_tune else plan.autotune(reps=3)
+    if args.cu_split is not None:
+        plan.set_cu_split(*args.cu_split)
+    if args.sum_form is not None:
+        plan.set_sum_form(*args.sum_form)
     if args.overlap >= 0:
         plan.set_overlap(args.overlap)
     if args.graph >= 0:
@@ -610,6 +619,8 @@ def main():
         "variants": {str(c["fftlen"]): B.lib().pzl_corr_variant_name(c["fftlen"], plan.variant(c["fftlen"])).decode()
                      for c in stats_all[0]["classes"]},
         "corr_launches": corr_group_desc(plan),
+        "cu_split": dict(zip(("sum_cus", "sum_only_cus"), plan.cu_split())),
+        "sum_form": dict(zip(("order", "r_per_thread"), plan.sum_form())),
         "candidates": len(cands),
         "hits": stats_all[0]["nhits"],
     }

@@ -160,6 +160,18 @@ int pzl_autotune(void *plan, int reps, char *report, int report_len);
  * default; on = 0 launches every kernel one by one.  Per-kernel timing is then
  * reported per batch (ms_graph) instead of per kernel. */
 int pzl_plan_set_graph(void *plan, int on);
+/* CU split: the harmonic sums run on the first sum_cus compute units (spread
+ * evenly over the dies and shader engines), and the segments and correlations
+ * on every compute unit but the first sum_only_cus (0 <= sum_only_cus <=
+ * sum_cus), so that the memory-bound sum of a batch runs beside the next
+ * batch's correlation rather than in the workgroup slots it leaves.  Counts
+ * are multiples of the die count, at least 4 per die (one per shader engine).
+ * sum_cus = 0 (the default): every kernel may use every compute unit.  Waits
+ * for the plan's work and drops its batch graphs; the autotuner chooses a
+ * split for the plan.  No counterpart in the reference (its kernels run on
+ * one stream). */
+int pzl_plan_set_cu_split(void *plan, int sum_cus, int sum_only_cus);
+int pzl_plan_cu_split(void *plan, int *sum_cus, int *sum_only_cus);
 /* Overlap: on = 1: the harmonic sum of a batch runs on its own stream,
  * concurrently with the segments and correlations of the next batch (three
  * buffer sets), which stay in one stream (the arrangement of a replayed

@@ -161,6 +161,8 @@ def lib():
         L.pzl_plan_variant.argtypes = [C.c_void_p, C.c_int, P(C.c_int)]
         L.pzl_plan_set_graph.argtypes = [C.c_void_p, C.c_int]
         L.pzl_plan_set_overlap.argtypes = [C.c_void_p, C.c_int]
+        L.pzl_plan_set_cu_split.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.pzl_plan_cu_split.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.pzl_tune_batchsize.argtypes = [P(Config), C.c_int, P(C.c_float), C.c_longlong, P(C.c_int), C.c_int,
                                          C.c_longlong, P(C.c_int), P(C.c_double)]
         L.pzl_plan_set_variant.argtypes = [C.c_void_p, C.c_int, C.c_int]
@@ -386,6 +388,18 @@ class Plan:
 
     def set_graph(self, on=True):
         check(lib().pzl_plan_set_graph(self.h, int(bool(on))))
+
+    def set_cu_split(self, sum_cus, sum_only_cus=0):
+        """Harmonic sums on the first sum_cus compute units; segments and
+        correlations on all but the first sum_only_cus (0: every kernel on
+        every compute unit)."""
+        check(lib().pzl_plan_set_cu_split(self.h, int(sum_cus), int(sum_only_cus)))
+
+    def cu_split(self):
+        """(sum_cus, sum_only_cus)."""
+        a, b = C.c_int(0), C.c_int(0)
+        check(lib().pzl_plan_cu_split(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def set_overlap(self, on=2):
         """0 / False: one stream; 1: harmonic sums on their own stream,

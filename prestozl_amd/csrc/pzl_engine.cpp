@@ -359,10 +359,14 @@ void free_plan(Plan *p)
 {
     if (!p) return;
     (void)hipSetDevice(p->device);
-    if (p->st) (void)hipStreamSynchronize(p->st);
-    if (p->st3) (void)hipStreamSynchronize(p->st3);
-    if (p->st4) (void)hipStreamSynchronize(p->st4);
-    if (p->st2) (void)hipStreamSynchronize(p->st2);
+    // every stream the plan made, once each (st, st3 and st4 are among the
+    // ones created with the plan or the CU-masked ones)
+    std::vector<hipStream_t> streams = {p->st, p->st2, p->st3, p->st4, p->st_all[0], p->st_all[1], p->st_all[2]};
+    for (auto &m : p->masked) streams.push_back(m.s);
+    std::sort(streams.begin(), streams.end());
+    streams.erase(std::unique(streams.begin(), streams.end()), streams.end());
+    for (hipStream_t s : streams)
+        if (s) (void)hipStreamSynchronize(s);
     release_bank(p);
     (void)hipFree(p->d_arena);
     (void)hipFree(p->d_fft_own);
@@ -385,10 +389,8 @@ void free_plan(Plan *p)
         if (p->gexec[k]) (void)hipGraphExecDestroy(p->gexec[k]);
     pinned_free(p->h_r0);
     (void)hipFree(p->d_sums);
-    if (p->st) (void)hipStreamDestroy(p->st);
-    if (p->st2) (void)hipStreamDestroy(p->st2);
-    if (p->st3) (void)hipStreamDestroy(p->st3);
-    if (p->st4) (void)hipStreamDestroy(p->st4);
+    for (hipStream_t s : streams)
+        if (s) (void)hipStreamDestroy(s);
     delete p;
 }
 
@@ -1104,6 +1106,9 @@ int pzl_plan_create(const PzlConfig *cfg, int device, void **plan_out)
     PCHK(hipStreamCreateWithFlags(&p->st2, hipStreamNonBlocking));
     PCHK(hipStreamCreateWithFlags(&p->st3, hipStreamNonBlocking));
     PCHK(hipStreamCreateWithFlags(&p->st4, hipStreamNonBlocking));
+    p->st_all[0] = p->st;
+    p->st_all[1] = p->st3;
+    p->st_all[2] = p->st4;
     p->nslots = sizes.nslots;
     for (int s = 0; s < p->nslots; s++) {
         PCHK(hipEventCreateWithFlags(&p->ev_count[s], hipEventDisableTiming));
@@ -1448,7 +1453,9 @@ int pzl_autotune(void *plan, int reps, char *report, int report_len)
     PZL_HIPCHK(hipSetDevice(p->device));
     if (reps < 1) reps = 1;
     {
-        int grc = drop_graphs(*p);  // the variants may change below
+        // the variants may change below (drops the graphs), and every kernel is
+        // first timed on the whole chip
+        int grc = p->cu_split ? pzl_plan_set_cu_split(plan, 0, 0) : drop_graphs(*p);
         if (grc) return grc;
     }
     const long long nr = p->g.nrsteps;
@@ -1510,16 +1517,23 @@ int pzl_autotune(void *plan, int reps, char *report, int report_len)
         float best_ms = 1e30f;
         int best = p->sum_order, best_rpt = p->sum_rpt;
         const bool ov = p->overlap_sum;
-        p->overlap_sum = false;  // on p->st, behind the correlations
+        // on p->st behind the correlations, or, with a CU split, on the summing
+        // stream's own compute units after them
+        p->overlap_sum = p->cu_split > 0;
+        hipStream_t ts = p->sum_stream();
+        if (ts != p->st) {
+            PZL_HIPCHK(hipEventRecord(e0, p->st));
+            PZL_HIPCHK(hipStreamWaitEvent(ts, e0, 0));
+        }
         for (int form = 0; form < 6; form++) {
             const int order = form & 1, rpt = 1 << (form >> 1);
             p->sum_order = order;
             p->sum_rpt = rpt;
             int rc2 = launch_sum_part(*p, r0, nb, 0, 0, nullptr, nullptr);
-            hipError_t e = rc2 ? hipErrorUnknown : hipEventRecord(e0, p->st);
+            hipError_t e = rc2 ? hipErrorUnknown : hipEventRecord(e0, ts);
             for (int k = 0; k < reps && e == hipSuccess; k++)
                 e = launch_sum_part(*p, r0, nb, 0, 0, nullptr, nullptr) ? hipErrorUnknown : hipSuccess;
-            if (e == hipSuccess) e = hipEventRecord(e1, p->st);
+            if (e == hipSuccess) e = hipEventRecord(e1, ts);
             if (e == hipSuccess) e = hipEventSynchronize(e1);
             float ms = 0;
             if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
@@ -1537,6 +1551,7 @@ int pzl_autotune(void *plan, int reps, char *report, int report_len)
         p->sum_order = best;
         p->sum_rpt = best_rpt;
         p->overlap_sum = ov;
+        PZL_HIPCHK(hipStreamSynchronize(ts));
         PZL_HIPCHK(hipMemsetAsync(p->d_counts, 0, 2 * sizeof(unsigned long long), p->st));
         PZL_HIPCHK(hipStreamSynchronize(p->st));
     }
@@ -1665,6 +1680,65 @@ int pzl_autotune(void *plan, int reps, char *report, int report_len)
             g_tuned_group[group_key(*p)] = best;
         }
     }
+    // CU split (pzl_plan_set_cu_split): the harmonic sums on 2 or 3 CUs per
+    // shader engine, the segments and correlations on every CU (keeping CUs
+    // for the sums alone measured slower), each with the harmonic sum's 1 and
+    // 4 r per thread (on few CUs the sum is bound by address issue, which 4 r
+    // per thread cuts),
+    // timed on a band of 48 batches (best of two; in a short band the last
+    // batch's sum, slower on fewer CUs, would weigh as much as the steady
+    // state gains); the fastest split is kept when it is more than 2% faster
+    // than the whole chip for every kernel
+    const long long sband = 48LL * p->B;
+    if (nr >= sband && p->overlap_sum) {
+        const long long lo = (nr - sband) / 2;
+        int ncus = 0, nxcc = 1;
+        PZL_HIPCHK(hipDeviceGetAttribute(&ncus, hipDeviceAttributeMultiprocessorCount, p->device));
+        PZL_HIPCHK(hipDeviceGetAttribute(&nxcc, hipDeviceAttributeNumberOfXccs, p->device));
+        const int u = 4 * std::max(nxcc, 1);  // one CU per shader engine
+        struct Split {
+            int sum, only, rpt;
+        };
+        std::vector<Split> splits = {{0, 0, p->sum_rpt}};
+        for (int k = 2; k <= 3; k++)
+            for (int rpt : {1, 4})
+                if ((k + 1) * u <= ncus) splits.push_back({k * u, 0, rpt});
+        const int rpt0 = p->sum_rpt;
+        float base_ms = 1e30f, best_ms = 1e30f;  // the whole chip; the fastest split
+        Split best = splits[0];
+        for (const Split &sp : splits) {
+            rc = pzl_plan_set_cu_split(plan, sp.sum, sp.only);
+            if (rc) return rc;
+            p->sum_rpt = sp.rpt;
+            float ms = 1e30f;
+            for (int k = 0; k < 2; k++) {
+                long long n = 0;
+                const auto t0 = std::chrono::steady_clock::now();
+                rc = pzl_search(plan, lo, lo + sband, nullptr, 0, &n, nullptr);
+                if (rc) {
+                    p->sum_rpt = rpt0;
+                    (void)pzl_plan_set_cu_split(plan, 0, 0);
+                    return rc;
+                }
+                ms = std::min(ms, (float)std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() -
+                                                                                  t0).count());
+            }
+            snprintf(line, sizeof line,
+                     "harmonic sums on %d CUs (%d of them only for the sums), %d r per thread, in a search band of "
+                     "%lld rsteps: %.3f ms\n", sp.sum, sp.only, sp.rpt, sband, ms);
+            rep += line;
+            if (&sp == &splits[0]) {
+                base_ms = ms;
+            } else if (ms < best_ms) {
+                best_ms = ms;
+                best = sp;
+            }
+        }
+        if (!(best_ms < 0.98f * base_ms)) best = splits[0];
+        rc = pzl_plan_set_cu_split(plan, best.sum, best.only);
+        if (rc) return rc;
+        p->sum_rpt = best.rpt;
+    }
     if (report && report_len > 0) {
         snprintf(report, (size_t)report_len, "%s", rep.c_str());
     }
@@ -1691,6 +1765,82 @@ int pzl_plan_set_overlap(void *plan, int on)
     if (rc) return rc;
     p->overlap_sum = on != 0;
     p->side_classes = on >= 2;
+    return 0;
+}
+
+// The correlation's waves take every register of a SIMD, so a harmonic sum
+// launched beside it only gets workgroup slots the correlation gives up, and
+// the two kernels' times add.  A CU split confines the summing stream to
+// sum_cus compute units, the first sum_only_cus of which the segments and
+// correlations do not use: the memory-bound sum of one batch then runs beside
+// the VALU-bound correlation of the next.  Mask bit i is the i-th CU in the
+// driver's order, which deals consecutive bits to the dies and, within a die,
+// to its shader engines in turn, so the first n bits spread n CUs evenly;
+// workgroups are spread evenly over the shader engines, so a split is worth
+// having only in whole CUs per shader engine.
+int pzl_plan_set_cu_split(void *plan, int sum_cus, int sum_only_cus)
+{
+    Plan *p = (Plan *)plan;
+    if (!p) return fail(PZL_ERR_ARG, "null plan");
+    PZL_HIPCHK(hipSetDevice(p->device));
+    int ncus = 0, nxcc = 1;
+    PZL_HIPCHK(hipDeviceGetAttribute(&ncus, hipDeviceAttributeMultiprocessorCount, p->device));
+    PZL_HIPCHK(hipDeviceGetAttribute(&nxcc, hipDeviceAttributeNumberOfXccs, p->device));
+    // every shader engine of every die keeps compute units in each mask (a
+    // queue whose mask leaves a die without any never finishes that die's
+    // share of a launch): the same count on each die, at least one per shader
+    // engine (4 per die)
+    nxcc = std::max(nxcc, 1);
+    auto ok = [&](int n) { return n == 0 || (n % nxcc == 0 && n >= 4 * nxcc && ncus - n >= 4 * nxcc); };
+    if (sum_cus < 0 || sum_only_cus < 0 || sum_only_cus > sum_cus || !ok(sum_cus) || !ok(sum_only_cus))
+        return fail(PZL_ERR_ARG, "cu split %d/%d: counts of 0, or multiples of %d from %d to %d, the second at "
+                                 "most the first", sum_cus, sum_only_cus, nxcc, 4 * nxcc, ncus - 4 * nxcc);
+    int rc = drop_graphs(*p);
+    if (rc) return rc;
+    const int words = (ncus + 31) / 32;
+    std::vector<uint32_t> sum_mask(words, 0), rest_mask(words, 0);
+    for (int i = 0; i < ncus; i++) {
+        if (i < sum_cus) sum_mask[i / 32] |= 1u << (i % 32);
+        if (i >= sum_only_cus) rest_mask[i / 32] |= 1u << (i % 32);
+    }
+    // st, st3, st4: the sums on their CUs, the segments and correlations on
+    // the rest (every CU when sum_only_cus is 0).  The three are made together,
+    // once per split, and kept: a summing stream made alone beside the plan's
+    // first streams has had its launches wait for the correlation's in some
+    // processes (as if both shared one hardware dispatcher), never one of
+    // three made at once
+    hipStream_t s[3] = {p->st_all[0], p->st_all[1], p->st_all[2]};
+    if (sum_cus) {
+        bool have = false;
+        for (const auto &m : p->masked)
+            if (m.first == sum_only_cus && m.count == sum_cus) {
+                s[m.role] = m.s;
+                have = true;
+            }
+        if (!have) {
+            hipError_t e = hipSuccess;
+            for (int k = 0; k < 3 && e == hipSuccess; k++) {
+                e = hipExtStreamCreateWithCUMask(&s[k], (uint32_t)words, (k == 1 ? sum_mask : rest_mask).data());
+                if (e == hipSuccess) p->masked.push_back({k, sum_only_cus, sum_cus, s[k]});
+            }
+            if (e != hipSuccess)
+                return fail(PZL_ERR_HIP, "cu split %d/%d: %s", sum_cus, sum_only_cus, hipGetErrorString(e));
+        }
+    }
+    p->st = s[0];
+    p->st3 = s[1];
+    p->st4 = s[2];
+    p->cu_split = sum_cus;
+    p->cu_split_only = sum_only_cus;
+    return 0;
+}
+
+int pzl_plan_cu_split(void *plan, int *sum_cus, int *sum_only_cus)
+{
+    Plan *p = (Plan *)plan;
+    if (!p) return fail(PZL_ERR_ARG, "null plan");
+    if (sum_cus) *sum_cus = p->cu_split;
+    if (sum_only_cus) *sum_only_cus = p->cu_split_only;
     return 0;
 }
 

@@ -56,6 +56,17 @@ struct Plan {
     bool overlap_sum = true;    // harmonic sums on st3, concurrent with the next batch's correlation
     bool side_classes = true;   // with overlap_sum: the next batch's segments ahead on st4, and the shorter
                                 // FFT lengths correlated there beside the longest (unless grouped)
+    int cu_split = 0;           // > 0: st3 on the first cu_split compute units ...
+    int cu_split_only = 0;      // ... and st, st4 on all but the first cu_split_only
+    hipStream_t st_all[3] = {};  // st, st3, st4 as created with the plan (every compute unit)
+    // CU-masked streams of each split used (role 0: st, 1: st3, 2: st4; the
+    // split's sum_only_cus and sum_cus), made together and kept until the plan
+    // is destroyed
+    struct MaskedStream {
+        int role, first, count;
+        hipStream_t s;
+    };
+    std::vector<MaskedStream> masked;
     // resident
     PzlHarm *d_harms = nullptr;
     float2 *d_bank = nullptr;  // shared, owned by the bank cache

@@ -79,6 +79,13 @@ hipError_t hipMemcpy2D(void *dst, size_t dpitch, const void *src, size_t spitch,
 hipError_t hipMemset(void *p, int v, size_t n);
 hipError_t hipMemsetAsync(void *p, int v, size_t n, hipStream_t st);
 hipError_t hipStreamCreateWithFlags(hipStream_t *st, unsigned flags);
+// CU masks are accepted and ignored by the stand-in (its streams are threads)
+hipError_t hipExtStreamCreateWithCUMask(hipStream_t *st, uint32_t mask_words, const uint32_t *mask);
+typedef enum hipDeviceAttribute_t {
+    hipDeviceAttributeMultiprocessorCount = 1,
+    hipDeviceAttributeNumberOfXccs = 2
+} hipDeviceAttribute_t;
+hipError_t hipDeviceGetAttribute(int *v, hipDeviceAttribute_t a, int device);
 hipError_t hipStreamDestroy(hipStream_t st);
 hipError_t hipStreamSynchronize(hipStream_t st);
 hipError_t hipStreamWaitEvent(hipStream_t st, hipEvent_t ev, unsigned flags);

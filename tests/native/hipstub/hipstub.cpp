@@ -247,6 +247,15 @@ hipError_t hipStreamCreateWithFlags(hipStream_t *st, unsigned)
     g_streams.insert(*st);
     return hipSuccess;
 }
+hipError_t hipExtStreamCreateWithCUMask(hipStream_t *st, uint32_t, const uint32_t *)
+{
+    return hipStreamCreateWithFlags(st, 0);
+}
+hipError_t hipDeviceGetAttribute(int *v, hipDeviceAttribute_t a, int)
+{
+    *v = a == hipDeviceAttributeMultiprocessorCount ? 256 : a == hipDeviceAttributeNumberOfXccs ? 8 : 0;
+    return hipSuccess;
+}
 hipError_t hipStreamDestroy(hipStream_t st)
 {
     {

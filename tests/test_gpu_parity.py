@@ -581,6 +581,37 @@ def test_graph_replay_identical(config1_data, which):
     assert all(np.array_equal(a, b) for a, b in zip(h_graph[0], h_eager[0]))
 
 
+@pytest.mark.parametrize("which", ["c1", "c2"])
+def test_cu_split_identical(config1_data, which):
+    """Harmonic sums confined to some compute units (with the correlations on
+    every CU, or kept off the sums' own) give the whole chip's hits and
+    candidates exactly, eager and replayed; counts that would leave a shader
+    engine without CUs are refused."""
+    fft, kw = _c1(config1_data) if which == "c1" else _c2_small()
+    plan = _plan(kw, fft)
+    lo, hi = (0, -1) if which == "c1" else (40, 80)
+    assert plan.cu_split() == (0, 0)
+    c0, _ = plan.search(lo, hi, stats=False)
+    h0 = plan.hits()
+    for split in ((64, 0), (64, 32), (96, 32)):
+        plan.set_cu_split(*split)
+        assert plan.cu_split() == split
+        for graph in (False, True):
+            plan.set_graph(graph)
+            c, _ = plan.search(lo, hi, stats=False)
+            h = plan.hits()
+            print("split %s graph %d: %d candidates" % (split, graph, len(c)))
+            assert c == c0
+            assert np.array_equal(h[1], h0[1])
+            assert all(np.array_equal(a, b) for a, b in zip(h[0], h0[0]))
+    for bad in ((8, 0), (64, 96), (36, 0), (-32, 0)):
+        with pytest.raises(RuntimeError):
+            plan.set_cu_split(*bad)
+    plan.set_cu_split(0, 0)
+    assert plan.search(lo, hi, stats=False)[0] == c0
+    plan.close()
+
+
 def test_tune_batchsize(config1_data):
     """The rsteps-per-launch autotuner returns one of the candidates, and the
     search it would configure gives the same candidates as batch size 8."""
