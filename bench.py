@@ -384,10 +384,13 @@ def config3_bench(steps):
             plan.autotune(reps=2)
             variants = {f: plan.variant(f) for f in plan.fftlens()}
             group = plan.corr_group()
+            split, sum_form = plan.cu_split(), plan.sum_form()
         else:  # the tuned plan's kernel variants: the same arithmetic, so the same candidates
             for f, v in variants.items():
                 plan.set_variant(f, v)
             plan.set_corr_group(*group)
+            plan.set_cu_split(*split)
+            plan.set_sum_form(*sum_form)
         plan.set_graph(graph)
         plan.search(stats=False)
         t0 = time.perf_counter()
@@ -402,7 +405,8 @@ def config3_bench(steps):
     return {"workload": "N=2^23, dt=1e-4 s, zmax=200, wmax=500, numharm=16, sigma=5",
             "value": cells / tb, "unit": "cells/s", "ms_per_search": tb * 1e3, "batchsize": bt, "graph": True,
             "batchsize_ms_per_rstep": ms, "at_batchsize_8": {"value": cells / t8, "ms_per_search": t8 * 1e3},
-            "same_candidates": cb == c8, "nrsteps": g.nrsteps, "fftlen": g.fftlen, "candidates": len(cb)}
+            "same_candidates": cb == c8, "nrsteps": g.nrsteps, "fftlen": g.fftlen, "candidates": len(cb),
+            "cu_split": dict(zip(("sum_cus", "sum_only_cus"), split))}
 
 
 def measured_traffic(kernel):
@@ -540,6 +544,10 @@ def main():
         roof_note = "kernel durations from one search after the timed graph-replay searches, launching the same " \
                     "kernels one by one in the same arrangement (segments and correlations in one stream, " \
                     "harmonic sums beside them)"
+    split = plan.cu_split()
+    if split[0] > 0:
+        roof_note += "; the harmonic sums confined to %d compute units that the correlation also uses (the CU " \
+                     "split), so the correlation's launches share the chip with them by design" % split[0]
     cls = {}
     for st in roof_stats:
         for c in st["classes"]:

@@ -1685,12 +1685,12 @@ int pzl_autotune(void *plan, int reps, char *report, int report_len)
     // for the sums alone measured slower), each with the harmonic sum's 1 and
     // 4 r per thread (on few CUs the sum is bound by address issue, which 4 r
     // per thread cuts),
-    // timed on a band of 48 batches (best of two; in a short band the last
-    // batch's sum, slower on fewer CUs, would weigh as much as the steady
-    // state gains); the fastest split is kept when it is more than 2% faster
-    // than the whole chip for every kernel
-    const long long sband = 48LL * p->B;
-    if (nr >= sband && p->overlap_sum) {
+    // timed on a band of 384 rsteps and at least 16 batches (best of two; in a
+    // short band the last batch's sum, slower on fewer CUs, would weigh as much
+    // as the steady state gains); the fastest split is kept when it is more
+    // than 2% faster than the whole chip for every kernel
+    const long long sband = std::min<long long>(nr, std::max<long long>(16LL * p->B, 384));
+    if (nr >= 16LL * p->B && p->overlap_sum) {
         const long long lo = (nr - sband) / 2;
         int ncus = 0, nxcc = 1;
         PZL_HIPCHK(hipDeviceGetAttribute(&ncus, hipDeviceAttributeMultiprocessorCount, p->device));
