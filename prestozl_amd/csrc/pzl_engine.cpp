@@ -1680,11 +1680,12 @@ int pzl_autotune(void *plan, int reps, char *report, int report_len)
             g_tuned_group[group_key(*p)] = best;
         }
     }
-    // CU split (pzl_plan_set_cu_split): the harmonic sums on 2 or 3 CUs per
+    // CU split (pzl_plan_set_cu_split): the harmonic sums on 3 or 2 CUs per
     // shader engine, the segments and correlations on every CU (keeping CUs
-    // for the sums alone measured slower), each with the harmonic sum's 1 and
-    // 4 r per thread (on few CUs the sum is bound by address issue, which 4 r
-    // per thread cuts),
+    // for the sums alone measured slower), each with the harmonic sum's 4 and
+    // 1 r per thread (on few CUs the sum is bound by address issue, which 4 r
+    // per thread cuts); a later split replaces an earlier one when it is 1%
+    // faster,
     // timed on a band of 384 rsteps and at least 16 batches (best of two; in a
     // short band the last batch's sum, slower on fewer CUs, would weigh as much
     // as the steady state gains); the fastest split is kept when it is more
@@ -1700,8 +1701,8 @@ int pzl_autotune(void *plan, int reps, char *report, int report_len)
             int sum, only, rpt;
         };
         std::vector<Split> splits = {{0, 0, p->sum_rpt}};
-        for (int k = 2; k <= 3; k++)
-            for (int rpt : {1, 4})
+        for (int k = 3; k >= 2; k--)
+            for (int rpt : {4, 1})
                 if ((k + 1) * u <= ncus) splits.push_back({k * u, 0, rpt});
         const int rpt0 = p->sum_rpt;
         float base_ms = 1e30f, best_ms = 1e30f;  // the whole chip; the fastest split
@@ -1729,7 +1730,9 @@ int pzl_autotune(void *plan, int reps, char *report, int report_len)
             rep += line;
             if (&sp == &splits[0]) {
                 base_ms = ms;
-            } else if (ms < best_ms) {
+            } else if (best_ms > 1e29f || ms < 0.99f * best_ms) {
+                // the first split, or one clearly faster: more summing CUs
+                // (tried first) leave the sums more room when hits come in bursts
                 best_ms = ms;
                 best = sp;
             }

@@ -1011,7 +1011,12 @@ struct HitPrepArgs {
 
 __global__ __launch_bounds__(256) void k_hitprep(HitPrepArgs a)
 {
-    for (long long i = blockIdx.x * 256ll + threadIdx.x; i < a.n; i += (long long)gridDim.x * 256) {
+    // one hit per thread, no loop: constants of the significance are not kept
+    // live across hits, so the kernel stays within the registers a finished
+    // correlation wave frees (it runs beside the correlation)
+    {
+        const long long i = blockIdx.x * 256ll + threadIdx.x;
+        if (i >= a.n) return;
         const long long key = a.keys[i];
         long long k = key;
         const int r = (int)(k % a.corr_uselen);
@@ -1115,8 +1120,8 @@ extern "C" hipError_t pzl_launch_hitprep(const long long *keys, const float *pow
     a.nstages = nstages;
     for (int s = 0; s < 6; s++) a.numindep[s] = numindep[s];
     a.out = out;
-    long long blocks = (n + 255) / 256;
-    if (blocks > 8192) blocks = 8192;
+    const long long blocks = (n + 255) / 256;
+    if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_hitprep, dim3((unsigned)blocks), dim3(256), 0, st, a);
     return hipGetLastError();
 }
