@@ -620,6 +620,8 @@ def main():
                                  "corr": np.mean([s["ms_corr"] for s in roof_stats]),
                                  "harmsum": np.mean([s["ms_sum"] for s in roof_stats]),
                                  "host": np.mean([s["ms_host"] for s in stats_all]),
+                                 "host_wait_count": np.mean([s["ms_wait_count"] for s in stats_all]),
+                                 "host_wait_pool": np.mean([s["ms_wait_pool"] for s in stats_all]),
                                  "by_fftlen": {str(k): v["ms"] / len(roof_stats) for k, v in sorted(cls.items())},
                                  "grouped": np.mean([s["ms_corr_group"] for s in roof_stats]),
                                  "graph_replay": np.mean([s["ms_graph"] for s in stats_all])},

@@ -99,6 +99,9 @@ typedef struct PzlStats {
     double ms_corr_group;         /* summed duration of the grouped correlation launches (the
                                      class_ms of the lengths they hold are then 0) */
     long long group_launches;
+    double ms_wait_count;  /* launching thread waiting for the oldest batch in flight (its hit count) */
+    double ms_wait_pool;   /* launching thread waiting for a pinned hit-record buffer (the insertion
+                              thread behind) */
 } PzlStats;
 
 const char *pzl_last_error(void);

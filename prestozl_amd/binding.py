@@ -57,7 +57,8 @@ class Stats(C.Structure):
                 ("class_ms", C.c_double * 16), ("class_launches", C.c_longlong * 16),
                 ("class_bytes", C.c_double * 16), ("ms_graph", C.c_double), ("graph_launches", C.c_longlong),
                 ("class_bank_bytes", C.c_double * 16), ("class_flops", C.c_double * 16),
-                ("ms_corr_group", C.c_double), ("group_launches", C.c_longlong)]
+                ("ms_corr_group", C.c_double), ("group_launches", C.c_longlong),
+                ("ms_wait_count", C.c_double), ("ms_wait_pool", C.c_double)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_ if not k.startswith("class_")}

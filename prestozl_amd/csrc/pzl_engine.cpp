@@ -698,6 +698,8 @@ struct SearchState {
     Plan *p = nullptr;
     CandList cl;
     double ms_host = 0;  // insertion thread busy time
+    double ms_wait_pool = 0;   // launching thread waiting for a free pinned record buffer
+    double ms_wait_count = 0;  // launching thread waiting for a batch's hit count (its harmonic sum)
     long long nhits = 0;
     int err = 0;         // first failure seen by the insertion thread
     std::string err_msg;
@@ -718,42 +720,32 @@ struct SearchState {
         cv.notify_all();
         if (th.joinable()) th.join();
     }
-    // A free pool buffer for n records: the smallest idle one large enough,
+    // Records travel to the insertion thread in chunks of kChunk records.
+    // Every pool buffer holds one chunk; it is made once (from the
+    // process-wide cache, else pinned) and reused, so that a plan stops
+    // pinning or freeing host memory once its pool has grown: pinning waits
+    // for the device and takes milliseconds, and a launching thread doing it
+    // in the middle of a search leaves the GPU without work.  A free buffer,
     // else a new one while the pool stays within its limit, else wait for the
-    // insertion thread to release one.  Buffers are not freed during a search
-    // (freeing pinned memory can wait for the device), except, with nothing in
-    // flight, idle ones too small to reuse.
-    int acquire(long long n)
+    // insertion thread to release one.
+    static constexpr long long kChunk = 1LL << 20;
+    int acquire()
     {
-        const long long want = std::max<long long>(2 * n, 1 << 20);
-        const long long add = (long long)sizeof(PzlHostHit) * want;
         std::unique_lock<std::mutex> lk(mu);
         for (;;) {
             if (err) return -1;
-            int fit = -1, nbusy = 0;
             for (int k = 0; k < (int)p->hpool.size(); k++) {
-                const auto &hb = p->hpool[k];
-                if (hb.busy) {
-                    nbusy++;
-                } else if (hb.cap >= n && (fit < 0 || hb.cap < p->hpool[fit].cap)) {
-                    fit = k;
+                auto &hb = p->hpool[k];
+                if (!hb.busy && hb.ptr && hb.cap >= kChunk) {
+                    hb.busy = true;
+                    return k;
                 }
             }
-            if (fit >= 0) {
-                p->hpool[fit].busy = true;
-                return fit;
-            }
-            if (p->hpool_bytes + add > p->hpool_limit && nbusy > 0) {
+            if (p->hpool_bytes + (long long)sizeof(PzlHostHit) * kChunk > p->hpool_limit) {
+                const auto w0 = std::chrono::steady_clock::now();
                 cv.wait(lk);
+                ms_wait_pool += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
                 continue;
-            }
-            if (p->hpool_bytes + add > p->hpool_limit) {
-                for (auto &hb : p->hpool) {
-                    pin_put(hb.ptr, hb.cap);
-                    hb.ptr = nullptr;
-                    hb.cap = 0;
-                }
-                p->hpool_bytes = 0;
             }
             int k = -1;
             for (int q = 0; q < (int)p->hpool.size(); q++)
@@ -767,9 +759,9 @@ struct SearchState {
                 }
             }
             auto &hb = p->hpool[k];
-            long long cap = want;
-            hb.ptr = pin_take(want, &cap);
-            if (!hb.ptr && pinned_alloc((void **)&hb.ptr, sizeof(PzlHostHit) * want) != hipSuccess) {
+            long long cap = kChunk;
+            hb.ptr = pin_take(kChunk, &cap);
+            if (!hb.ptr && pinned_alloc((void **)&hb.ptr, sizeof(PzlHostHit) * kChunk) != hipSuccess) {
                 hb.ptr = nullptr;
                 return -1;
             }
@@ -864,16 +856,21 @@ int apply_slot(Plan &p, int slot, long long n, SearchState &ss)
     if (rc) return rc;
     PZL_HIPCHK(pzl_launch_hitprep(p.d_skeys, p.d_spows, n, p.d_startr, p.g.corr_uselen, p.fnumz, p.fnumw,
                                   p.g.numharmstages, p.numindep_d, p.d_recs, p.st2));
-    const int buf = ss.acquire(n);
-    if (buf < 0)
-        return fail(ss.err ? ss.err : PZL_ERR_HIP, "no pinned buffer for %lld hit records%s%s", n,
-                    ss.err_msg.empty() ? "" : ": ", ss.err_msg.c_str());
-    if (p.hpool[buf].cap < n) return fail(PZL_ERR_HIP, "pinned buffer of %lld records for %lld hits", p.hpool[buf].cap, n);
-    PZL_PINNED_CHECK(p.hpool[buf].ptr, sizeof(PzlHostHit) * n);
-    PZL_HIPCHK(hipMemcpyAsync(p.hpool[buf].ptr, p.d_recs, sizeof(PzlHostHit) * n, hipMemcpyDeviceToHost, p.st2));
-    PZL_HIPCHK(hipEventRecord(p.hpool[buf].ev, p.st2));
+    // the records in chunks of pool buffers, in order (the insertion thread
+    // applies the jobs first in, first out)
+    for (long long off = 0; off < n; off += SearchState::kChunk) {
+        const long long m = std::min(SearchState::kChunk, n - off);
+        const int buf = ss.acquire();
+        if (buf < 0)
+            return fail(ss.err ? ss.err : PZL_ERR_HIP, "no pinned buffer for %lld hit records%s%s", m,
+                        ss.err_msg.empty() ? "" : ": ", ss.err_msg.c_str());
+        PZL_PINNED_CHECK(p.hpool[buf].ptr, sizeof(PzlHostHit) * m);
+        PZL_HIPCHK(hipMemcpyAsync(p.hpool[buf].ptr, p.d_recs + off, sizeof(PzlHostHit) * m, hipMemcpyDeviceToHost,
+                                  p.st2));
+        PZL_HIPCHK(hipEventRecord(p.hpool[buf].ev, p.st2));
+        ss.submit(buf, m);
+    }
     ss.nhits += n;
-    ss.submit(buf, n);
     return 0;
 }
 
@@ -889,7 +886,9 @@ int reset_slot(Plan &p, int slot, hipStream_t st)
 // exceeds a slot: hitcap >= cells per rstep x stages).
 int finalize(Plan &p, int slot, long long r0, int nb, SearchState &ss, Timers *tm)
 {
+    const auto w0 = std::chrono::steady_clock::now();
     PZL_HIPCHK(hipEventSynchronize(p.ev_count[slot]));
+    ss.ms_wait_count += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
     const unsigned long long n = p.h_counts[2 * slot];
     if ((long long)n <= p.hitcap) return apply_slot(p, slot, (long long)n, ss);  // resets the slot
     // overflow: let every batch in flight finish, then search this batch
@@ -1396,6 +1395,8 @@ int pzl_search(void *plan, long long rstep_lo, long long rstep_hi, PzlCand *cand
         stats->ms_gpu = ms;
         stats->ms_wall = std::chrono::duration<double, std::milli>(t2 - t0).count();
         stats->ms_host = ss.ms_host;
+        stats->ms_wait_pool = ss.ms_wait_pool;
+        stats->ms_wait_count = ss.ms_wait_count;
         stats->nhits = ss.nhits;
         stats->nrsteps = rstep_hi - rstep_lo;
         stats->cells = (double)stats->nrsteps * p->g.corr_uselen * p->g.numz * std::max(p->g.numw, 1);

@@ -110,9 +110,10 @@ struct Plan {
     PzlHostHit *d_recs = nullptr;
     void *d_sort_temp = nullptr;
     size_t sort_temp_bytes = 0;
-    // pinned hit-record buffers handed to the insertion thread: each carries
-    // the event of the copy that fills it; the pool grows up to hpool_limit
-    // bytes, so that bursts of hits never hold up the launching thread
+    // pinned hit-record buffers handed to the insertion thread, one chunk of a
+    // batch's records each, with the event of the copy that fills it; the
+    // pool grows up to hpool_limit bytes and keeps its buffers, so that bursts
+    // of hits never make the launching thread pin memory
     struct HostBuf {
         PzlHostHit *ptr = nullptr;
         long long cap = 0;
