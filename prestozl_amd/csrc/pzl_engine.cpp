@@ -361,7 +361,8 @@ void free_plan(Plan *p)
     (void)hipSetDevice(p->device);
     // every stream the plan made, once each (st, st3 and st4 are among the
     // ones created with the plan or the CU-masked ones)
-    std::vector<hipStream_t> streams = {p->st, p->st2, p->st3, p->st4, p->st_all[0], p->st_all[1], p->st_all[2]};
+    std::vector<hipStream_t> streams = {p->st,        p->st2,       p->st3,      p->st4,
+                                        p->st5,       p->st_all[0], p->st_all[1], p->st_all[2]};
     for (auto &m : p->masked) streams.push_back(m.s);
     std::sort(streams.begin(), streams.end());
     streams.erase(std::unique(streams.begin(), streams.end()), streams.end());
@@ -380,6 +381,8 @@ void free_plan(Plan *p)
         if (p->ev_count[s]) (void)hipEventDestroy(p->ev_count[s]);
         if (p->ev_free[s]) (void)hipEventDestroy(p->ev_free[s]);
     }
+    if (p->ev_prep) (void)hipEventDestroy(p->ev_prep);
+    if (p->ev_copied) (void)hipEventDestroy(p->ev_copied);
     pinned_free(p->h_counts);
     for (auto &hb : p->hpool) {
         pin_put(hb.ptr, hb.cap);
@@ -686,6 +689,7 @@ int sync_all(Plan &p)
     PZL_HIPCHK(hipStreamSynchronize(p.st));
     PZL_HIPCHK(hipStreamSynchronize(p.st3));
     PZL_HIPCHK(hipStreamSynchronize(p.st2));
+    PZL_HIPCHK(hipStreamSynchronize(p.st5));
     return 0;
 }
 
@@ -854,8 +858,15 @@ int apply_slot(Plan &p, int slot, long long n, SearchState &ss)
     // while the records are prepared and copied
     int rc = reset_slot(p, slot, p.st2);
     if (rc) return rc;
+    // The records go to host memory on a stream of their own (st5), so that
+    // the next slot's sort, which releases that slot, does not queue behind
+    // the copies (hundreds of MB for a batch with millions of hits); only the
+    // next preparation waits for them, as it rewrites the record buffer.
+    PZL_HIPCHK(hipStreamWaitEvent(p.st2, p.ev_copied, 0));
     PZL_HIPCHK(pzl_launch_hitprep(p.d_skeys, p.d_spows, n, p.d_startr, p.g.corr_uselen, p.fnumz, p.fnumw,
                                   p.g.numharmstages, p.numindep_d, p.d_recs, p.st2));
+    PZL_HIPCHK(hipEventRecord(p.ev_prep, p.st2));
+    PZL_HIPCHK(hipStreamWaitEvent(p.st5, p.ev_prep, 0));
     // the records in chunks of pool buffers, in order (the insertion thread
     // applies the jobs first in, first out)
     for (long long off = 0; off < n; off += SearchState::kChunk) {
@@ -866,10 +877,11 @@ int apply_slot(Plan &p, int slot, long long n, SearchState &ss)
                         ss.err_msg.empty() ? "" : ": ", ss.err_msg.c_str());
         PZL_PINNED_CHECK(p.hpool[buf].ptr, sizeof(PzlHostHit) * m);
         PZL_HIPCHK(hipMemcpyAsync(p.hpool[buf].ptr, p.d_recs + off, sizeof(PzlHostHit) * m, hipMemcpyDeviceToHost,
-                                  p.st2));
-        PZL_HIPCHK(hipEventRecord(p.hpool[buf].ev, p.st2));
+                                  p.st5));
+        PZL_HIPCHK(hipEventRecord(p.hpool[buf].ev, p.st5));
         ss.submit(buf, m);
     }
+    PZL_HIPCHK(hipEventRecord(p.ev_copied, p.st5));
     ss.nhits += n;
     return 0;
 }
@@ -1105,6 +1117,9 @@ int pzl_plan_create(const PzlConfig *cfg, int device, void **plan_out)
     PCHK(hipStreamCreateWithFlags(&p->st2, hipStreamNonBlocking));
     PCHK(hipStreamCreateWithFlags(&p->st3, hipStreamNonBlocking));
     PCHK(hipStreamCreateWithFlags(&p->st4, hipStreamNonBlocking));
+    PCHK(hipStreamCreateWithFlags(&p->st5, hipStreamNonBlocking));
+    PCHK(hipEventCreateWithFlags(&p->ev_prep, hipEventDisableTiming));
+    PCHK(hipEventCreateWithFlags(&p->ev_copied, hipEventDisableTiming));
     p->st_all[0] = p->st;
     p->st_all[1] = p->st3;
     p->st_all[2] = p->st4;

@@ -53,6 +53,9 @@ struct Plan {
     hipStream_t st2 = nullptr;  // hit ordering
     hipStream_t st3 = nullptr;  // harmonic sums (when overlapped)
     hipStream_t st4 = nullptr;  // segments (when overlapped), ahead of the correlation
+    hipStream_t st5 = nullptr;  // hit records to host memory
+    hipEvent_t ev_prep = nullptr;    // the last slot's records are prepared (st2)
+    hipEvent_t ev_copied = nullptr;  // the last slot's records are copied out (st5)
     bool overlap_sum = true;    // harmonic sums on st3, concurrent with the next batch's correlation
     bool side_classes = true;   // with overlap_sum: the next batch's segments ahead on st4, and the shorter
                                 // FFT lengths correlated there beside the longest (unless grouped)
