@@ -156,10 +156,12 @@ int pzl_plan_set_deadline(void *plan, double ms);
  * same shape on the device.  report (optional) receives one line per variant.
  * Replaces the reference's per-shape cuFFT plan choice (accel_kernels.cu:96-158). */
 int pzl_autotune(void *plan, int reps, char *report, int report_len);
-/* Graph replay: the segments and correlations of every full batch of rsteps
- * are launched as one captured hipGraph (one per buffer set, captured on the
- * set's first full batch), the batch's first rstep read from device memory;
- * the harmonic sum stays a launch of its own, beside the next batch.  On by
+/* Graph replay: the correlations of every full batch of rsteps are launched
+ * as one hipGraph (one per buffer set, built from the launches of the set's
+ * first full batch); the batch's segments are launched before it (ahead on
+ * their own stream, beside the previous batch's correlation, when
+ * overlapped), and the harmonic sum stays a launch of its own, beside the
+ * next batch.  On by
  * default; on = 0 launches every kernel one by one.  Per-kernel timing is then
  * reported per batch (ms_graph) instead of per kernel. */
 int pzl_plan_set_graph(void *plan, int on);

@@ -190,7 +190,6 @@ std::vector<long long> plan_arena_items(const PzlGeometry &g, const PlanSizes &z
         v.push_back((long long)sizeof(float) * z.hitcap);
     }
     v.push_back((long long)sizeof(unsigned long long) * 2 * Plan::kSlots);
-    v.push_back((long long)sizeof(long long) * Plan::kSets);
     v.push_back((long long)sizeof(long long) * z.hitcap);
     v.push_back((long long)sizeof(float) * z.hitcap);
     v.push_back((long long)sizeof(PzlHostHit) * z.hitcap);
@@ -390,7 +389,6 @@ void free_plan(Plan *p)
     }
     for (int k = 0; k < Plan::kSets; k++)
         if (p->gexec[k]) (void)hipGraphExecDestroy(p->gexec[k]);
-    pinned_free(p->h_r0);
     (void)hipFree(p->d_sums);
     for (hipStream_t s : streams)
         if (s) (void)hipStreamDestroy(s);
@@ -463,10 +461,13 @@ PzlCorrArgs corr_args(const Plan &p, const SizeClass &c, int nb, int set)
     return ca;
 }
 
-// Segments and correlations of rsteps [r0, r0 + nb) into buffer set 'set',
-// on the correlation stream (r0_dev: read the first rstep from there).
+int launch_corrs(Plan &p, int nb, int set, Timers *tm, bool seg_stream);
+
+// Segments of rsteps [r0, r0 + nb) into buffer set 'set' (r0_dev: read the
+// first rstep from there), then, unless segments_only, their correlations on
+// the correlation stream.
 int launch_corr_part(Plan &p, long long r0, int nb, int set, Timers *tm, const long long *r0_dev,
-                     bool seg_stream = false)
+                     bool seg_stream = false, bool segments_only = false)
 {
     PzlBatchArgs ba{};
     ba.rstep0_dev = r0_dev;
@@ -497,6 +498,14 @@ int launch_corr_part(Plan &p, long long r0, int nb, int set, Timers *tm, const l
         PZL_HIPCHK(hipEventRecord(p.ev_seg[set], ss));
         PZL_HIPCHK(hipStreamWaitEvent(p.st, p.ev_seg[set], 0));
     }
+    if (segments_only) return 0;
+    return launch_corrs(p, nb, set, tm, seg_stream);
+}
+
+// The correlations of a batch whose segments are in buffer set 'set'.
+int launch_corrs(Plan &p, int nb, int set, Timers *tm, bool seg_stream)
+{
+    hipStream_t ss = seg_stream ? p.st4 : p.st;
     // With the segments on their own stream, the shorter FFT lengths (small
     // grids that alone leave most of the chip idle) can follow them there and
     // run beside the longest length's correlation on the main stream.
@@ -592,11 +601,13 @@ int drop_graphs(Plan &p)
     return 0;
 }
 
-// The correlation part of a full batch as one graph launch: the graph of a
-// buffer set (segments and every correlation launch) is built once, with the
-// batch's first rstep read from device memory, and replayed for every later
-// full batch in that set; the first batch of each set runs eagerly (it also
-// builds what the kernels read once per device).  The graph is built from the
+// The correlations of a full batch as one graph launch: the graph of a
+// buffer set (every correlation launch of a batch; they do not depend on the
+// batch's rsteps) is built once and replayed for every later full batch in
+// that set, after the batch's segments, which are launched as they are (ahead
+// on their own stream when overlapped, beside the previous batch's
+// correlation); the first batch of each set runs eagerly (it also builds what
+// the kernels read once per device).  The graph is built from the
 // recorded launches with hipGraphAddKernelNode, in launch order, each node
 // depending on the one before: no stream is captured, so HIP calls made
 // meanwhile by other threads are unaffected.  A graph the runtime refuses is
@@ -606,7 +617,7 @@ int build_batch_graph(Plan &p, int set, hipGraphExec_t *out)
 {
     std::vector<PzlKernelRec> recs;
     pzl_recorder = &recs;
-    const int rc = launch_corr_part(p, 0, p.B, set, nullptr, p.d_r0 + set);
+    const int rc = launch_corrs(p, p.B, set, nullptr, false);
     pzl_recorder = nullptr;
     if (rc) return rc;
     hipGraph_t g = nullptr;
@@ -635,10 +646,10 @@ int build_batch_graph(Plan &p, int set, hipGraphExec_t *out)
     return 0;
 }
 
-int launch_corr_graph(Plan &p, long long r0, int set, Timers *tm)
+int launch_corr_graph(Plan &p, long long r0, int set, Timers *tm, bool seg_ahead)
 {
     if (!p.gexec[set]) {
-        int rc = launch_corr_part(p, r0, p.B, set, tm, nullptr);
+        int rc = launch_corr_part(p, r0, p.B, set, tm, nullptr, seg_ahead);
         if (rc) return rc;
         if (build_batch_graph(p, set, &p.gexec[set])) {
             (void)hipGetLastError();
@@ -647,12 +658,8 @@ int launch_corr_graph(Plan &p, long long r0, int set, Timers *tm)
         }
         return 0;
     }
-    // staged through a ring longer than the batches the host can run ahead
-    // (it finalizes batch i - nslots before launching batch i)
-    long long *h = p.h_r0 + (p.r0_ring++ % Plan::kR0Ring);
-    *h = r0;
-    PZL_PINNED_CHECK(h, sizeof(long long));
-    PZL_HIPCHK(hipMemcpyAsync(p.d_r0 + set, h, sizeof(long long), hipMemcpyHostToDevice, p.st));
+    int rc = launch_corr_part(p, r0, p.B, set, tm, nullptr, seg_ahead, true);
+    if (rc) return rc;
     Scope s(tm, 3, 0, p.st);
     PZL_HIPCHK(hipGraphLaunch(p.gexec[set], p.st));
     return 0;
@@ -667,7 +674,7 @@ int launch_batch(Plan &p, long long r0, int nb, int set, int slot, float *d_sums
                  bool graph = false, bool seg_ahead = false)
 {
     PZL_HIPCHK(hipStreamWaitEvent(p.st, p.ev_sum[set], 0));
-    const int rc = (graph && nb == p.B) ? launch_corr_graph(p, r0, set, tm)
+    const int rc = (graph && nb == p.B) ? launch_corr_graph(p, r0, set, tm, seg_ahead)
                                         : launch_corr_part(p, r0, nb, set, tm, nullptr, seg_ahead);
     if (rc) return rc;
     PZL_HIPCHK(hipEventRecord(p.ev_corr[set], p.st));
@@ -1164,7 +1171,6 @@ int pzl_plan_create(const PzlConfig *cfg, int device, void **plan_out)
         p->d_hpows[s] = (float *)carve();
     }
     p->d_counts = (unsigned long long *)carve();
-    p->d_r0 = (long long *)carve();
     p->d_skeys = (long long *)carve();
     p->d_spows = (float *)carve();
     p->d_recs = (PzlHostHit *)carve();
@@ -1207,7 +1213,6 @@ int pzl_plan_create(const PzlConfig *cfg, int device, void **plan_out)
         PCHK(hipEventRecord(p->ev_corr[k], p->st));
         PCHK(hipEventRecord(p->ev_sum[k], p->st));
     }
-    PCHK(pinned_alloc((void **)&p->h_r0, Plan::kR0Ring * sizeof(long long)));
     PCHK(hipMemset(p->d_counts, 0, 2 * Plan::kSlots * sizeof(unsigned long long)));
     PCHK(pinned_alloc((void **)&p->h_counts, 2 * Plan::kSlots * sizeof(unsigned long long)));
     // The kernel bank: built once per shape and kept resident.

@@ -134,13 +134,9 @@ struct Plan {
     static constexpr int kMaxGraphFailures = 8;
     int graph_failures = 0;  // batch graphs the runtime refused (those batches ran eagerly)
     hipGraphExec_t gexec[kSets] = {};  // correlation part, per buffer set
-    long long *d_r0 = nullptr;  // [kSets] first rstep of the batch in each set
-    static constexpr int kR0Ring = 2 * kSlots;
-    long long *h_r0 = nullptr;  // pinned staging ring of those values
     // pzl_plan_set_deadline: searches stop launching batches after this time
     bool has_deadline = false;
     std::chrono::steady_clock::time_point deadline{};
-    unsigned long long r0_ring = 0;
     hipStream_t sum_stream() const { return overlap_sum ? st3 : st; }
     int side_group = -1;     // >= 0: the shorter FFT lengths in one grouped launch (autotuned) ...
     bool group_all = false;  // ... and the longest length in it too: one correlation launch per batch
