@@ -579,6 +579,22 @@ def main():
     flops_per_launch = dom["flops"] / dom["launches"]
     tflops = flops_per_launch / (avg_ms * 1e-3) / 1e12
     traffic = measured_traffic(dom_name)
+    # the same kernel alone on the whole chip: a band of the search with every
+    # kernel in one stream (nothing runs beside the correlation), eager
+    alone = None
+    if dom_name == "k_corr_group" and g.nrsteps >= 2048:
+        plan.set_graph(False)
+        plan.set_overlap(0)
+        blo = g.nrsteps // 2 - 512
+        sta = plan.search(blo, blo + 1024, stats=True)[1]
+        plan.set_overlap(args.overlap if args.overlap >= 0 else 2)
+        plan.set_graph(args.graph != 0)
+        if sta["group_launches"] > 0:
+            ams = sta["ms_corr_group"] / sta["group_launches"]
+            alone = {"avg_launch_ms": ams, "achieved": bytes_per_launch / (ams * 1e-3) / 1e9,
+                     "frac": bytes_per_launch / (ams * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "timing": "rsteps [%d, %d) with every kernel in one stream, launched one by one "
+                               "(%d launches, HIP events on every 8th)" % (blo, blo + 1024, sta["group_launches"])}
     out = {
         "metric": "f-fdot-fdotdot cells/s at zmax200/wmax300/16 harm; p50/p99 latency per search",
         "value": value,
@@ -604,6 +620,7 @@ def main():
                      "traffic": traffic,
                      "avg_launch_ms": avg_ms, "alg_bytes_per_launch": bytes_per_launch,
                      "bank_bytes_reread_per_launch": dom["bank"],
+                     "alone": alone,
                      "traffic_over_alg": traffic / bytes_per_launch if traffic else None,
                      "binding": "FP32 vector issue (VALU), not HBM: see compute",
                      "compute": {"flops_per_launch": flops_per_launch, "achieved": tflops, "unit": "TFLOP/s",
