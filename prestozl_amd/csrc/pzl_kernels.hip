@@ -1261,7 +1261,7 @@ using CorrGroup256 =
 // hold many values per thread: every member stays within 128 VGPRs, so the
 // group keeps 4 waves per SIMD.
 using CorrGroup256Lean =
-    CorrGroup<256, 4, CorrV<2048, 256, 1, true, 0, 3, 8, 8, 8, 4>, CorrV<1280, 256, 1, true, 0, 4, 4, 8, 8, 5>,
+    CorrGroup<256, 4, CorrV<2048, 256, 1, true, 0, 5, 8, 8, 4, 8>, CorrV<1280, 256, 1, true, 0, 4, 4, 8, 8, 5>,
               CorrV<1024, 256, 2, true, 0, 5, 8, 8, 4, 4>, CorrV<768, 256, 1, true, 0, 4, 3, 4, 4, 4, 4>,
               CorrV<512, 256, 4, true, 0, 4, 8, 8, 8>, CorrV<384, 256, 4, true, 0, 4, 8, 8, 6>,
               CorrV<256, 256, 8, true, 0, 4, 8, 8, 4>, CorrV<192, 256, 8, true, 0, 4, 8, 4, 6>,
