@@ -220,6 +220,11 @@ int pzl_get_hits(void *plan, long long *keys, float *powers, long long maxhits, 
 /* How many hits a search keeps for pzl_get_hits (default 2^22; a search with
  * more keeps none and pzl_get_hits fails). */
 int pzl_plan_set_keep_hits(void *plan, long long limit);
+/* Hit records travel to the host in chunks of this many records (default
+ * 2^20), each in a pinned buffer that the plan keeps and reuses (2 GiB in
+ * all); results do not depend on it.  No counterpart in the reference (it
+ * copies each batch's hits in one piece). */
+int pzl_plan_set_hit_chunk(void *plan, long long records);
 
 /* Parity hooks: one FFT'd kernel (fftlen complex64) of harmonic idx, and the
  * power plane ([numw][numz][numrs]) of (rstep, harmonic) as

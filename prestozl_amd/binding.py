@@ -163,6 +163,7 @@ def lib():
         L.pzl_plan_set_graph.argtypes = [C.c_void_p, C.c_int]
         L.pzl_plan_set_overlap.argtypes = [C.c_void_p, C.c_int]
         L.pzl_plan_set_cu_split.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.pzl_plan_set_hit_chunk.argtypes = [C.c_void_p, C.c_longlong]
         L.pzl_plan_cu_split.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.pzl_tune_batchsize.argtypes = [P(Config), C.c_int, P(C.c_float), C.c_longlong, P(C.c_int), C.c_int,
                                          C.c_longlong, P(C.c_int), P(C.c_double)]
@@ -467,6 +468,10 @@ class Plan:
         out = [(cands[i].power, cands[i].sigma, cands[i].numharm, cands[i].r, cands[i].z, cands[i].w)
                for i in range(min(n.value, maxcands))]
         return out, (st.as_dict() if st is not None else None)
+
+    def set_hit_chunk(self, records):
+        """Records per pinned buffer on their way to the host (default 2^20)."""
+        check(lib().pzl_plan_set_hit_chunk(self.h, int(records)))
 
     def set_keep_hits(self, limit):
         """Hits kept by a search for hits() (default 2^22)."""

@@ -731,7 +731,7 @@ struct SearchState {
         cv.notify_all();
         if (th.joinable()) th.join();
     }
-    // Records travel to the insertion thread in chunks of kChunk records.
+    // Records travel to the insertion thread in chunks of p->hit_chunk records.
     // Every pool buffer holds one chunk; it is made once (from the
     // process-wide cache, else pinned) and reused, so that a plan stops
     // pinning or freeing host memory once its pool has grown: pinning waits
@@ -739,24 +739,40 @@ struct SearchState {
     // in the middle of a search leaves the GPU without work.  A free buffer,
     // else a new one while the pool stays within its limit, else wait for the
     // insertion thread to release one.
-    static constexpr long long kChunk = 1LL << 20;
     int acquire()
     {
+        const long long chunk = p->hit_chunk;
+        const long long chunk_bytes = (long long)sizeof(PzlHostHit) * chunk;
         std::unique_lock<std::mutex> lk(mu);
         for (;;) {
             if (err) return -1;
+            int nbusy = 0;
             for (int k = 0; k < (int)p->hpool.size(); k++) {
                 auto &hb = p->hpool[k];
-                if (!hb.busy && hb.ptr && hb.cap >= kChunk) {
+                if (hb.busy) {
+                    nbusy++;
+                } else if (hb.ptr && hb.cap >= chunk) {
                     hb.busy = true;
                     return k;
                 }
             }
-            if (p->hpool_bytes + (long long)sizeof(PzlHostHit) * kChunk > p->hpool_limit) {
-                const auto w0 = std::chrono::steady_clock::now();
-                cv.wait(lk);
-                ms_wait_pool += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
-                continue;
+            if (p->hpool_bytes + chunk_bytes > p->hpool_limit) {
+                if (nbusy > 0) {
+                    const auto w0 = std::chrono::steady_clock::now();
+                    cv.wait(lk);
+                    ms_wait_pool +=
+                        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
+                    continue;
+                }
+                // nothing in flight: the idle buffers are too small for the
+                // chunk (it was made larger); back to the process-wide cache
+                for (auto &hb : p->hpool)
+                    if (hb.ptr) {
+                        pin_put(hb.ptr, hb.cap);
+                        p->hpool_bytes -= (long long)sizeof(PzlHostHit) * hb.cap;
+                        hb.ptr = nullptr;
+                        hb.cap = 0;
+                    }
             }
             int k = -1;
             for (int q = 0; q < (int)p->hpool.size(); q++)
@@ -770,9 +786,9 @@ struct SearchState {
                 }
             }
             auto &hb = p->hpool[k];
-            long long cap = kChunk;
-            hb.ptr = pin_take(kChunk, &cap);
-            if (!hb.ptr && pinned_alloc((void **)&hb.ptr, sizeof(PzlHostHit) * kChunk) != hipSuccess) {
+            long long cap = chunk;
+            hb.ptr = pin_take(chunk, &cap);
+            if (!hb.ptr && pinned_alloc((void **)&hb.ptr, (size_t)chunk_bytes) != hipSuccess) {
                 hb.ptr = nullptr;
                 return -1;
             }
@@ -876,8 +892,8 @@ int apply_slot(Plan &p, int slot, long long n, SearchState &ss)
     PZL_HIPCHK(hipStreamWaitEvent(p.st5, p.ev_prep, 0));
     // the records in chunks of pool buffers, in order (the insertion thread
     // applies the jobs first in, first out)
-    for (long long off = 0; off < n; off += SearchState::kChunk) {
-        const long long m = std::min(SearchState::kChunk, n - off);
+    for (long long off = 0; off < n; off += p.hit_chunk) {
+        const long long m = std::min(p.hit_chunk, n - off);
         const int buf = ss.acquire();
         if (buf < 0)
             return fail(ss.err ? ss.err : PZL_ERR_HIP, "no pinned buffer for %lld hit records%s%s", m,
@@ -1999,6 +2015,17 @@ int pzl_get_cands(void *plan, PzlCand *cands, long long maxcands, long long *nca
     if (!p) return fail(PZL_ERR_ARG, "null plan");
     if (ncands) *ncands = (long long)p->last_cands.size();
     for (long long k = 0; k < (long long)p->last_cands.size() && k < maxcands; k++) cands[k] = p->last_cands[k];
+    return 0;
+}
+
+int pzl_plan_set_hit_chunk(void *plan, long long records)
+{
+    Plan *p = (Plan *)plan;
+    if (!p) return fail(PZL_ERR_ARG, "null plan");
+    if (records < 1 || records * (long long)sizeof(PzlHostHit) > p->hpool_limit)
+        return fail(PZL_ERR_ARG, "hit chunk of %lld records: 1 .. %lld", records,
+                    p->hpool_limit / (long long)sizeof(PzlHostHit));
+    p->hit_chunk = records;
     return 0;
 }
 

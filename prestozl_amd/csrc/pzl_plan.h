@@ -126,6 +126,7 @@ struct Plan {
     std::vector<HostBuf> hpool;
     long long hpool_bytes = 0;
     long long hpool_limit = 2LL << 30;
+    long long hit_chunk = 1LL << 20;  // records per pool buffer (pzl_plan_set_hit_chunk)
     hipEvent_t ev_count[kSlots] = {};
     hipEvent_t ev_free[kSlots] = {};
     float *d_sums = nullptr;

@@ -611,6 +611,30 @@ def test_cu_split_identical(config1_data, which):
     plan.close()
 
 
+def test_hit_chunks_identical(config1_data):
+    """Hit records travel to the host in chunks of pinned pool buffers: tiny
+    chunks (a batch's records split over many buffers, copies and insertion
+    jobs) give the default chunk's hits and candidates exactly; invalid sizes
+    are refused."""
+    fft, kw = _c1(config1_data)
+    plan = _plan(kw, fft)
+    c0, _ = plan.search(stats=False)
+    h0 = plan.hits()
+    assert len(h0[1]) > 1000
+    for chunk in (97, 4096, 1 << 20):
+        plan.set_hit_chunk(chunk)
+        c, _ = plan.search(stats=False)
+        h = plan.hits()
+        print("chunk %d: %d hits, %d candidates" % (chunk, len(h[1]), len(c)))
+        assert c == c0
+        assert np.array_equal(h[1], h0[1])
+        assert all(np.array_equal(a, b) for a, b in zip(h[0], h0[0]))
+    for bad in (0, -5, 1 << 40):
+        with pytest.raises(RuntimeError):
+            plan.set_hit_chunk(bad)
+    plan.close()
+
+
 def test_tune_batchsize(config1_data):
     """The rsteps-per-launch autotuner returns one of the candidates, and the
     search it would configure gives the same candidates as batch size 8."""
