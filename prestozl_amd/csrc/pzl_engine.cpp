@@ -1732,7 +1732,10 @@ int pzl_autotune(void *plan, int reps, char *report, int report_len)
         const long long lo = (nr - sband) / 2;
         int ncus = 0, nxcc = 1;
         PZL_HIPCHK(hipDeviceGetAttribute(&ncus, hipDeviceAttributeMultiprocessorCount, p->device));
-        PZL_HIPCHK(hipDeviceGetAttribute(&nxcc, hipDeviceAttributeNumberOfXccs, p->device));
+        if (hipDeviceGetAttribute(&nxcc, hipDeviceAttributeNumberOfXccs, p->device) != hipSuccess) {
+            (void)hipGetLastError();
+            nxcc = 1;
+        }
         const int u = 4 * std::max(nxcc, 1);  // one CU per shader engine
         struct Split {
             int sum, only, rpt;
@@ -1745,8 +1748,15 @@ int pzl_autotune(void *plan, int reps, char *report, int report_len)
         float base_ms = 1e30f, best_ms = 1e30f;  // the whole chip; the fastest split
         Split best = splits[0];
         for (const Split &sp : splits) {
-            rc = pzl_plan_set_cu_split(plan, sp.sum, sp.only);
-            if (rc) return rc;
+            if (pzl_plan_set_cu_split(plan, sp.sum, sp.only)) {
+                // CU masks refused (not a failure of the search): keep the whole chip
+                snprintf(line, sizeof line, "harmonic sums on %d CUs: not available (%s)\n", sp.sum,
+                         pzl::g_err.c_str());
+                rep += line;
+                (void)hipGetLastError();
+                pzl::g_err.clear();
+                break;
+            }
             p->sum_rpt = sp.rpt;
             float ms = 1e30f;
             for (int k = 0; k < 2; k++) {
@@ -1825,7 +1835,10 @@ int pzl_plan_set_cu_split(void *plan, int sum_cus, int sum_only_cus)
     PZL_HIPCHK(hipSetDevice(p->device));
     int ncus = 0, nxcc = 1;
     PZL_HIPCHK(hipDeviceGetAttribute(&ncus, hipDeviceAttributeMultiprocessorCount, p->device));
-    PZL_HIPCHK(hipDeviceGetAttribute(&nxcc, hipDeviceAttributeNumberOfXccs, p->device));
+    if (hipDeviceGetAttribute(&nxcc, hipDeviceAttributeNumberOfXccs, p->device) != hipSuccess) {
+        (void)hipGetLastError();
+        nxcc = 1;  // a runtime that does not report dies: one
+    }
     // every shader engine of every die keeps compute units in each mask (a
     // queue whose mask leaves a die without any never finishes that die's
     // share of a launch): the same count on each die, at least one per shader
