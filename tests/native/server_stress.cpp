@@ -5,6 +5,8 @@
 // "device" spectra, with plan eviction (a budget below the mix), deadlines
 // that pass in the queue and while running, and repeated rounds so that
 // pinned hit buffers move through the process-wide cache between plans.
+// Before that, one plan per shape searches again under CU splits, hit-record
+// chunk sizes and graph settings changed between its searches.
 // Exit status 0 when every check holds.  Test infrastructure only.
 #include <hip/hip_runtime.h>
 
@@ -88,6 +90,39 @@ int main(int argc, char **argv)
         printf("shape %zu: %zu candidates alone\n", ref.size() - 1, ref.back().size());
     }
     int bad = 0;
+    // plan options between searches of one plan: CU splits (masked streams
+    // made once per split and kept), hit-record chunks from 16 records up
+    // (the pool's undersized buffers go back to the cache), graphs on and off
+    struct Opt {
+        int sum, only;
+        long long chunk;
+        int graph;
+    };
+    const Opt opts[] = {{96, 0, 16, 1}, {64, 64, 1 << 20, 1}, {96, 0, 64, 0}, {0, 0, 16, 1}, {96, 0, 1 << 20, 1}};
+    for (size_t s = 0; s < shapes.size(); s++) {
+        void *plan = nullptr;
+        if (pzl_plan_create(&shapes[s], 0, &plan) || pzl_set_spectrum(plan, spectra[s].data(), shapes[s].numbins)) {
+            fprintf(stderr, "plan: %s\n", pzl_last_error());
+            return 2;
+        }
+        for (const Opt &o : opts) {
+            long long n = 0;
+            if (pzl_plan_set_cu_split(plan, o.sum, o.only) || pzl_plan_set_hit_chunk(plan, o.chunk) ||
+                pzl_plan_set_graph(plan, o.graph) || pzl_search(plan, 0, -1, nullptr, 0, &n, nullptr)) {
+                fprintf(stderr, "options: %s\n", pzl_last_error());
+                return 2;
+            }
+            std::vector<PzlCand> v((size_t)n);
+            pzl_get_cands(plan, v.data(), n, &n);
+            if (!same(v, ref[s])) {
+                fprintf(stderr, "shape %zu split %d/%d chunk %lld graph %d: %lld candidates vs %zu alone\n", s, o.sum,
+                        o.only, o.chunk, o.graph, n, ref[s].size());
+                bad++;
+            }
+        }
+        pzl_plan_destroy(plan);
+    }
+    printf("plan options: %s\n", bad ? "mismatch" : "identical");
     for (int round = 0; round < rounds; round++) {
         // odd rounds: room for the largest request and a little (plans are
         // evicted and rebuilt); even rounds: room for three at once

@@ -1,6 +1,7 @@
 """The host logic of the search driver and the request loop -- plans, batches
 in flight, hit slots, the pinned-buffer pool and its process-wide cache, the
-insertion thread, plan eviction, deadlines -- built for the CPU against a
+insertion thread, plan eviction, deadlines, CU splits, hit-record chunk
+sizes and graph settings changed between one plan's searches -- built for the CPU against a
 stand-in of the HIP runtime whose streams run asynchronously on threads of
 their own (tests/native/hipstub), under AddressSanitizer and under
 ThreadSanitizer, with three requests in flight.  Every served request must
