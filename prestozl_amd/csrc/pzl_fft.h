@@ -285,11 +285,11 @@ template <int R, int SIGN, typename C> PZL_FFT_HD void dftR(C *v)
 // its butterflies' inputs (twiddled by exp(SIGN 2 pi i jm r / (Ns R)), from
 // sincospi in double) into registers, the block synchronises, then writes them
 // in place.  R and the butterflies per thread are compile-time, so the
-// register arrays need no scratch.
-template <int SIGN, int R, int NT>
+// register arrays need no scratch; they are sized for n <= NMAX.
+template <int SIGN, int R, int NT, int NMAX>
 __device__ __forceinline__ void lds_fft_pass_f64(double2 *buf, int n, int Ns, int tid)
 {
-    constexpr int MAXB = (4096 / R + NT - 1) / NT;  // butterflies per thread, n <= 4096
+    constexpr int MAXB = (NMAX / R + NT - 1) / NT;  // butterflies per thread
     const int nbf = n / R;
     double2 v[MAXB][R];
 #pragma unroll
@@ -328,26 +328,28 @@ __device__ __forceinline__ void lds_fft_pass_f64(double2 *buf, int n, int Ns, in
 }
 
 // Block-cooperative double-precision Stockham FFT of n points held in LDS
-// (buf, n <= 4096), any n = 2^a 3^b 5^c, by NT threads: passes of radix 4
-// while 4 divides what is left, then 2, 3, 5.  Used by the kernel-bank build
-// and the segment transforms.
-template <int SIGN, int NT>
+// (buf, n <= NMAX <= 4096), any n = 2^a 3^b 5^c, by NT threads: passes of
+// radix 4 while 4 divides what is left, then 2, 3, 5.  Used by the
+// kernel-bank build and the segment transforms; a smaller NMAX holds fewer
+// butterflies per thread in registers.
+template <int SIGN, int NT, int NMAX = 4096>
 __device__ __forceinline__ void lds_fft_f64(double2 *buf, int n, int tid)
 {
+    static_assert(NMAX <= 4096, "LDS transforms hold at most 4096 points");
     int Ns = 1;
     while (Ns < n) {
         const int rem = n / Ns;
         if (rem % 4 == 0) {
-            lds_fft_pass_f64<SIGN, 4, NT>(buf, n, Ns, tid);
+            lds_fft_pass_f64<SIGN, 4, NT, NMAX>(buf, n, Ns, tid);
             Ns *= 4;
         } else if (rem % 2 == 0) {
-            lds_fft_pass_f64<SIGN, 2, NT>(buf, n, Ns, tid);
+            lds_fft_pass_f64<SIGN, 2, NT, NMAX>(buf, n, Ns, tid);
             Ns *= 2;
         } else if (rem % 3 == 0) {
-            lds_fft_pass_f64<SIGN, 3, NT>(buf, n, Ns, tid);
+            lds_fft_pass_f64<SIGN, 3, NT, NMAX>(buf, n, Ns, tid);
             Ns *= 3;
         } else {
-            lds_fft_pass_f64<SIGN, 5, NT>(buf, n, Ns, tid);
+            lds_fft_pass_f64<SIGN, 5, NT, NMAX>(buf, n, Ns, tid);
             Ns *= 5;
         }
     }

@@ -93,6 +93,7 @@ __global__ __launch_bounds__(BANK_NT) void k_bank_build(const PzlHarm *__restric
 // ---------------------------------------------------------------------------
 
 #define SEG_NT 256
+#define SEG_MAXDATA 2048  // segment points: half the longest FFT length (4096)
 
 // The element of rank k (0-based, ascending) of n non-negative floats in LDS,
 // by the SEG_NT threads of a workgroup: a radix select on their bit patterns
@@ -187,8 +188,9 @@ __global__ __launch_bounds__(SEG_NT) void k_segments(PzlBatchArgs a)
         sbuf[i] = double2{(double)(float)(x.x * norm), (double)(float)(x.y * norm)};
     }
     __syncthreads();
-    // FFT of the fftlen-point spread array == numdata-point FFT, repeated.
-    lds_fft_f64<-1, SEG_NT>(sbuf, numdata, tid);
+    // FFT of the fftlen-point spread array == numdata-point FFT, repeated
+    // (numdata = fftlen / 2 <= 2048)
+    lds_fft_f64<-1, SEG_NT, SEG_MAXDATA>(sbuf, numdata, tid);
     float2 *seg = a.segs + (long long)b * a.seg_stride + H.seg_off;
     for (int k = tid; k < numdata; k += SEG_NT) seg[k] = float2{(float)sbuf[k].x, (float)sbuf[k].y};
     if (h > 0) {
@@ -1137,6 +1139,7 @@ extern "C" hipError_t pzl_launch_bank_build(const PzlHarm *d_harms, int nh, int 
 extern "C" hipError_t pzl_launch_segments(const PzlBatchArgs *a, int max_fftlen, hipStream_t st)
 {
     const int numdata = max_fftlen / 2;
+    if (numdata > SEG_MAXDATA) return hipErrorInvalidValue;  // the transform's registers hold no more
     int n2 = 1;
     while (n2 < numdata) n2 <<= 1;
     const size_t smem = (size_t)numdata * sizeof(double2) + (size_t)n2 * sizeof(float);
