@@ -673,7 +673,9 @@ int launch_corr_graph(Plan &p, long long r0, int set, Timers *tm, bool seg_ahead
 int launch_batch(Plan &p, long long r0, int nb, int set, int slot, float *d_sums, Timers *tm, bool with_sum = true,
                  bool graph = false, bool seg_ahead = false)
 {
-    PZL_HIPCHK(hipStreamWaitEvent(p.st, p.ev_sum[set], 0));
+    // with the segments ahead, the correlation stream waits for them, and
+    // they for the set's last sum (the wait below would repeat theirs)
+    if (!seg_ahead) PZL_HIPCHK(hipStreamWaitEvent(p.st, p.ev_sum[set], 0));
     const int rc = (graph && nb == p.B) ? launch_corr_graph(p, r0, set, tm, seg_ahead)
                                         : launch_corr_part(p, r0, nb, set, tm, nullptr, seg_ahead);
     if (rc) return rc;
@@ -1218,12 +1220,16 @@ int pzl_plan_create(const PzlConfig *cfg, int device, void **plan_out)
                 }
         PCHK(hipMemcpy(p->d_zoff, zoff.data(), sizeof(int) * zoff.size(), hipMemcpyHostToDevice));
     }
+    // The buffer-set events order the plan's streams on this device and
+    // nothing on the host waits for them: their records release to device
+    // scope, without the system-scope fence of a host-visible event.
+    constexpr unsigned kDevEv = hipEventDisableTiming | hipEventDisableSystemFence;
     for (int k = 0; k < Plan::kSets; k++) {
-        PCHK(hipEventCreateWithFlags(&p->ev_corr[k], hipEventDisableTiming));
-        PCHK(hipEventCreateWithFlags(&p->ev_sum[k], hipEventDisableTiming));
-        PCHK(hipEventCreateWithFlags(&p->ev_seg[k], hipEventDisableTiming));
+        PCHK(hipEventCreateWithFlags(&p->ev_corr[k], kDevEv));
+        PCHK(hipEventCreateWithFlags(&p->ev_sum[k], kDevEv));
+        PCHK(hipEventCreateWithFlags(&p->ev_seg[k], kDevEv));
         PCHK(hipEventRecord(p->ev_seg[k], p->st));
-        PCHK(hipEventCreateWithFlags(&p->ev_side[k], hipEventDisableTiming));
+        PCHK(hipEventCreateWithFlags(&p->ev_side[k], kDevEv));
         PCHK(hipEventRecord(p->ev_side[k], p->st));
         // recorded once so that the first waits on them are satisfied
         PCHK(hipEventRecord(p->ev_corr[k], p->st));

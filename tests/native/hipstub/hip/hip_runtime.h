@@ -59,6 +59,7 @@ enum hipStreamCaptureMode {
 };
 #define hipStreamNonBlocking 1u
 #define hipEventDisableTiming 2u
+#define hipEventDisableSystemFence 0x20000000u
 #define hipHostMallocDefault 0u
 
 const char *hipGetErrorString(hipError_t e);
