@@ -69,6 +69,21 @@ def test_geometry_equals_oracle(cfg):
     assert list(g.powcut)[: g.numharmstages] == list(obs.powcut)[: obs.numharmstages]
 
 
+def test_correlation_lengths_fit_the_segment_transform():
+    """Every correlation length the library supports is at most 4096 points:
+    a segment holds half of it, and the segment transform's registers are
+    sized for 2048 points.  The fundamental length of every configured shape
+    is supported (including config 3's 4096)."""
+    L = B.lib()
+    L.pzl_corr_supported.argtypes = [C.c_int]
+    L.pzl_corr_supported.restype = C.c_int
+    supported = [n for n in range(1, 16385) if L.pzl_corr_supported(n)]
+    assert supported and max(supported) == 4096, supported[-5:]
+    for N, dt, zmax, wmax, nh, sig, wmaxP in CONFIGS:
+        g = B.geometry(B.make_config(N, dt, N // 2, zmax, wmax, nh, sig, wmaxP=wmaxP))
+        assert L.pzl_corr_supported(g.fftlen), g.fftlen
+
+
 def _insert_both(rr, sig, pw, nh, z, w):
     L = B.lib()
     n = len(rr)
